@@ -1,0 +1,171 @@
+/*
+ * tigerpm_hip.h -- C ABI of the MI355X TreePM kick library (libtigerpm_hip.so).
+ *
+ * The reference (dmarce1/tigerpm) drives the kick through module-global C++
+ * functions over global particle arrays.  This ABI keeps the same steps and
+ * argument meanings but hands state over explicitly: a context owns the
+ * device-resident particles, chain mesh, PM mesh and scratch, and every call
+ * returns a status instead of printing (the reference's CUDA_CHECK prints and
+ * continues, cuda.hpp:14).  Plain pointers and sizes only.
+ *
+ * Position layout is the reference's fixed32 SoA (particles.hpp:17-25,
+ * fixed.hpp:63-129): x = raw * 2^-32 in the unit periodic box.
+ * Calls on one context are serialised on the context's HIP stream; different
+ * contexts may be used from different host threads.
+ */
+#ifndef TIGERPM_HIP_H_
+#define TIGERPM_HIP_H_
+
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define TPM_API __attribute__((visibility("default")))
+#else
+#define TPM_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tpm_ctx tpm_ctx;
+typedef int tpm_status;
+
+#define TPM_OK 0
+#define TPM_ERR_HIP (-1)      /* a HIP runtime call failed (message in tpm_last_error) */
+#define TPM_ERR_INVALID (-2)  /* invalid argument or parameter combination */
+#define TPM_ERR_STATE (-3)    /* call out of order (e.g. kick before chain mesh / PM) */
+#define TPM_ERR_FFT (-4)      /* rocFFT failure */
+#define TPM_ERR_BUDGET (-5)   /* device memory budget exceeded */
+
+/* Mass-assignment orders. */
+#define TPM_CIC 2
+#define TPM_TSC 3
+#define TPM_CLOUD4 4
+
+/* Kick flags (bit set = stage skipped); 0 = the reference's full kick. */
+#define TPM_SKIP_PP 1u   /* no short-range particle-particle term */
+#define TPM_SKIP_PM 2u   /* no mesh interpolation term */
+#define TPM_SKIP_KICK 4u /* store g / phi only: velocities and rungs untouched */
+
+/* Physical and numerical parameters.  Replaces the options struct
+ * (options.hpp:5-24, derived in options.cpp:81-96) and the scalars that
+ * kick_treepm copies into treepm_params (kick_treepm.cu:833-853). */
+typedef struct {
+    int mesh_dim;  /* M, PM mesh cells per side (reference four_dim) */
+    int chain_dim; /* C, chain-mesh cells per side; 0 = floor(1/rcut) */
+    int order;     /* TPM_CIC / TPM_TSC / TPM_CLOUD4 */
+    int do_phi;    /* accumulate the potential (reference do_phi) */
+    double rs;     /* split scale in box units (reference rs = 1/four_dim) */
+    double rcut;   /* short-range cutoff in box units (reference 1/chain_dim) */
+    double hsoft;  /* softening length (reference 2/(25 parts_dim)) */
+    double GM;     /* multiplies g and phi; 1/N for masses m = 1/N */
+    double eta;    /* time-step parameter (reference 0.2/sqrt(2)) */
+    double n_total;/* unit masses in the box (reference parts_dim^3) */
+} tpm_params;
+
+/* Per-call statistics (the reference prints similar diagnostics,
+ * kick_treepm.cu:1084-1087).  Times are HIP-event durations on the context
+ * stream, milliseconds. */
+typedef struct {
+    double candidate_pairs;  /* W_sr = sum_c n_c * sum_{27 neighbours} n_c' */
+    double executed_pairs;   /* pairs evaluated after bounding-box pruning */
+    double ms_chainmesh;
+    double ms_pm;
+    double ms_pp;            /* the fused PP + interpolation + kick kernel */
+    double ms_total;
+    int64_t n_active;
+    int64_t n_tiles;
+    int max_rung;
+    int pad_;
+} tpm_stats;
+
+/* Context lifetime.  device = HIP device ordinal. */
+TPM_API tpm_status tpm_ctx_create(int device, tpm_ctx** ctx);
+TPM_API tpm_status tpm_ctx_destroy(tpm_ctx* ctx);
+TPM_API const char* tpm_last_error(const tpm_ctx* ctx);
+TPM_API tpm_status tpm_sync(tpm_ctx* ctx);
+
+/* Parameters; validates and derives chain_dim.  Replaces get_options(). */
+TPM_API tpm_status tpm_set_params(tpm_ctx* ctx, const tpm_params* params);
+TPM_API tpm_status tpm_get_params(const tpm_ctx* ctx, tpm_params* params);
+
+/* Particle store (particles.hpp:17-25 SoA).  Host pointers; the copy is
+ * synchronous.  vx/vy/vz/rung may be NULL (zeros). */
+TPM_API tpm_status tpm_particles_upload(tpm_ctx* ctx, int64_t n, const uint32_t* x, const uint32_t* y,
+                                const uint32_t* z, const float* vx, const float* vy, const float* vz,
+                                const int8_t* rung);
+/* Any output pointer may be NULL. */
+TPM_API tpm_status tpm_particles_download(tpm_ctx* ctx, uint32_t* x, uint32_t* y, uint32_t* z, float* vx, float* vy,
+                                  float* vz, int8_t* rung, float* gx, float* gy, float* gz, float* pot);
+TPM_API int64_t tpm_particles_size(const tpm_ctx* ctx);
+
+/* chainmesh_create (chainmesh.cpp:139-166, sort :168-213): reorders the
+ * resident particles into chain-cell order (cells z-major, the reference's
+ * partition order; in-cell order by 4x4x4 sub-cell Morton code then previous
+ * order) and builds the cell table. */
+TPM_API tpm_status tpm_chainmesh_create(tpm_ctx* ctx);
+/* chainmesh_get (chainmesh.cpp:48-51): cell (ix, iy, iz), periodic. */
+TPM_API tpm_status tpm_chainmesh_get(tpm_ctx* ctx, int ix, int iy, int iz, int64_t* pbegin, int64_t* pend);
+/* All C^3 + 1 cell offsets, cell id = (iz*C + iy)*C + ix. */
+TPM_API tpm_status tpm_chainmesh_cells(tpm_ctx* ctx, int64_t* cell_begin);
+/* perm[new] = index before the last tpm_chainmesh_create. */
+TPM_API tpm_status tpm_chainmesh_perm(tpm_ctx* ctx, int64_t* perm);
+
+/* gravity_long_compute (gravity_long.cpp:36-48) with the PME filter
+ * (type GRAVITY_LONG_PME): int64 fixed-point deposit, R2C FFT, Green's
+ * function, C2R FFT; phi stays on the device. */
+TPM_API tpm_status tpm_gravity_long_compute(tpm_ctx* ctx);
+/* gravity_long_get_phi (gravity_long.cpp:19-34): the whole periodic mesh,
+ * [x][y][z] z fastest, M^3 floats. */
+TPM_API tpm_status tpm_gravity_long_get_phi(tpm_ctx* ctx, float* phi);
+/* The integer density grid of the last deposit (M^3, Q32 unit masses). */
+TPM_API tpm_status tpm_gravity_long_get_density(tpm_ctx* ctx, uint64_t* grid);
+/* The FP32 Green's table in use, M*M*(M/2+1) floats. */
+TPM_API tpm_status tpm_gravity_long_get_green(tpm_ctx* ctx, float* table);
+
+/* kick_treepm / kick_pme kernel step (kick_treepm.cu:460-759,
+ * kick_pme.cu:128-357): per active sink (rung >= min_rung) the 6-point mesh
+ * force, the short-range PP sum over the 27 neighbour chain cells and the
+ * kick epilogue.  Needs tpm_chainmesh_create and tpm_gravity_long_compute.
+ * stats may be NULL. */
+TPM_API tpm_status tpm_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
+                    tpm_stats* stats);
+
+/* One whole step: chain mesh + PM + kick (kick_treepm_test without IC and
+ * comparison, test.cpp:237-290). */
+TPM_API tpm_status tpm_kick_treepm(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call,
+                           tpm_stats* stats);
+
+/* Launch timing log (the reference prints wall-clock timers per phase,
+ * kick_treepm.cu:826-1087).  enable != 0 clears the log and records HIP
+ * events on the context stream around every later chain-mesh / PM / PP stage
+ * of tpm_kick_treepm and around the PP kernel of tpm_kick; enable == 0 stops.
+ * tpm_profile_read waits for the recorded work and returns sums in ms:
+ * ms[0] chain mesh, ms[1] PM, ms[2] PP kernel, ms[3] whole steps, and the
+ * number of PP launches and of whole steps recorded. */
+TPM_API tpm_status tpm_profile(tpm_ctx* ctx, int enable);
+TPM_API tpm_status tpm_profile_read(tpm_ctx* ctx, double* ms, int64_t* pp_launches, int64_t* steps);
+
+/* FP64 Ewald judge (gravity_short_ewald_call_kernel, gravity_short.cu:10-91,
+ * reformulated): for each sample (an index into the current particle
+ * order) the periodic potential and force of all resident unit masses,
+ * softened inside hsoft, excluding the sample itself, plus its own image /
+ * background term.  alpha <= 0 picks the split automatically. */
+TPM_API tpm_status tpm_ewald(tpm_ctx* ctx, int64_t nsamples, const int64_t* samples, double alpha, double* phi,
+                     double* gx, double* gy, double* gz);
+
+/* Zel'dovich initial conditions on the device: n_side^3 particles on the
+ * lattice (i+1/2)/n_side displaced by a Gaussian field with power
+ * P(k) ~ k^spectral_index exp(-(k/k_Nyq)^2), scaled so that the rms 3-D
+ * displacement is rms_disp lattice cells.  Velocities and rungs zero.
+ * Replaces initialize() (initialize.cpp:91-221) for the build's workloads. */
+TPM_API tpm_status tpm_ic_zeldovich(tpm_ctx* ctx, int n_side, uint64_t seed, double rms_disp, double spectral_index);
+/* n uniform random positions from a counter-based RNG (seeded). */
+TPM_API tpm_status tpm_ic_uniform(tpm_ctx* ctx, int64_t n, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TIGERPM_HIP_H_ */

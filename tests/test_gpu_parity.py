@@ -1,0 +1,186 @@
+"""GPU parity: every stage of the HIP kick against the CPU oracle.
+
+Integer stages (chain mesh, deposit) must be bit-exact; FP32 stages are
+compared as relative rms over particles with the tolerance written in each
+test.  All calls go through the C ABI (tigerpm_amd.Context).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import tigerpm_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(((a - b) ** 2).sum() / max((b ** 2).sum(), 1e-300)))
+
+
+def uniform(n, seed):
+    rng = np.random.default_rng(seed)
+    return [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+
+
+def clustered(n, seed, nclump=20):
+    """Uniform background plus tight Gaussian clumps (uneven cell occupancy)."""
+    rng = np.random.default_rng(seed)
+    m = n // 2
+    pos = [rng.random(n - m) for _ in range(3)]
+    centres = rng.random((nclump, 3))
+    which = rng.integers(0, nclump, m)
+    for d in range(3):
+        c = (centres[which, d] + 0.01 * rng.standard_normal(m)) % 1.0
+        pos[d] = np.concatenate([pos[d], c])
+    return [np.minimum(np.floor(p * 2.0 ** 32), 2.0 ** 32 - 1).astype(np.uint32) for p in pos]
+
+
+def oparams(p):
+    return O.make_params(p.mesh_dim, p.chain_dim, p.order, p.rs, p.rcut, p.hsoft, p.GM, p.eta, p.n_total, p.do_phi)
+
+
+@pytest.mark.parametrize("dist,n_side,C", [("uniform", 24, 5), ("clustered", 32, 7), ("uniform", 16, 3)])
+def test_chainmesh_bit_exact(gpu_ctx, dist, n_side, C):
+    n = n_side ** 3
+    x, y, z = uniform(n, 11) if dist == "uniform" else clustered(n, 12)
+    p = T.baseline_params(n_side, n_side, 3)
+    p.chain_dim = C
+    p.rcut = min(p.rcut, 0.99 / C)
+    p = gpu_ctx.set_params(p)
+    gpu_ctx.upload(x, y, z)
+    gpu_ctx.chainmesh_create()
+    perm = gpu_ctx.chainmesh_perm()
+    cb = gpu_ctx.chainmesh_cells()
+    operm, ocb = O.chainmesh_sort(x, y, z, C)
+    assert np.array_equal(perm, operm)
+    assert np.array_equal(cb, ocb)
+    d = gpu_ctx.download()
+    assert np.array_equal(d["x"], x[operm]) and np.array_equal(d["z"], z[operm])
+    # the reference's recursive partition gives the same cell ranges
+    _, rb, re_ = O.chainmesh_partition_ref(x, y, z, C)
+    assert np.array_equal(rb, ocb[:-1]) and np.array_equal(re_, ocb[1:])
+    b, e = gpu_ctx.chainmesh_get(1, 2, 0)
+    cid = (0 * C + 2) * C + 1
+    assert (b, e) == (ocb[cid], ocb[cid + 1])
+
+
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_deposit_bit_exact(gpu_ctx, order):
+    n_side, M = 24, 20
+    x, y, z = clustered(n_side ** 3, 21 + order)
+    p = T.baseline_params(n_side, M, order)
+    p = gpu_ctx.set_params(p)
+    gpu_ctx.upload(x, y, z)
+    gpu_ctx.gravity_long_compute()
+    g = gpu_ctx.get_density()
+    og = O.deposit(x, y, z, M, order)
+    assert np.array_equal(g, og)
+    assert int(g.sum(dtype=np.uint64)) == len(x) << 32
+
+
+@pytest.mark.parametrize("M,order", [(32, 2), (24, 3), (30, 4)])
+def test_green_and_phi(gpu_ctx, M, order):
+    n_side = 24
+    x, y, z = uniform(n_side ** 3, 31)
+    p = gpu_ctx.set_params(T.baseline_params(n_side, M, order))
+    gpu_ctx.upload(x, y, z)
+    gpu_ctx.gravity_long_compute()
+    tab = gpu_ctx.get_green()
+    otab = O.green_table(M, p.rs, order)
+    # FP64 build on both sides, one float rounding: allow 1 ulp
+    ulp = np.abs(tab.view(np.int32).astype(np.int64) - otab.view(np.int32).astype(np.int64))
+    assert ulp.max() <= 1
+    phi = gpu_ctx.get_phi()
+    rho = O.density_float(O.deposit(x, y, z, M, order))
+    ophi = O.pm_phi(M, rho, otab)
+    assert rel_rms(phi, ophi) < 2e-6
+
+
+@pytest.mark.parametrize("dist", ["uniform", "clustered"])
+def test_pp_only_vs_oracle(gpu_ctx, dist):
+    n_side = 32
+    n = n_side ** 3
+    x, y, z = uniform(n, 41) if dist == "uniform" else clustered(n, 42)
+    p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, 3))
+    gpu_ctx.upload(x, y, z)
+    gpu_ctx.chainmesh_create()
+    st = gpu_ctx.kick(flags=T.TPM_SKIP_PM | T.TPM_SKIP_KICK)
+    d = gpu_ctx.download()
+    operm, ocb = O.chainmesh_sort(x, y, z, p.chain_dim)
+    xs, ys, zs = x[operm], y[operm], z[operm]
+    op = oparams(p)
+    (gx, gy, gz, pot), cand = O.pp_sum(op, xs, ys, zs, ocb, np.arange(n), fp64=False)
+    assert st.candidate_pairs == cand
+    assert st.executed_pairs <= cand
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1)
+    # GPU stores GM * (sum) and GM * (phi0 + sum)
+    OG = np.stack([gx, gy, gz], 1) * p.GM
+    assert rel_rms(G, OG) < 1e-5
+    phi0 = p.n_total * 4 * np.pi * p.rs ** 2 + (35.0 / 16.0) / p.hsoft
+    assert rel_rms(d["pot"], (pot + phi0) * p.GM) < 1e-5
+    # and against the FP64 truth of the same pair formula
+    (hx, hy, hz, hp), _ = O.pp_sum(op, xs, ys, zs, ocb, np.arange(n), fp64=True)
+    assert rel_rms(G, np.stack([hx, hy, hz], 1) * p.GM) < 2e-5
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_full_kick_vs_oracle(gpu_ctx, order):
+    n_side = 32
+    n = n_side ** 3
+    x, y, z = clustered(n, 51)
+    rng = np.random.default_rng(5)
+    vx, vy, vz = [rng.standard_normal(n).astype(np.float32) * 1e-3 for _ in range(3)]
+    rung = rng.integers(0, 3, n).astype(np.int8)
+    p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, order))
+    gpu_ctx.upload(x, y, z, vx, vy, vz, rung)
+    gpu_ctx.kick_treepm(min_rung=1, scale=1.0, t0=0.5, first_call=False)
+    d = gpu_ctx.download()
+    r = O.kick_full(oparams(p), x, y, z, vx, vy, vz, rung, min_rung=1, scale=1.0, t0=0.5, first_call=False)
+    for k in ("x", "y", "z"):
+        assert np.array_equal(d[k], r[k])
+    operm, _ = O.chainmesh_sort(x, y, z, p.chain_dim)
+    act = rung[operm] >= 1
+    assert 0 < act.sum() < n
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1)[act]
+    OG = np.stack([r["gx"], r["gy"], r["gz"]], 1)[act]
+    assert rel_rms(G, OG) < 1e-5
+    assert rel_rms(d["pot"][act], r["pot"][act]) < 1e-5
+    V = np.stack([d["vx"], d["vy"], d["vz"]], 1)
+    OV = np.stack([r["vx"], r["vy"], r["vz"]], 1)
+    assert np.array_equal(V[~act], OV[~act])  # inactive sinks untouched
+    assert rel_rms(V[act], OV[act]) < 1e-5
+    # rungs identical except where log2 of the time step sits at an integer
+    mism = d["rung"] != r["rung"]
+    assert mism.mean() < 1e-3, mism.mean()
+    assert np.array_equal(d["rung"][~act], r["rung"][~act])
+
+
+def test_ewald_gpu_vs_oracle(gpu_ctx):
+    n = 4096
+    x, y, z = clustered(n, 61)
+    p = gpu_ctx.set_params(T.baseline_params(16, 32, 3))
+    gpu_ctx.upload(x, y, z)
+    s = np.arange(0, n, 97)
+    ph, gx, gy, gz = gpu_ctx.ewald(s)
+    oph, ogx, ogy, ogz = O.ewald_sf(x, y, z, s, p.hsoft, 9.6)
+    assert rel_rms(np.stack([gx, gy, gz], 1), np.stack([ogx, ogy, ogz], 1)) < 1e-10
+    assert rel_rms(ph, oph) < 1e-10
+
+
+@pytest.mark.parametrize("order,tol", [(3, 1e-3), (4, 1e-3)])
+def test_kick_vs_ewald(gpu_ctx, order, tol):
+    n_side = 32
+    n = n_side ** 3
+    x, y, z = uniform(n, 71)
+    p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, order))
+    gpu_ctx.upload(x, y, z)
+    gpu_ctx.kick_treepm()
+    d = gpu_ctx.download()
+    rng = np.random.default_rng(3)
+    s = np.sort(rng.choice(n, 512, replace=False))
+    ph, gx, gy, gz = gpu_ctx.ewald(s)
+    G = np.stack([d["gx"][s], d["gy"][s], d["gz"][s]], 1)
+    E = np.stack([gx, gy, gz], 1) * p.GM
+    assert rel_rms(G, E) < tol

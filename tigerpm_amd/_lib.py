@@ -1,0 +1,262 @@
+"""ctypes binding of libtigerpm_hip.so (C ABI: include/tigerpm_hip.h)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ROOT = os.path.dirname(_HERE)
+_LIB = os.path.join(_HERE, "libtigerpm_hip.so")
+_HEADER = os.path.join(_ROOT, "include", "tigerpm_hip.h")
+
+TPM_CIC, TPM_TSC, TPM_CLOUD4 = 2, 3, 4
+TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK = 1, 2, 4
+
+_STATUS = {0: "ok", -1: "HIP error", -2: "invalid argument", -3: "call out of order", -4: "rocFFT error",
+           -5: "device memory budget exceeded"}
+
+
+class TpmError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("tigerpm_hip: %s (%d): %s" % (_STATUS.get(status, "error"), status, msg))
+        self.status = status
+
+
+class TpmParams(ctypes.Structure):
+    _fields_ = [("mesh_dim", ctypes.c_int), ("chain_dim", ctypes.c_int), ("order", ctypes.c_int),
+                ("do_phi", ctypes.c_int), ("rs", ctypes.c_double), ("rcut", ctypes.c_double),
+                ("hsoft", ctypes.c_double), ("GM", ctypes.c_double), ("eta", ctypes.c_double),
+                ("n_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TpmStats(ctypes.Structure):
+    _fields_ = [("candidate_pairs", ctypes.c_double), ("executed_pairs", ctypes.c_double),
+                ("ms_chainmesh", ctypes.c_double), ("ms_pm", ctypes.c_double), ("ms_pp", ctypes.c_double),
+                ("ms_total", ctypes.c_double), ("n_active", ctypes.c_int64), ("n_tiles", ctypes.c_int64),
+                ("max_rung", ctypes.c_int), ("pad_", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+
+
+_lib = None
+_P = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def lib_path():
+    return _LIB
+
+
+def _declared_symbols():
+    txt = open(_HEADER).read()
+    return sorted(set(re.findall(r"^\s*TPM_API\s+(?:tpm_status|const char\*|int64_t)\s+(tpm_\w+)\s*\(", txt, re.M)))
+
+
+DECLARED_SYMBOLS = _declared_symbols()
+
+
+def exported_symbols():
+    """Names exported by the built library (via nm -D)."""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", _LIB], capture_output=True, text=True, check=True).stdout
+    return sorted({ln.split()[-1] for ln in out.splitlines() if " T " in ln and ln.split()[-1].startswith("tpm_")})
+
+
+def lib():
+    """Load libtigerpm_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB):
+        raise TpmError(-3, "libtigerpm_hip.so not built (run python -m tigerpm_amd.build)")
+    L = ctypes.CDLL(_LIB)
+    vp = ctypes.c_void_p
+    sig = {
+        "tpm_ctx_create": (_int, [_int, ctypes.POINTER(vp)]),
+        "tpm_ctx_destroy": (_int, [vp]),
+        "tpm_last_error": (ctypes.c_char_p, [vp]),
+        "tpm_sync": (_int, [vp]),
+        "tpm_set_params": (_int, [vp, ctypes.POINTER(TpmParams)]),
+        "tpm_get_params": (_int, [vp, ctypes.POINTER(TpmParams)]),
+        "tpm_particles_upload": (_int, [vp, _i64, vp, vp, vp, vp, vp, vp, vp]),
+        "tpm_particles_download": (_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "tpm_particles_size": (_i64, [vp]),
+        "tpm_chainmesh_create": (_int, [vp]),
+        "tpm_chainmesh_get": (_int, [vp, _int, _int, _int, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+        "tpm_chainmesh_cells": (_int, [vp, _i64p]),
+        "tpm_chainmesh_perm": (_int, [vp, _i64p]),
+        "tpm_gravity_long_compute": (_int, [vp]),
+        "tpm_gravity_long_get_phi": (_int, [vp, _f32p]),
+        "tpm_gravity_long_get_density": (_int, [vp, _u64p]),
+        "tpm_gravity_long_get_green": (_int, [vp, _f32p]),
+        "tpm_kick": (_int, [vp, _int, _dbl, _dbl, _int, ctypes.c_uint, ctypes.POINTER(TpmStats)]),
+        "tpm_kick_treepm": (_int, [vp, _int, _dbl, _dbl, _int, ctypes.POINTER(TpmStats)]),
+        "tpm_ewald": (_int, [vp, _i64, _i64p, _dbl, _f64p, _f64p, _f64p, _f64p]),
+        "tpm_ic_zeldovich": (_int, [vp, _int, ctypes.c_uint64, _dbl, _dbl]),
+        "tpm_ic_uniform": (_int, [vp, _i64, ctypes.c_uint64]),
+        "tpm_profile": (_int, [vp, _int]),
+        "tpm_profile_read": (_int, [vp, _f64p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One device-resident TreePM state (tpm_ctx)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        s = L.tpm_ctx_create(int(device), ctypes.byref(h))
+        if s != 0:
+            raise TpmError(s, "tpm_ctx_create(%d) failed (no GPU visible?)" % device)
+        self._h = h
+        self._L = L
+        self.params = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.tpm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, s):
+        if s != 0:
+            raise TpmError(s, self._L.tpm_last_error(self._h).decode())
+
+    def set_params(self, p):
+        self._chk(self._L.tpm_set_params(self._h, ctypes.byref(p)))
+        q = TpmParams()
+        self._chk(self._L.tpm_get_params(self._h, ctypes.byref(q)))
+        self.params = q
+        return q
+
+    def upload(self, x, y, z, vx=None, vy=None, vz=None, rung=None):
+        x, y, z = [np.ascontiguousarray(a, np.uint32) for a in (x, y, z)]
+        vs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
+        r = None if rung is None else np.ascontiguousarray(rung, np.int8)
+        self._keep = (x, y, z, vs, r)
+        self._chk(self._L.tpm_particles_upload(self._h, len(x), _ptr(x), _ptr(y), _ptr(z), _ptr(vs[0]),
+                                               _ptr(vs[1]), _ptr(vs[2]), _ptr(r)))
+
+    def size(self):
+        return int(self._L.tpm_particles_size(self._h))
+
+    def download(self):
+        n = self.size()
+        out = dict(x=np.empty(n, np.uint32), y=np.empty(n, np.uint32), z=np.empty(n, np.uint32),
+                   vx=np.empty(n, np.float32), vy=np.empty(n, np.float32), vz=np.empty(n, np.float32),
+                   rung=np.empty(n, np.int8), gx=np.empty(n, np.float32), gy=np.empty(n, np.float32),
+                   gz=np.empty(n, np.float32), pot=np.empty(n, np.float32))
+        k = ["x", "y", "z", "vx", "vy", "vz", "rung", "gx", "gy", "gz", "pot"]
+        self._chk(self._L.tpm_particles_download(self._h, *[_ptr(out[a]) for a in k]))
+        return out
+
+    def chainmesh_create(self):
+        self._chk(self._L.tpm_chainmesh_create(self._h))
+
+    def chainmesh_cells(self):
+        C = self.params.chain_dim
+        cb = np.empty(C ** 3 + 1, np.int64)
+        self._chk(self._L.tpm_chainmesh_cells(self._h, cb))
+        return cb
+
+    def chainmesh_get(self, ix, iy, iz):
+        b, e = _i64(), _i64()
+        self._chk(self._L.tpm_chainmesh_get(self._h, ix, iy, iz, ctypes.byref(b), ctypes.byref(e)))
+        return b.value, e.value
+
+    def chainmesh_perm(self):
+        p = np.empty(self.size(), np.int64)
+        self._chk(self._L.tpm_chainmesh_perm(self._h, p))
+        return p
+
+    def gravity_long_compute(self):
+        self._chk(self._L.tpm_gravity_long_compute(self._h))
+
+    def get_phi(self):
+        M = self.params.mesh_dim
+        a = np.empty(M ** 3, np.float32)
+        self._chk(self._L.tpm_gravity_long_get_phi(self._h, a))
+        return a
+
+    def get_density(self):
+        M = self.params.mesh_dim
+        a = np.empty(M ** 3, np.uint64)
+        self._chk(self._L.tpm_gravity_long_get_density(self._h, a))
+        return a
+
+    def get_green(self):
+        M = self.params.mesh_dim
+        a = np.empty(M * M * (M // 2 + 1), np.float32)
+        self._chk(self._L.tpm_gravity_long_get_green(self._h, a))
+        return a
+
+    def kick(self, min_rung=0, scale=1.0, t0=1.0, first_call=True, flags=0):
+        st = TpmStats()
+        self._chk(self._L.tpm_kick(self._h, int(min_rung), float(scale), float(t0), int(bool(first_call)),
+                                   int(flags), ctypes.byref(st)))
+        return st
+
+    def kick_treepm(self, min_rung=0, scale=1.0, t0=1.0, first_call=True, stats=True):
+        st = TpmStats()
+        self._chk(self._L.tpm_kick_treepm(self._h, int(min_rung), float(scale), float(t0), int(bool(first_call)),
+                                          ctypes.byref(st) if stats else None))
+        return st
+
+    def ewald(self, samples, alpha=0.0):
+        samples = np.ascontiguousarray(samples, np.int64)
+        out = [np.empty(len(samples), np.float64) for _ in range(4)]
+        self._chk(self._L.tpm_ewald(self._h, len(samples), samples, float(alpha), *out))
+        return out
+
+    def ic_zeldovich(self, n_side, seed, rms_disp, spectral_index=-2.0):
+        self._chk(self._L.tpm_ic_zeldovich(self._h, int(n_side), int(seed), float(rms_disp), float(spectral_index)))
+
+    def ic_uniform(self, n, seed):
+        self._chk(self._L.tpm_ic_uniform(self._h, int(n), int(seed)))
+
+    def profile(self, enable):
+        self._chk(self._L.tpm_profile(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        """Summed ms of recorded launches and their counts."""
+        ms = np.zeros(4, np.float64)
+        npp, nst = _i64(), _i64()
+        self._chk(self._L.tpm_profile_read(self._h, ms, ctypes.byref(npp), ctypes.byref(nst)))
+        return dict(chainmesh_ms=ms[0], pm_ms=ms[1], pp_ms=ms[2], step_ms=ms[3], pp_launches=npp.value,
+                    steps=nst.value)
+
+    def sync(self):
+        self._chk(self._L.tpm_sync(self._h))
