@@ -80,6 +80,19 @@ typedef struct {
     int pad_;
 } tpm_stats;
 
+/* Launch-shape tuning (no reference counterpart: the reference fixes 32-thread
+ * blocks and 128-sink buckets, tigerpm.hpp:29-32).  Results do not depend on
+ * it beyond FP32 summation order.
+ *   tile_sinks:    sinks per PP work item, 16, 32 or 64 (one wave64 walks
+ *                  64/tile_sinks source streams of one culled source list);
+ *   deposit_block: chain cells per edge of an LDS-privatised deposit block,
+ *                  0 = largest that fits 64 KiB, -1 = per-particle global
+ *                  atomics. */
+typedef struct {
+    int tile_sinks;
+    int deposit_block;
+} tpm_tuning;
+
 /* Context lifetime.  device = HIP device ordinal. */
 TPM_API tpm_status tpm_ctx_create(int device, tpm_ctx** ctx);
 TPM_API tpm_status tpm_ctx_destroy(tpm_ctx* ctx);
@@ -89,6 +102,9 @@ TPM_API tpm_status tpm_sync(tpm_ctx* ctx);
 /* Parameters; validates and derives chain_dim.  Replaces get_options(). */
 TPM_API tpm_status tpm_set_params(tpm_ctx* ctx, const tpm_params* params);
 TPM_API tpm_status tpm_get_params(const tpm_ctx* ctx, tpm_params* params);
+/* Changing tile_sinks invalidates the chain mesh (call tpm_chainmesh_create). */
+TPM_API tpm_status tpm_set_tuning(tpm_ctx* ctx, const tpm_tuning* tuning);
+TPM_API tpm_status tpm_get_tuning(const tpm_ctx* ctx, tpm_tuning* tuning);
 
 /* Particle store (particles.hpp:17-25 SoA).  Host pointers; the copy is
  * synchronous.  vx/vy/vz/rung may be NULL (zeros). */

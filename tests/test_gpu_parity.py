@@ -98,15 +98,17 @@ def test_green_and_phi(gpu_ctx, M, order):
     assert rel_rms(phi, ophi) < 2e-6
 
 
-@pytest.mark.parametrize("dist", ["uniform", "clustered"])
-def test_pp_only_vs_oracle(gpu_ctx, dist):
+@pytest.mark.parametrize("dist,tile", [("uniform", 32), ("clustered", 32), ("clustered", 16), ("uniform", 64)])
+def test_pp_only_vs_oracle(gpu_ctx, dist, tile):
     n_side = 32
     n = n_side ** 3
     x, y, z = uniform(n, 41) if dist == "uniform" else clustered(n, 42)
     p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, 3))
+    gpu_ctx.set_tuning(tile_sinks=tile)
     gpu_ctx.upload(x, y, z)
     gpu_ctx.chainmesh_create()
     st = gpu_ctx.kick(flags=T.TPM_SKIP_PM | T.TPM_SKIP_KICK)
+    gpu_ctx.set_tuning(tile_sinks=32)
     d = gpu_ctx.download()
     operm, ocb = O.chainmesh_sort(x, y, z, p.chain_dim)
     xs, ys, zs = x[operm], y[operm], z[operm]

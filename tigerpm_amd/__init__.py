@@ -7,6 +7,6 @@ missing or no GPU is visible.
 """
 from ._lib import (  # noqa: F401
     TPM_CIC, TPM_TSC, TPM_CLOUD4, TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK,
-    Context, TpmError, TpmParams, TpmStats, lib, lib_path, exported_symbols, DECLARED_SYMBOLS,
+    Context, TpmError, TpmParams, TpmStats, TpmTuning, lib, lib_path, exported_symbols, DECLARED_SYMBOLS,
 )
 from .workloads import baseline_params, reference_params  # noqa: F401

@@ -43,6 +43,10 @@ class TpmStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
 
 
+class TpmTuning(ctypes.Structure):
+    _fields_ = [("tile_sinks", ctypes.c_int), ("deposit_block", ctypes.c_int)]
+
+
 _lib = None
 _P = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -106,6 +110,8 @@ def lib():
         "tpm_ewald": (_int, [vp, _i64, _i64p, _dbl, _f64p, _f64p, _f64p, _f64p]),
         "tpm_ic_zeldovich": (_int, [vp, _int, ctypes.c_uint64, _dbl, _dbl]),
         "tpm_ic_uniform": (_int, [vp, _i64, ctypes.c_uint64]),
+        "tpm_set_tuning": (_int, [vp, ctypes.POINTER(TpmTuning)]),
+        "tpm_get_tuning": (_int, [vp, ctypes.POINTER(TpmTuning)]),
         "tpm_profile": (_int, [vp, _int]),
         "tpm_profile_read": (_int, [vp, _f64p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     }
@@ -161,6 +167,14 @@ class Context:
         self._chk(self._L.tpm_get_params(self._h, ctypes.byref(q)))
         self.params = q
         return q
+
+    def set_tuning(self, tile_sinks=32, deposit_block=0):
+        self._chk(self._L.tpm_set_tuning(self._h, ctypes.byref(TpmTuning(int(tile_sinks), int(deposit_block)))))
+
+    def get_tuning(self):
+        t = TpmTuning()
+        self._chk(self._L.tpm_get_tuning(self._h, ctypes.byref(t)))
+        return dict(tile_sinks=t.tile_sinks, deposit_block=t.deposit_block)
 
     def upload(self, x, y, z, vx=None, vy=None, vz=None, rung=None):
         x, y, z = [np.ascontiguousarray(a, np.uint32) for a in (x, y, z)]

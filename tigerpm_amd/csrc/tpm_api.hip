@@ -91,6 +91,24 @@ extern "C" tpm_status tpm_get_params(const tpm_ctx* ctx, tpm_params* p) {
     return TPM_OK;
 }
 
+extern "C" tpm_status tpm_set_tuning(tpm_ctx* ctx, const tpm_tuning* t) {
+    if (!ctx || !t) return TPM_ERR_INVALID;
+    if (t->tile_sinks != 16 && t->tile_sinks != 32 && t->tile_sinks != 64)
+        return tpm_fail(ctx, TPM_ERR_INVALID, "tile_sinks must be 16, 32 or 64");
+    if (t->deposit_block < -1 || t->deposit_block > 8)
+        return tpm_fail(ctx, TPM_ERR_INVALID, "deposit_block must be in [-1, 8]");
+    ctx->tile = t->tile_sinks;
+    ctx->deposit_block = t->deposit_block;
+    return TPM_OK;
+}
+
+extern "C" tpm_status tpm_get_tuning(const tpm_ctx* ctx, tpm_tuning* t) {
+    if (!ctx || !t) return TPM_ERR_INVALID;
+    t->tile_sinks = ctx->tile;
+    t->deposit_block = ctx->deposit_block;
+    return TPM_OK;
+}
+
 tpm_status tpm_ensure_particles(tpm_ctx* ctx, int64_t n) {
     if (n > (int64_t)INT32_MAX - 1024) return tpm_fail(ctx, TPM_ERR_INVALID, "particle count exceeds 2^31");
     if (n <= ctx->cap && ctx->x) return TPM_OK;

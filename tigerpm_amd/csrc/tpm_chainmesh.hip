@@ -9,7 +9,8 @@
 // array, and a dense cell table cell_begin[C^3 + 1] found from the sorted
 // keys.  Cell ranges equal the reference partition's; the in-cell order is
 // deterministic (sub-cell, then previous order) so CPU and GPU agree bit for
-// bit.  Tiles of <= 64 consecutive same-cell particles are the PP work items.
+// bit.  Tiles of <= ctx->tile consecutive same-cell particles are the PP
+// work items.
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -64,24 +65,24 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
         p[i] = v;
 }
 
-__global__ void tiles_per_cell_kernel(int64_t ncell, const int32_t* __restrict__ cell_begin,
+__global__ void tiles_per_cell_kernel(int64_t ncell, int ts, const int32_t* __restrict__ cell_begin,
                                       int32_t* __restrict__ nt) {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
         const int32_t cnt = cell_begin[c + 1] - cell_begin[c];
-        nt[c] = (cnt + TPM_TILE - 1) / TPM_TILE;
+        nt[c] = (cnt + ts - 1) / ts;
     }
 }
 
-__global__ void tiles_write_kernel(int64_t ncell, const int32_t* __restrict__ cell_begin,
+__global__ void tiles_write_kernel(int64_t ncell, int ts, const int32_t* __restrict__ cell_begin,
                                    const int32_t* __restrict__ tile_end, int4* __restrict__ tiles,
                                    int32_t* __restrict__ ntiles) {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
         const int32_t b = cell_begin[c], e = cell_begin[c + 1];
-        const int32_t nt = (e - b + TPM_TILE - 1) / TPM_TILE;
+        const int32_t nt = (e - b + ts - 1) / ts;
         const int32_t t0 = tile_end[c] - nt;
         for (int32_t t = 0; t < nt; t++) {
-            const int32_t tb = b + t * TPM_TILE;
-            const int32_t te = min(tb + TPM_TILE, e);
+            const int32_t tb = b + t * ts;
+            const int32_t te = min(tb + ts, e);
             tiles[t0 + t] = make_int4((int)c, tb, te, 0);
         }
         if (c == ncell - 1) *ntiles = tile_end[c];
@@ -116,7 +117,8 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
         TPM_HIP(ctx, hipMalloc(&ctx->tile_cnt, sizeof(int32_t) * (ncell + 1)));
         ctx->cells_cap = ncell + 1;
     }
-    const int64_t tiles_max = n / TPM_TILE + ncell + 1;
+    const int ts = ctx->tile;
+    const int64_t tiles_max = n / ts + ncell + 1;
     if (tiles_max > ctx->tiles_cap) {
         if (ctx->tiles) TPM_HIP(ctx, hipFree(ctx->tiles));
         TPM_HIP(ctx, hipMalloc(&ctx->tiles, sizeof(int4) * tiles_max));
@@ -130,6 +132,7 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
         TPM_HIP(ctx, hipGetLastError());
         ctx->chain_valid = true;
         ctx->chain_C = C;
+        ctx->chain_tile = ts;
         return TPM_OK;
     }
 
@@ -168,7 +171,7 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
 
     cell_bounds_kernel<<<g, 256, 0, s>>>(n, ctx->keys2, ncell, ctx->cell_begin);
     // tiles per cell, then their inclusive scan.
-    tiles_per_cell_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ctx->cell_begin, ctx->tile_cnt);
+    tiles_per_cell_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ts, ctx->cell_begin, ctx->tile_cnt);
     TPM_HIP(ctx, hipGetLastError());
     size_t need2 = 0;
     TPM_HIP(ctx, rocprim::inclusive_scan(nullptr, need2, ctx->tile_cnt, ctx->tile_end, (size_t)ncell,
@@ -180,10 +183,11 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
     }
     TPM_HIP(ctx, rocprim::inclusive_scan(ctx->scan_tmp, need2, ctx->tile_cnt, ctx->tile_end, (size_t)ncell,
                                          rocprim::plus<int32_t>(), s));
-    tiles_write_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ctx->cell_begin, ctx->tile_end, ctx->tiles,
+    tiles_write_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ts, ctx->cell_begin, ctx->tile_end, ctx->tiles,
                                                             ctx->ntiles);
     TPM_HIP(ctx, hipGetLastError());
     ctx->chain_valid = true;
     ctx->chain_C = C;
+    ctx->chain_tile = ts;
     return TPM_OK;
 }

@@ -13,7 +13,6 @@
 
 #include "tigerpm_hip.h"
 
-#define TPM_TILE 64          // sinks per PP work item (one wave64)
 #define TPM_MAX_CHAIN 406    // C^3 * 64 must fit the 32-bit sort key
 
 struct tpm_fft {
@@ -63,6 +62,9 @@ struct tpm_ctx {
     size_t scan_tmp_bytes = 0;
     bool chain_valid = false;
     int chain_C = 0;
+    int tile = 32;        // sinks per PP tile: 16, 32 or 64 (tuning)
+    int chain_tile = 0;   // tile size the current tile list was built with
+    int deposit_block = 0;  // chain cells per deposit block edge; 0 auto, -1 per-particle atomics
 
     // PM mesh.
     int64_t mesh_cap = 0;  // M^3 capacity

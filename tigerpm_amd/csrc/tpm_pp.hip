@@ -1,22 +1,29 @@
 // tpm_pp.hip -- fused short-range PP + mesh interpolation + kick.
 //
 // Reference: kick_treepm_kernel (kick_treepm.cu:460-759) and kick_pme_kernel
-// (kick_pme.cu:128-357).  Work item here = one tile of <= 64 consecutive
-// sinks of one chain cell, one wave64 per tile, one sink per lane.
+// (kick_pme.cu:128-357).  Work item here = one tile of <= TS consecutive
+// sinks of one chain cell, processed by one wave64: lane l holds sink
+// l % TS and walks source stream l / TS (64 / TS streams over one shared
+// source list), the streams are summed at the end.
 //
 // Source staging: the wave walks the 27 neighbour cells (the candidate set
 // of kick_pme.cu:265-318) 64 particles at a time; each lane tests one source
 // against the integer bounding box of the tile's active sinks and survivors
-// are compacted (ballot + mbcnt) into an LDS ring as exact int32 offsets from
-// the tile's first particle.  A source whose box distance exceeds rcut has
-// r >= rcut to every sink, so dropping it changes no result.  The inner loop
-// reads each staged source with one broadcast ds_read_b128 and evaluates
-// compute_pp_interaction (kick_treepm.cu:242-283) for its sink.
+// are compacted (ballot + mbcnt) into an LDS buffer as exact int32 offsets
+// from the tile's first particle plus the source index.  A source whose box
+// distance exceeds rcut has r >= rcut to every sink, so dropping it changes
+// no result; smaller tiles have tighter boxes and drop more.  The inner loop
+// reads one staged source per stream with a ds_read_b128 (prefetched one
+// step ahead) and evaluates compute_pp_interaction (kick_treepm.cu:242-283).
 //
 // Pair distances are int32(sink - source) converted to float, which is the
 // reference's distance() (gravity_short.hpp:20-22) in units of 2^-32; every
 // constant is pre-scaled by the matching power of two so the FP32 arithmetic
-// is the reference's up to exact power-of-two factors.
+// is the reference's up to exact power-of-two factors.  The self pair, which
+// the reference evaluates inside the softening branch (it contributes
+// -35/16 h^-1 to phi and nothing to g), is masked out of the loop and added
+// once per sink, so the softening branch is taken only by distinct pairs
+// closer than h.
 #include "tpm_internal.hpp"
 
 #include <climits>
@@ -51,7 +58,9 @@ struct PPArgs {
     float GM, phi0, eta, t0, scale, hsoft;
 };
 
-__device__ __forceinline__ unsigned lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+__device__ __forceinline__ unsigned lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
 
 __device__ __forceinline__ unsigned count_below(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -69,9 +78,10 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // compute_pp_interaction (kick_treepm.cu:242-283) with erfcexp
-// (util.hpp:36-59, Abramowitz & Stegun 7.1.26), in 2^-32 units.
-__device__ __forceinline__ void pp_pair(const PPArgs& a, int ox, int oy, int oz, int4 s, float& gx, float& gy,
-                                        float& gz, float& ph) {
+// (util.hpp:36-59, Abramowitz & Stegun 7.1.26), in 2^-32 units.  self is
+// the sink's own particle index: that pair is excluded here.
+__device__ __forceinline__ void pp_pair(const PPArgs& a, int ox, int oy, int oz, int self, int4 s, float& gx,
+                                        float& gy, float& gz, float& ph) {
     const float dx = (float)(int)((unsigned)ox - (unsigned)s.x);
     const float dy = (float)(int)((unsigned)oy - (unsigned)s.y);
     const float dz = (float)(int)((unsigned)oz - (unsigned)s.z);
@@ -90,9 +100,10 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int ox, int oy, int oz,
     const float tre = a.tsp * r0 * e;
     float rinv3 = (erfc0 + tre) * (rinv * rinv) * rinv;
     float rinv1 = rinv * erfc0;
-    if (r2 <= a.h2s) {
-        // Inside the softening: Plummer polynomial plus the (erfc - 1)
-        // correction; the self pair (r = 0) gives -35/16 h^-1 in phi.
+    const bool other = s.w != self;
+    if (r2 <= a.h2s && other) {
+        // Distinct particles closer than h: Plummer polynomial plus the
+        // (erfc - 1) correction (kick_treepm.cu:263-276).
         const float ri = r2 > 0.f ? rinv : 0.f;
         const float q2 = r2 * a.h2inv_s;
         float d1 = 15.0f / 8.0f;
@@ -107,7 +118,7 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int ox, int oy, int oz,
         d0 *= a.hinv_s;
         rinv1 = (erfc0 - 1.0f) * ri + d0;
     }
-    const bool in = r2 < a.rcut2s;
+    const bool in = r2 < a.rcut2s && other;
     rinv3 = in ? rinv3 : 0.f;
     rinv1 = in ? rinv1 : 0.f;
     gx = fmaf(-dx, rinv3, gx);
@@ -132,7 +143,9 @@ __device__ __forceinline__ void interp6_weights(float x, float w[6], float dw[6]
     dw[5] = x * x * ((21.f / 24.f) + x * (-2.0f + x * (25.f / 24.f)));
 }
 
+template <int TS>
 __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
+    constexpr int SPLIT = 64 / TS;  // source streams per wave
     __shared__ int4 sbuf[SBUF];
     // XCD-aware tile order: the hardware deals workgroups round-robin over
     // the 8 XCDs, so give each XCD one contiguous (spatially coherent) run of
@@ -146,7 +159,9 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
 
     const int4 tile = a.tiles[t];
     const int lane = (int)lane_id();
-    const int i = tile.y + lane;
+    const int slot = lane % TS;
+    const int stream = lane / TS;
+    const int i = tile.y + slot;
     const bool valid = i < tile.z;
     const uint32_t rx = a.x[tile.y], ry = a.y[tile.y], rz = a.z[tile.y];
     uint32_t px = rx, py = ry, pz = rz;
@@ -172,6 +187,23 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
         const int cx = cid % C, cy = (cid / C) % C, cz = cid / (C * C);
         int nbuf = 0;
         unsigned long long staged = 0ull;
+        // Far dummy that pads the list to a multiple of SPLIT: r >= rcut.
+        const int4 far = make_int4(0x40000000, 0x40000000, 0x40000000, -1);
+        auto flush = [&]() {
+            const int padded = (nbuf + SPLIT - 1) / SPLIT * SPLIT;
+            if (lane < padded - nbuf) sbuf[nbuf + lane] = far;
+            __syncthreads();
+            int4 s = sbuf[stream];
+#pragma unroll 2
+            for (int q = 0; q < padded; q += SPLIT) {
+                const int4 sn = sbuf[min(q + SPLIT, padded - SPLIT) + stream];
+                pp_pair(a, ox, oy, oz, i, s, gx, gy, gz, ph);
+                s = sn;
+            }
+            staged += (unsigned long long)nbuf;
+            nbuf = 0;
+            __syncthreads();
+        };
         for (int nb = 0; nb < 27; nb++) {
             int jx = cx + (nb % 3) - 1, jy = cy + (nb / 3) % 3 - 1, jz = cz + nb / 9 - 1;
             jx += (jx < 0) ? C : 0;
@@ -185,7 +217,7 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
             for (int j0 = sb; j0 < se; j0 += 64) {
                 const int j = j0 + lane;
                 bool keep = false;
-                int4 sv = make_int4(0, 0, 0, 0);
+                int4 sv = make_int4(0, 0, 0, j);
                 if (j < se) {
                     sv.x = (int)(a.x[j] - rx);
                     sv.y = (int)(a.y[j] - ry);
@@ -198,30 +230,29 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
                 const unsigned long long m = __ballot(keep);
                 if (keep) sbuf[nbuf + (int)count_below(m)] = sv;
                 nbuf += __popcll(m);
-                if (nbuf > SBUF - 64) {
-                    __syncthreads();
-#pragma unroll 4
-                    for (int q = 0; q < nbuf; q++) pp_pair(a, ox, oy, oz, sbuf[q], gx, gy, gz, ph);
-                    staged += (unsigned long long)nbuf;
-                    nbuf = 0;
-                    __syncthreads();
-                }
+                if (nbuf > SBUF - 2 * 64) flush();
             }
         }
-        if (nbuf > 0) {
-            __syncthreads();
-#pragma unroll 4
-            for (int q = 0; q < nbuf; q++) pp_pair(a, ox, oy, oz, sbuf[q], gx, gy, gz, ph);
-            staged += (unsigned long long)nbuf;
+        if (nbuf > 0) flush();
+        // sum the source streams
+#pragma unroll
+        for (int o = TS; o < 64; o <<= 1) {
+            gx += __shfl_xor(gx, o);
+            gy += __shfl_xor(gy, o);
+            gz += __shfl_xor(gz, o);
+            ph += __shfl_xor(ph, o);
         }
-        if (lane == 0 && a.counters) atomicAdd(&a.counters[0], staged * (unsigned long long)__popcll(amask));
+        if (lane == 0 && a.counters)
+            atomicAdd(&a.counters[0], staged * (unsigned long long)__popcll(amask & ((TS == 64) ? ~0ull : ((1ull << TS) - 1))));
+        // the self pair (kick_treepm.cu:263-276 at r = 0): -35/16 h^-1
+        ph -= (35.0f / 16.0f) * a.hinv_s;
         // back to box units: g in 2^-64, phi in 2^-32
         gx *= 0x1p64f;
         gy *= 0x1p64f;
         gz *= 0x1p64f;
         ph *= 0x1p32f;
     }
-    if (!active) return;
+    if (!active || stream != 0) return;
 
     // Mesh force and potential (kick_treepm.cu:505-573) on the periodic mesh.
     float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
@@ -325,7 +356,7 @@ __global__ void candidate_pairs_kernel(int C, const int32_t* __restrict__ cell_b
 
 tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                        tpm_stats* stats) {
-    if (!ctx->chain_valid || ctx->chain_C != ctx->C)
+    if (!ctx->chain_valid || ctx->chain_C != ctx->C || ctx->chain_tile != ctx->tile)
         return tpm_fail(ctx, TPM_ERR_STATE, "tpm_kick: chain mesh not built (call tpm_chainmesh_create)");
     const bool use_pm = !(flags & TPM_SKIP_PM);
     if (use_pm && (!ctx->pm_valid || ctx->green_M != ctx->M))
@@ -385,7 +416,7 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     a.hsoft = (float)p.hsoft;
 
     const int64_t ncell = (int64_t)ctx->C * ctx->C * ctx->C;
-    const int64_t tiles_max = ctx->n / TPM_TILE + ncell + 1;
+    const int64_t tiles_max = ctx->n / ctx->tile + ncell + 1;
     const int64_t blocks = (tiles_max + 7) / 8 * 8;
     if (blocks > INT_MAX) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: too many tiles");
     if (!ctx->ev[0]) {
@@ -400,7 +431,11 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     }
     TPM_HIP(ctx, hipEventRecord(e0, s));
     if (ctx->n > 0) {
-        pp_kick_kernel<<<(unsigned)blocks, 64, 0, s>>>(a);
+        switch (ctx->tile) {
+            case 16: pp_kick_kernel<16><<<(unsigned)blocks, 64, 0, s>>>(a); break;
+            case 64: pp_kick_kernel<64><<<(unsigned)blocks, 64, 0, s>>>(a); break;
+            default: pp_kick_kernel<32><<<(unsigned)blocks, 64, 0, s>>>(a); break;
+        }
         TPM_HIP(ctx, hipGetLastError());
     }
     TPM_HIP(ctx, hipEventRecord(e1, s));
