@@ -1,0 +1,51 @@
+"""C ABI and host-side checks that need no GPU."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+import tigerpm_amd as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    declared = set(T.DECLARED_SYMBOLS)
+    assert len(declared) >= 20
+    assert declared <= set(T.exported_symbols())
+    T.lib()  # loads, every signature resolves
+
+
+def test_context_without_gpu_fails_cleanly():
+    import torch  # noqa: F401  (only to know whether a GPU is visible)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(T.TpmError):
+        T.Context(0)
+
+
+def test_workload_parameters():
+    p = T.baseline_params(512, 512, 3)
+    assert p.chain_dim == 91 and abs(p.rs * 512 - 1.25) < 1e-12 and abs(p.rcut / p.rs - 4.5) < 1e-12
+    assert abs(p.GM * 512 ** 3 - 1) < 1e-12 and abs(p.hsoft - 2 / (25 * 512)) < 1e-15
+    assert T.baseline_params(64, 64, 2).chain_dim == 11
+    assert T.baseline_params(256, 256, 3).chain_dim == 45
+    r = T.reference_params(130 - 130 % 15)
+    assert r.mesh_dim * 3 == 120 and r.chain_dim == 8 and r.order == 4
+    with pytest.raises(ValueError):
+        T.reference_params(130)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_header_and_binding_agree():
+    txt = open(os.path.join(ROOT, "include", "tigerpm_hip.h")).read()
+    for name in T.DECLARED_SYMBOLS:
+        assert name in open(os.path.join(ROOT, "tigerpm_amd", "_lib.py")).read(), name
+    assert "extern \"C\"" in txt
