@@ -168,7 +168,7 @@ class Context:
         self.params = q
         return q
 
-    def set_tuning(self, tile_sinks=32, deposit_block=0):
+    def set_tuning(self, tile_sinks=16, deposit_block=0):
         self._chk(self._L.tpm_set_tuning(self._h, ctypes.byref(TpmTuning(int(tile_sinks), int(deposit_block)))))
 
     def get_tuning(self):

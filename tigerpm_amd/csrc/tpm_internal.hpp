@@ -62,7 +62,7 @@ struct tpm_ctx {
     size_t scan_tmp_bytes = 0;
     bool chain_valid = false;
     int chain_C = 0;
-    int tile = 32;        // sinks per PP tile: 16, 32 or 64 (tuning)
+    int tile = 16;        // sinks per PP tile: 16, 32 or 64 (tuning)
     int chain_tile = 0;   // tile size the current tile list was built with
     int deposit_block = 0;  // chain cells per deposit block edge; 0 auto, -1 per-particle atomics
 

@@ -54,6 +54,7 @@ struct PPArgs {
     unsigned flags;
     // pair constants in 2^-32 box units
     float rcut2s, cull2s, inv2rs_s, tsp, h2s, h2inv_s, hinv_s, h3inv_s;
+    float e2s;  // -log2(e) / (4 rs^2), 2^-64 units: exp(-x^2) = exp2(r2 * e2s)
     // epilogue
     float GM, phi0, eta, t0, scale, hsoft;
 };
@@ -86,21 +87,24 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int ox, int oy, int oz,
     const float dy = (float)(int)((unsigned)oy - (unsigned)s.y);
     const float dz = (float)(int)((unsigned)oz - (unsigned)s.z);
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
+    // Distinct particles differ by >= 1 raw unit, so r2 >= 1 unless this is
+    // the self pair; the clamp keeps rinv^3 finite for it.
+    const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0f));
     const float r = r2 * rinv;
     const float r0 = r * a.inv2rs_s;
     const float t1 = __builtin_amdgcn_rcpf(fmaf(0.3275911f, r0, 1.f));
-    const float t2 = t1 * t1;
-    const float t3 = t2 * t1;
-    const float t4 = t2 * t2;
-    const float t5 = t2 * t3;
-    const float e = __expf(-r0 * r0);
-    const float erfc0 = fmaf(0.254829592f, t1, fmaf(-0.284496736f, t2, fmaf(1.421413741f, t3,
-                             fmaf(-1.453152027f, t4, 1.061405429f * t5)))) * e;
-    const float tre = a.tsp * r0 * e;
-    float rinv3 = (erfc0 + tre) * (rinv * rinv) * rinv;
-    float rinv1 = rinv * erfc0;
+    // The cutoff test (kick_treepm.cu:254) and the self exclusion zero the
+    // Gaussian factor, which zeroes both erfc and the force factor below.
     const bool other = s.w != self;
+    const bool in = r2 < a.rcut2s && other;
+    const float e = in ? __builtin_amdgcn_exp2f(r2 * a.e2s) : 0.f;
+    const float poly =
+        t1 * fmaf(t1, fmaf(t1, fmaf(t1, fmaf(t1, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
+                  0.254829592f);
+    const float erfc0 = poly * e;
+    const float tre = a.tsp * r0 * e;
+    float rinv3 = fmaf(a.tsp, r0, poly) * e * (rinv * rinv) * rinv;
+    float rinv1 = rinv * erfc0;
     if (r2 <= a.h2s && other) {
         // Distinct particles closer than h: Plummer polynomial plus the
         // (erfc - 1) correction (kick_treepm.cu:263-276).
@@ -118,9 +122,6 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int ox, int oy, int oz,
         d0 *= a.hinv_s;
         rinv1 = (erfc0 - 1.0f) * ri + d0;
     }
-    const bool in = r2 < a.rcut2s && other;
-    rinv3 = in ? rinv3 : 0.f;
-    rinv1 = in ? rinv1 : 0.f;
     gx = fmaf(-dx, rinv3, gx);
     gy = fmaf(-dy, rinv3, gy);
     gz = fmaf(-dz, rinv3, gz);
@@ -179,9 +180,13 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
 
     float gx = 0.f, gy = 0.f, gz = 0.f, ph = 0.f;
     if (!(a.flags & TPM_SKIP_PP)) {
-        const int lox = wave_min(active ? ox : INT_MAX), hix = wave_max(active ? ox : INT_MIN);
-        const int loy = wave_min(active ? oy : INT_MAX), hiy = wave_max(active ? oy : INT_MIN);
-        const int loz = wave_min(active ? oz : INT_MAX), hiz = wave_max(active ? oz : INT_MIN);
+        // Tile bounding box as floats.  float(int32) rounds by <= 64 raw
+        // units, so box distances below carry <= 192 units of error, inside
+        // the 512-unit margin of cull2s: the test never drops a source with
+        // r < rcut to any sink.
+        const float lox = (float)wave_min(active ? ox : INT_MAX), hix = (float)wave_max(active ? ox : INT_MIN);
+        const float loy = (float)wave_min(active ? oy : INT_MAX), hiy = (float)wave_max(active ? oy : INT_MIN);
+        const float loz = (float)wave_min(active ? oz : INT_MAX), hiz = (float)wave_max(active ? oz : INT_MIN);
         const int C = a.C;
         const int cid = tile.x;
         const int cx = cid % C, cy = (cid / C) % C, cz = cid / (C * C);
@@ -222,9 +227,10 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
                     sv.x = (int)(a.x[j] - rx);
                     sv.y = (int)(a.y[j] - ry);
                     sv.z = (int)(a.z[j] - rz);
-                    const float ex = (float)max(max((long long)lox - sv.x, (long long)sv.x - hix), 0ll);
-                    const float ey = (float)max(max((long long)loy - sv.y, (long long)sv.y - hiy), 0ll);
-                    const float ez = (float)max(max((long long)loz - sv.z, (long long)sv.z - hiz), 0ll);
+                    const float fx = (float)sv.x, fy = (float)sv.y, fz = (float)sv.z;
+                    const float ex = fmaxf(fmaxf(lox - fx, fx - hix), 0.f);
+                    const float ey = fmaxf(fmaxf(loy - fy, fy - hiy), 0.f);
+                    const float ez = fmaxf(fmaxf(loz - fz, fz - hiz), 0.f);
                     keep = !a.cull || fmaf(ex, ex, fmaf(ey, ey, ez * ez)) < a.cull2s;
                 }
                 const unsigned long long m = __ballot(keep);
@@ -395,10 +401,11 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     const float rcut = (float)p.rcut, rs = (float)p.rs, h = (float)p.hsoft;
     const float S = 0x1p32f;
     a.rcut2s = (rcut * rcut) * (S * S);
-    // Culling keeps a margin of 2^8 raw units over rcut (float rounding of
-    // the box distance is far below that); off for C < 5, where a source's
+    // Culling keeps a margin of 512 raw units over rcut (float rounding of
+    // the box distance stays below ~200); off for C < 5, where a source's
     // offset from the tile origin can wrap to the far image.
-    const double cull = (double)p.rcut * 4294967296.0 + 256.0;
+    const double cull = (double)p.rcut * 4294967296.0 + 512.0;
+    a.e2s = (float)(-1.4426950408889634 / (4.0 * p.rs * p.rs) / (4294967296.0 * 4294967296.0));
     a.cull2s = (float)(cull * cull);
     a.cull = ctx->C >= 5 ? 1 : 0;
     a.inv2rs_s = (1.0f / rs / 2.0f) / S;
