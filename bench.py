@@ -89,6 +89,23 @@ def cpu_baseline(ctx, p, budget_s, seed):
     }
 
 
+def copy_probe(nbytes):
+    """One device-to-device hipMemcpy of nbytes (the runtime's copy kernel),
+    the known-size reference for FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return nbytes
+
+
 def accuracy(ctx, p, nsamp, seed):
     d = ctx.download()
     n = ctx.size()
@@ -121,6 +138,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=16.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ewald", action="store_true")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported (multi-GPU sharding is "
@@ -188,6 +207,13 @@ def main():
                       "pm": prof["pm_ms"] / max(prof["steps"], 1), "pp_kick": pp_ms,
                       "step": prof["step_ms"] / max(prof["steps"], 1)},
     }
+    # HBM bytes per PP launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs (scaled by a plain copy of known size).
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = t["source"]
     if not args.no_ewald:
         log("FP64 Ewald on %d samples..." % w["samples"])
         out["accuracy"] = accuracy(ctx, p, w["samples"], w["seed"])
@@ -197,6 +223,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(ctx, p, args.cpu_budget, w["seed"])
         log("cpu", out["cpu_baseline"])
     ctx.close()
+    if args.probe_copy:
+        out["copy_probe_bytes"] = copy_probe(1 << 30)
     print(json.dumps(out), flush=True)
 
 
