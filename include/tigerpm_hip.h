@@ -38,6 +38,10 @@ typedef int tpm_status;
 #define TPM_ERR_FFT (-4)      /* rocFFT failure */
 #define TPM_ERR_BUDGET (-5)   /* device memory budget exceeded */
 
+/* Long-range solver types (GRAVITY_LONG_PM / GRAVITY_LONG_PME). */
+#define TPM_LONG_PM 0
+#define TPM_LONG_PME 1
+
 /* Mass-assignment orders. */
 #define TPM_CIC 2
 #define TPM_TSC 3
@@ -56,6 +60,9 @@ typedef struct {
     int chain_dim; /* C, chain-mesh cells per side; 0 = floor(1/rcut) */
     int order;     /* TPM_CIC / TPM_TSC / TPM_CLOUD4 */
     int do_phi;    /* accumulate the potential (reference do_phi) */
+    int long_type; /* TPM_LONG_PM: 1/k^2 only; TPM_LONG_PME: times exp(-k^2 rs^2)
+                      (gravity_long_type, gravity_long.hpp:17-19) */
+    int pad_;
     double rs;     /* split scale in box units (reference rs = 1/four_dim) */
     double rcut;   /* short-range cutoff in box units (reference 1/chain_dim) */
     double hsoft;  /* softening length (reference 2/(25 parts_dim)) */

@@ -1,0 +1,189 @@
+// tigerpm_ref.hpp -- the reference's C++ entry points for the TreePM kick
+// path, implemented over the C ABI of libtigerpm_hip.so (tigerpm_hip.h).
+//
+// Names, argument meanings and the module-global particle store follow the
+// reference headers (options.hpp, particles.hpp, chainmesh.hpp,
+// gravity_long.hpp, gravity_short.hpp, kick_treepm.hpp, kick_pme.hpp,
+// kick_pm.hpp, initialize.hpp, range.hpp, fixed.hpp), so code written
+// against them links here unchanged on one GPU.  Multi-locality pieces
+// (HPX fan-out, halo exchange, domain sort) reduce to no-ops on one device.
+// Errors from the device library throw std::runtime_error carrying
+// tpm_last_error(); the reference prints and aborts on the same conditions.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#define NDIM 3
+#define XDIM 0
+#define YDIM 1
+#define ZDIM 2
+#define MAX_RUNG 32
+#define SELF_PHI float(-35.0 / 16.0)
+
+// ---- fixed.hpp:23-235 ------------------------------------------------------
+// 32-bit fixed point in [0, 1): raw = uint32(2^32 * x) (truncating).
+class fixed32 {
+    uint32_t i_ = 0;
+
+public:
+    fixed32() = default;
+    fixed32(double x) : i_((uint32_t)(4294967296.0 * x)) {}
+    static fixed32 from_raw(uint32_t r) {
+        fixed32 f;
+        f.i_ = r;
+        return f;
+    }
+    uint32_t raw() const { return i_; }
+    float to_float() const { return (float)i_ * 0x1p-32f; }
+    double to_double() const { return (double)i_ * 0x1p-32; }
+    bool operator<(fixed32 o) const { return i_ < o.i_; }
+    bool operator==(fixed32 o) const { return i_ == o.i_; }
+};
+static_assert(sizeof(fixed32) == 4, "fixed32 must be the raw 32-bit word");
+
+// gravity_short.hpp:20-22: periodic minimum image.
+inline float distance(fixed32 a, fixed32 b) { return (float)(int32_t)(a.raw() - b.raw()) * 0x1p-32f; }
+
+// ---- range.hpp:31-203 (the parts the kick path uses) ------------------------
+template <class T>
+struct range {
+    std::array<T, NDIM> begin{};
+    std::array<T, NDIM> end{};
+    T volume() const {
+        T v = 1;
+        for (int d = 0; d < NDIM; d++) v *= (end[d] > begin[d]) ? end[d] - begin[d] : 0;
+        return v;
+    }
+    // row-major, z fastest (range.hpp:149-165)
+    T index(const std::array<T, NDIM>& i) const {
+        return (end[2] - begin[2]) * ((end[1] - begin[1]) * (i[0] - begin[0]) + (i[1] - begin[1])) + (i[2] - begin[2]);
+    }
+    bool contains(const std::array<T, NDIM>& i) const {
+        for (int d = 0; d < NDIM; d++)
+            if (i[d] < begin[d] || i[d] >= end[d]) return false;
+        return true;
+    }
+    range pad(T dx = 1) const {
+        range r = *this;
+        for (int d = 0; d < NDIM; d++) {
+            r.begin[d] -= dx;
+            r.end[d] += dx;
+        }
+        return r;
+    }
+};
+
+// ---- options.hpp:5-49, options.cpp:37-118 ----------------------------------
+struct options {
+    int four_dim = 0;
+    int chain_dim = 0;
+    int parts_dim = 130;
+    int parts_o_four = 3;
+    int parts_o_chain = 15;
+    int four_o_chain = 5;
+    double eta = 0.0;
+    double GM = 1.0;
+    double hsoft = 0.0;
+    double box_size = 1.0;
+    double rs = 0.0;
+    std::string config_file;
+    std::string test;
+    // Workload extensions of this build (defaults reproduce the reference):
+    double rs_cells = 0.0;   // split scale in mesh cells; 0 = reference (1 / four_dim)
+    double rcut_rs = 0.0;    // cutoff in units of rs; 0 = reference (1 / chain_dim)
+    double rcut = 0.0;       // derived
+    int four_dim_override = 0;  // mesh cells per side; 0 = parts_dim / parts_o_four
+    int order = 4;           // mass assignment: 4 cloud4 (reference), 3 TSC, 2 CIC
+    bool unit_mass = true;   // true: unit masses, GM = 1 (reference); false: m = 1/N
+    std::string ic = "uniform";  // particles_random_init / initialize(): uniform | zeldovich
+    double rms_disp = 0.5;   // Zel'dovich rms displacement in lattice cells
+    uint64_t seed = 42;
+    int tile_sinks = 16;     // PP work-item size (tpm_tuning)
+};
+// Parses --parts_dim --parts_o_four --four_o_chain --test plus the extensions
+// (--four_dim --rs_cells --rcut_rs --order --mass=unit|1/N --ic --rms_disp
+// --seed --tile_sinks); derives the dependent fields (options.cpp:81-96).
+bool process_options(int argc, char* argv[]);
+const options& get_options();
+void set_options(const options& opts);  // derives and installs
+
+// ---- particles.hpp:17-122 ----------------------------------------------------
+struct particle {
+    std::array<fixed32, NDIM> x;
+    std::array<float, NDIM> v;
+    char r;
+    std::array<float, NDIM> g;
+    float p;
+};
+extern std::array<std::vector<fixed32>, NDIM> particles_X;
+extern std::array<std::vector<float>, NDIM> particles_U;
+extern std::vector<char> particles_R;
+extern std::vector<float> particles_P;
+extern std::array<std::vector<float>, NDIM> particles_G;
+
+int particles_size();
+int particles_local_size();
+void particles_resize(size_t new_size);
+void particles_random_init();  // parts_dim^3 uniform positions (seeded)
+void particles_domain_sort();  // one device: nothing to migrate
+particle particles_get_particle(int index);
+std::vector<particle> particles_sample(int nsamples);
+std::array<int, NDIM> particles_mesh_loc(int index);
+
+// ---- initialize.hpp ------------------------------------------------------------
+void initialize();  // Zel'dovich lattice (options::ic / rms_disp / seed)
+
+// ---- chainmesh.hpp:8-17 ------------------------------------------------------
+struct chaincell {
+    int pbegin;
+    int pend;
+};
+void chainmesh_create();
+void chainmesh_exchange_begin();  // one device: periodic neighbours are local
+void chainmesh_exchange_end();
+range<int> chainmesh_interior_box();
+chaincell chainmesh_get(const std::array<int, NDIM>& cell);
+
+// ---- gravity_long.hpp:17-25 --------------------------------------------------
+using gravity_long_type = int;
+#define GRAVITY_LONG_PM 0
+#define GRAVITY_LONG_PME 1
+void gravity_long_compute(gravity_long_type type = GRAVITY_LONG_PM);
+std::vector<float> gravity_long_get_phi(const range<int>& box);  // periodic
+range<int> gravity_long_get_phi_box();
+std::pair<float, std::array<float, NDIM>> gravity_long_force_at(const std::array<double, NDIM>& pos);
+
+// ---- kick_treepm.hpp / kick_pme.hpp / kick_pm.hpp ----------------------------
+void kick_treepm_begin(int min_rung, double scale, double t0, bool first_call);
+void kick_treepm_end();
+void kick_pme_begin(int min_rung, double scale, double t0, bool first_call);
+void kick_pme_end();
+void kick_pm();
+
+// ---- gravity_short.hpp:25-28 -------------------------------------------------
+std::pair<std::vector<double>, std::array<std::vector<double>, NDIM>> gravity_short_ewald_call_kernel(
+    const std::vector<fixed32>& sinkx, const std::vector<fixed32>& sinky, const std::vector<fixed32>& sinkz);
+void gravity_short_ewald_compare(int nsamples);
+// The numbers gravity_short_ewald_compare prints (gravity_short.cpp:53-79)
+// plus the vector rms, for callers that want to check them.
+struct ewald_errors {
+    double force_rms_magnitude;  // reference metric: sqrt(sum (|g|-|gE|)^2 / sum |gE|^2)
+    double force_max;            // reference "max": sum ||g|-|gE|| / sqrt(sum |gE|^2)
+    double force_rms_vector;     // sqrt(sum |g-gE|^2 / sum |gE|^2)
+    double potential_rms;
+    int nsamples;
+};
+ewald_errors gravity_short_ewald_errors(int nsamples);
+
+// ---- test.hpp -----------------------------------------------------------------
+void run_test(std::string test);
+// Last kick's device statistics (milliseconds and pair counts).
+struct kick_report {
+    double ms_chainmesh, ms_pm, ms_pp;
+    double candidate_pairs, executed_pairs;
+};
+kick_report last_kick_report();

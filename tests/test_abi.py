@@ -49,3 +49,23 @@ def test_header_and_binding_agree():
     for name in T.DECLARED_SYMBOLS:
         assert name in open(os.path.join(ROOT, "tigerpm_amd", "_lib.py")).read(), name
     assert "extern \"C\"" in txt
+
+
+def test_reference_named_cxx_layer_exports():
+    lib = os.path.join(ROOT, "tigerpm_amd", "libtigerpm_ref.so")
+    out = subprocess.run(["nm", "-DC", "--defined-only", lib], capture_output=True, text=True, check=True).stdout
+    for sig in ["kick_treepm_begin(int, double, double, bool)", "kick_treepm_end()", "gravity_long_compute(int)",
+                "chainmesh_create()", "chainmesh_get(std::array<int, 3ul> const&)", "kick_pm()",
+                "gravity_short_ewald_compare(int)", "gravity_long_force_at(std::array<double, 3ul> const&)",
+                "particles_random_init()", "initialize()", "kick_pme_begin(int, double, double, bool)"]:
+        assert sig in out, sig
+    assert "particles_X" in out
+
+
+def test_driver_fails_cleanly_without_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    drv = os.path.join(ROOT, "tigerpm_amd", "bin", "tigerpm")
+    r = subprocess.run([drv, "--test=treepm", "--parts_dim=60"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "tpm_ctx_create" in r.stderr

@@ -51,9 +51,33 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
     os.replace(tmp, LIB)
+    build_host()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, REF_LIB, DRIVER)
     return LIB
+
+
+HOST = os.path.join(HERE, "host")
+REF_LIB = os.path.join(HERE, "libtigerpm_ref.so")
+DRIVER = os.path.join(HERE, "bin", "tigerpm")
+
+
+def build_host():
+    """The reference-named C++ layer (libtigerpm_ref.so) and the --test=
+    driver, host-only code linked against libtigerpm_hip.so."""
+    inc = ["-I" + os.path.join(ROOT, "include")]
+    rpath = "-Wl,-rpath,$ORIGIN:$ORIGIN/.."
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall"] + inc + [
+        os.path.join(HOST, "tigerpm_ref.cpp"), "-o", REF_LIB, "-L" + HERE, "-ltigerpm_hip", rpath]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall"] + inc + [
+        os.path.join(HOST, "tigerpm_main.cpp"), "-o", DRIVER, "-L" + HERE, "-ltigerpm_ref", "-ltigerpm_hip", rpath]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-8000:]))
 
 
 if __name__ == "__main__":

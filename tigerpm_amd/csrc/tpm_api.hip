@@ -67,6 +67,9 @@ extern "C" tpm_status tpm_set_params(tpm_ctx* ctx, const tpm_params* p) {
     tpm_params q = *p;
     if (q.mesh_dim < 4 || q.mesh_dim > 2048) return tpm_fail(ctx, TPM_ERR_INVALID, "mesh_dim out of range [4, 2048]");
     if (q.order < 2 || q.order > 4) return tpm_fail(ctx, TPM_ERR_INVALID, "order must be 2 (CIC), 3 (TSC) or 4");
+    if (q.long_type != TPM_LONG_PM && q.long_type != TPM_LONG_PME)
+        return tpm_fail(ctx, TPM_ERR_INVALID, "long_type must be TPM_LONG_PM or TPM_LONG_PME");
+    q.pad_ = 0;
     if (!(q.rs > 0.0) || !(q.rcut > 0.0) || !(q.hsoft > 0.0) || !(q.rcut < 0.5))
         return tpm_fail(ctx, TPM_ERR_INVALID, "rs, rcut, hsoft must be positive and rcut < 1/2");
     if (!(q.eta > 0.0) || !std::isfinite(q.GM) || !(q.n_total >= 0.0))
@@ -76,7 +79,8 @@ extern "C" tpm_status tpm_set_params(tpm_ctx* ctx, const tpm_params* p) {
         return tpm_fail(ctx, TPM_ERR_INVALID, "chain_dim must be in [3, 406]");
     if ((double)q.chain_dim * q.rcut > 1.0)
         return tpm_fail(ctx, TPM_ERR_INVALID, "chain cells smaller than rcut: the 27-cell stencil would miss pairs");
-    if (ctx->params_set && (q.mesh_dim != ctx->p.mesh_dim || q.order != ctx->p.order || q.rs != ctx->p.rs))
+    if (ctx->params_set && (q.mesh_dim != ctx->p.mesh_dim || q.order != ctx->p.order || q.rs != ctx->p.rs ||
+                            q.long_type != ctx->p.long_type))
         ctx->pm_valid = false;
     ctx->p = q;
     ctx->C = q.chain_dim;

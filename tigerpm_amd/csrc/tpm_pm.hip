@@ -295,11 +295,13 @@ tpm_status tpm_pm_solve(tpm_ctx* ctx) {
     const int64_t MC = (int64_t)M * M * (M / 2 + 1);
     hipStream_t s = ctx->stream;
     TPM_TRY(tpm_ensure_mesh(ctx, M));
-    if (ctx->green_M != M || ctx->green_rs != ctx->p.rs || ctx->green_order != ctx->p.order) {
-        green_kernel<<<grid_for(MC, 256), 256, 0, s>>>(M, ctx->p.rs, ctx->p.order, ctx->green);
+    // GRAVITY_LONG_PM has no Gaussian filter: the table with rs = 0.
+    const double rs_eff = ctx->p.long_type == TPM_LONG_PM ? 0.0 : ctx->p.rs;
+    if (ctx->green_M != M || ctx->green_rs != rs_eff || ctx->green_order != ctx->p.order) {
+        green_kernel<<<grid_for(MC, 256), 256, 0, s>>>(M, rs_eff, ctx->p.order, ctx->green);
         TPM_HIP(ctx, hipGetLastError());
         ctx->green_M = M;
-        ctx->green_rs = ctx->p.rs;
+        ctx->green_rs = rs_eff;
         ctx->green_order = ctx->p.order;
     }
     TPM_TRY(tpm_fill_u64(ctx, ctx->rho_q, M3));

@@ -27,6 +27,7 @@ def baseline_params(n_side, mesh, order, n_total=None):
     rs = 1.25 / mesh
     rcut = 4.5 * rs
     return TpmParams(mesh_dim=int(mesh), chain_dim=int(math.floor(1.0 / rcut)), order=int(order), do_phi=1,
+                     long_type=1,
                      rs=rs, rcut=rcut, hsoft=2.0 / (25.0 * n_side), GM=1.0 / n, eta=0.2 / math.sqrt(2.0),
                      n_total=float(n))
 
@@ -36,6 +37,6 @@ def reference_params(parts_dim, parts_o_four=3, four_o_chain=5):
         raise ValueError("parts_dim must be a multiple of %d" % (parts_o_four * four_o_chain))
     four_dim = parts_dim // parts_o_four
     chain_dim = parts_dim // (parts_o_four * four_o_chain)
-    return TpmParams(mesh_dim=four_dim, chain_dim=chain_dim, order=4, do_phi=1, rs=1.0 / four_dim,
+    return TpmParams(mesh_dim=four_dim, chain_dim=chain_dim, order=4, do_phi=1, long_type=1, rs=1.0 / four_dim,
                      rcut=1.0 / chain_dim, hsoft=2.0 / 25.0 / parts_dim, GM=1.0, eta=0.2 / math.sqrt(2.0),
                      n_total=float(parts_dim) ** 3)
