@@ -45,7 +45,8 @@ def cpu_baseline(ctx, p, budget_s, seed):
     d = ctx.download()
     x, y, z = d["x"], d["y"], d["z"]
     cb = ctx.chainmesh_cells()
-    op = O.make_params(p.mesh_dim, p.chain_dim, p.order, p.rs, p.rcut, p.hsoft, p.GM, p.eta, p.n_total)
+    op = O.make_params(p.mesh_dim, p.chain_dim, p.order, p.rs, p.rcut, p.hsoft, p.GM, p.eta, p.n_total, 1,
+                       p.interlace)
     rng = np.random.default_rng(seed)
     threads = O.num_threads()
     # PP + interpolation over a random sample of sinks (27-cell stencil, the
@@ -138,6 +139,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=16.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ewald", action="store_true")
+    ap.add_argument("--no-pcie", action="store_true", help="skip the host-buffer (PCIe-inclusive) measurement")
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -148,7 +150,7 @@ def main():
 
     w = WORKLOADS[args.workload]
     ctx = T.Context(0)
-    p = ctx.set_params(T.baseline_params(w["n_side"], w["mesh"], w["order"]))
+    p = ctx.set_params(T.baseline_params(w["n_side"], w["mesh"], w["order"], interlace=w["interlace"]))
     log("workload", args.workload, "params", p.as_dict())
     t = time.perf_counter()
     if w["ic"] == "zeldovich":
@@ -207,6 +209,20 @@ def main():
                       "pm": prof["pm_ms"] / max(prof["steps"], 1), "pp_kick": pp_ms,
                       "step": prof["step_ms"] / max(prof["steps"], 1)},
     }
+    # PCIe-inclusive rate of the host-buffer boundary (not the metric): host
+    # SoA arrays in, one full kick, velocities/rungs/forces/potential out.
+    if not args.no_pcie:
+        h = ctx.download()
+        t = time.perf_counter()
+        ctx.upload(h["x"], h["y"], h["z"], h["vx"], h["vy"], h["vz"], h["rung"])
+        ctx.kick_treepm(0, 1.0, 1.0, True, stats=False)
+        ctx.download()
+        tp = time.perf_counter() - t
+        out["pcie_inclusive"] = {"kicks_per_s": n / tp, "seconds": tp,
+                                 "note": "tpm_particles_upload + tpm_kick_treepm + tpm_particles_download, "
+                                         "pageable host memory"}
+        log("PCIe-inclusive", out["pcie_inclusive"])
+        del h
     # HBM bytes per PP launch from the committed rocprofv3 FETCH_SIZE /
     # WRITE_SIZE runs (scaled by a plain copy of known size).
     tfile = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)

@@ -62,7 +62,9 @@ typedef struct {
     int do_phi;    /* accumulate the potential (reference do_phi) */
     int long_type; /* TPM_LONG_PM: 1/k^2 only; TPM_LONG_PME: times exp(-k^2 rs^2)
                       (gravity_long_type, gravity_long.hpp:17-19) */
-    int pad_;
+    int interlace; /* 1: add a deposit on the grid displaced by half a cell and
+                      combine in k space (cancels odd aliases; not in the
+                      reference, whose single deposit is interlace = 0) */
     double rs;     /* split scale in box units (reference rs = 1/four_dim) */
     double rcut;   /* short-range cutoff in box units (reference 1/chain_dim) */
     double hsoft;  /* softening length (reference 2/(25 parts_dim)) */

@@ -103,6 +103,7 @@ struct options {
     double rms_disp = 0.5;   // Zel'dovich rms displacement in lattice cells
     uint64_t seed = 42;
     int tile_sinks = 16;     // PP work-item size (tpm_tuning)
+    int interlace = 0;       // 1: interlaced (two-grid) deposit
 };
 // Parses --parts_dim --parts_o_four --four_o_chain --test plus the extensions
 // (--four_dim --rs_cells --rcut_rs --order --mass=unit|1/N --ic --rms_disp

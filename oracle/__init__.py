@@ -33,6 +33,8 @@ class OrcParams(ctypes.Structure):
         ("chain_dim", c_int),
         ("order", c_int),
         ("do_phi", c_int),
+        ("interlace", c_int),
+        ("pad_", c_int),
         ("rs", c_dbl),
         ("rcut", c_dbl),
         ("hsoft", c_dbl),
@@ -68,6 +70,8 @@ def lib():
             "orc_weights_1d": (c_int, [c_int, ctypes.c_uint32, c_int, ctypes.POINTER(c_i64), u64p,
                                        ctypes.POINTER(c_int)]),
             "orc_deposit": (None, [c_i64, u32p, u32p, u32p, c_int, c_int, u64p]),
+            "orc_deposit_shifted": (None, [c_i64, u32p, u32p, u32p, c_int, c_int, ctypes.c_uint64, u64p]),
+            "orc_pm_phi_interlaced": (None, [c_int, f32p, f32p, f32p, f32p]),
             "orc_density_float": (None, [c_i64, u64p, f32p]),
             "orc_fft3d_r2c": (None, [c_int, f64p, f64p]),
             "orc_fft3d_c2r": (None, [c_int, f64p, f64p]),
@@ -97,9 +101,9 @@ def lib():
     return _lib
 
 
-def make_params(mesh_dim, chain_dim, order, rs, rcut, hsoft, GM, eta, n_total, do_phi=1):
-    return OrcParams(int(mesh_dim), int(chain_dim), int(order), int(do_phi), float(rs), float(rcut),
-                     float(hsoft), float(GM), float(eta), float(n_total))
+def make_params(mesh_dim, chain_dim, order, rs, rcut, hsoft, GM, eta, n_total, do_phi=1, interlace=0):
+    return OrcParams(int(mesh_dim), int(chain_dim), int(order), int(do_phi), int(interlace), 0, float(rs),
+                     float(rcut), float(hsoft), float(GM), float(eta), float(n_total))
 
 
 def _u32(a):
@@ -145,6 +149,20 @@ def deposit(x, y, z, M, order):
     g = np.empty(M ** 3, np.uint64)
     lib().orc_deposit(len(x), x, y, z, M, order, g)
     return g
+
+
+def deposit_shifted(x, y, z, M, order, shift):
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    g = np.empty(M ** 3, np.uint64)
+    lib().orc_deposit_shifted(len(x), x, y, z, M, order, int(shift), g)
+    return g
+
+
+def pm_phi_interlaced(M, rho0, rho1, table):
+    phi = np.empty(M ** 3, np.float32)
+    lib().orc_pm_phi_interlaced(M, np.ascontiguousarray(rho0, np.float32), np.ascontiguousarray(rho1, np.float32),
+                                np.ascontiguousarray(table, np.float32), phi)
+    return phi
 
 
 def density_float(grid):

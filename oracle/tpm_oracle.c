@@ -25,6 +25,8 @@ typedef struct {
     int chain_dim;  /* C: chain-mesh cells per side (reference chain_dim)        */
     int order;      /* mass assignment: 2 CIC, 3 TSC, 4 cloud4 (cubic B-spline)  */
     int do_phi;     /* accumulate the potential (reference do_phi)               */
+    int interlace;  /* second deposit on the half-cell-shifted grid (this build) */
+    int pad_;
     double rs;      /* split scale, box units                                    */
     double rcut;    /* short-range cutoff, box units                             */
     double hsoft;   /* Plummer-polynomial softening length, box units            */
@@ -229,8 +231,11 @@ ORC_API void orc_chainmesh_partition_ref(int64_t n, const uint32_t* x, const uin
  *        Q16 so that the cubic fits 64 bits.
  * Returns the first node index (may be -1 or M; wrapped by the caller), the
  * number of weights and the weight precision in bits. */
-static inline int weights_1d(int order, uint32_t raw, int M, int64_t* base, uint64_t w[4], int* qbits) {
-    const uint64_t P = (uint64_t)raw * (uint64_t)M;
+static inline int weights_1d(int order, uint32_t raw, int M, uint64_t shift, int64_t* base, uint64_t w[4],
+                             int* qbits) {
+    /* shift = 2^31 deposits onto the grid displaced by half a cell (node j
+     * at (j - 1/2) / M), the second grid of an interlaced deposit. */
+    const uint64_t P = (uint64_t)raw * (uint64_t)M + shift;
     if (order == 2) {
         const uint64_t t = (uint32_t)P >> 11;
         *base = (int64_t)(P >> 32);
@@ -269,19 +274,19 @@ static inline int weights_1d(int order, uint32_t raw, int M, int64_t* base, uint
 }
 
 ORC_API int orc_weights_1d(int order, uint32_t raw, int M, int64_t* base, uint64_t* w, int* qbits) {
-    return weights_1d(order, raw, M, base, w, qbits);
+    return weights_1d(order, raw, M, 0, base, w, qbits);
 }
 
 /* 3-D weights in Q32 (2^32 = one unit mass).  Products are truncated to Q32
  * and the truncation deficit is added to the centre node, so every particle
  * deposits exactly 2^32 and the grid total is exactly n * 2^32. */
-static inline int weights_3d(int order, uint32_t x, uint32_t y, uint32_t z, int M, int64_t base[3],
-                             uint64_t w3[64]) {
+static inline int weights_3d(int order, uint32_t x, uint32_t y, uint32_t z, int M, uint64_t grid_shift,
+                             int64_t base[3], uint64_t w3[64]) {
     uint64_t wx[4], wy[4], wz[4];
     int q;
-    const int np = weights_1d(order, x, M, &base[0], wx, &q);
-    weights_1d(order, y, M, &base[1], wy, &q);
-    weights_1d(order, z, M, &base[2], wz, &q);
+    const int np = weights_1d(order, x, M, grid_shift, &base[0], wx, &q);
+    weights_1d(order, y, M, grid_shift, &base[1], wy, &q);
+    weights_1d(order, z, M, grid_shift, &base[2], wz, &q);
     const int shift = 3 * q - 32;
     uint64_t sum = 0;
     for (int a = 0; a < np; a++)
@@ -298,15 +303,15 @@ static inline int weights_3d(int order, uint32_t x, uint32_t y, uint32_t z, int 
 
 /* compute_source (gravity_long.cpp:142-199) with integer accumulation.
  * grid is [x][y][z], z fastest (range::index, range.hpp:149-165). */
-ORC_API void orc_deposit(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int M, int order,
-                         uint64_t* grid) {
+ORC_API void orc_deposit_shifted(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int M,
+                                 int order, uint64_t shift, uint64_t* grid) {
     const int64_t M3 = (int64_t)M * M * M;
     memset(grid, 0, sizeof(uint64_t) * (size_t)M3);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         int64_t base[3];
         uint64_t w3[64];
-        const int np = weights_3d(order, x[i], y[i], z[i], M, base, w3);
+        const int np = weights_3d(order, x[i], y[i], z[i], M, shift, base, w3);
         for (int a = 0; a < np; a++) {
             const int64_t ix = (base[0] + a + M) % M;
             for (int b2 = 0; b2 < np; b2++) {
@@ -319,6 +324,11 @@ ORC_API void orc_deposit(int64_t n, const uint32_t* x, const uint32_t* y, const 
             }
         }
     }
+}
+
+ORC_API void orc_deposit(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int M, int order,
+                         uint64_t* grid) {
+    orc_deposit_shifted(n, x, y, z, M, order, 0, grid);
 }
 
 /* Mass per node in unit masses: float(double(sum) * 2^-32) (both roundings
@@ -517,6 +527,43 @@ ORC_API void orc_pm_phi(int M, const float* rho, const float* table, float* phi)
     for (int64_t i = 0; i < M3; i++) phi[i] = (float)r[i];
     free(r);
     free(c);
+}
+
+/* Interlaced potential: rho1 is the deposit onto the grid displaced by half
+ * a cell along every axis (node j at (j - 1/2)/M).  A mode aliased by an
+ * integer vector n carries (-1)^(nx+ny+nz) on that grid relative to the
+ * unshifted one, so rho_hat = (rho0_hat + exp(i (kx+ky+kz)/2) rho1_hat) / 2
+ * cancels the aliases with odd nx+ny+nz. */
+ORC_API void orc_pm_phi_interlaced(int M, const float* rho0, const float* rho1, const float* table, float* phi) {
+    const int64_t M3 = (int64_t)M * M * M;
+    const int Mh = M / 2 + 1;
+    const int64_t MC = (int64_t)M * M * Mh;
+    double* r = malloc(sizeof(double) * (size_t)M3);
+    double* c0 = malloc(sizeof(double) * 2 * (size_t)MC);
+    double* c1 = malloc(sizeof(double) * 2 * (size_t)MC);
+    for (int64_t i = 0; i < M3; i++) r[i] = rho0[i];
+    orc_fft3d_r2c(M, r, c0);
+    for (int64_t i = 0; i < M3; i++) r[i] = rho1[i];
+    orc_fft3d_r2c(M, r, c1);
+    const double c = 2.0 * M_PI / M;
+    for (int64_t ix = 0; ix < M; ix++)
+        for (int64_t iy = 0; iy < M; iy++)
+            for (int64_t iz = 0; iz < Mh; iz++) {
+                const int64_t l = (ix * M + iy) * Mh + iz;
+                const double th = 0.5 * c *
+                                  (double)((2 * ix < M ? ix : ix - M) + (2 * iy < M ? iy : iy - M) +
+                                           (2 * iz < M ? iz : iz - M));
+                const double cs = cos(th), sn = sin(th);
+                const double re = 0.5 * (c0[2 * l] + cs * c1[2 * l] - sn * c1[2 * l + 1]);
+                const double im = 0.5 * (c0[2 * l + 1] + sn * c1[2 * l] + cs * c1[2 * l + 1]);
+                c0[2 * l] = re * table[l];
+                c0[2 * l + 1] = im * table[l];
+            }
+    orc_fft3d_c2r(M, c0, r);
+    for (int64_t i = 0; i < M3; i++) phi[i] = (float)r[i];
+    free(r);
+    free(c0);
+    free(c1);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -907,15 +954,25 @@ ORC_API void orc_kick_full(const orc_params* p, int64_t n, uint32_t* x, uint32_t
     orc_deposit(n, x, y, z, M, p->order, grid);
     float* rho = malloc(sizeof(float) * (size_t)M3);
     orc_density_float(M3, grid, rho);
+    float* rho1 = NULL;
+    if (p->interlace) {
+        orc_deposit_shifted(n, x, y, z, M, p->order, 1ull << 31, grid);
+        rho1 = malloc(sizeof(float) * (size_t)M3);
+        orc_density_float(M3, grid, rho1);
+    }
     free(grid);
     times[1] = omp_get_wtime() - t;
     t = omp_get_wtime();
     float* table = malloc(sizeof(float) * (size_t)M * M * (M / 2 + 1));
     orc_green_table(M, p->rs, p->order, table);
     float* phi = malloc(sizeof(float) * (size_t)M3);
-    orc_pm_phi(M, rho, table, phi);
+    if (p->interlace)
+        orc_pm_phi_interlaced(M, rho, rho1, table, phi);
+    else
+        orc_pm_phi(M, rho, table, phi);
     free(table);
     free(rho);
+    free(rho1);
     times[2] = omp_get_wtime() - t;
     t = omp_get_wtime();
     int64_t* sinks = malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));

@@ -146,6 +146,32 @@ def test_weights_match_assignment_functions(order):
         assert np.all(np.abs(w / 2.0 ** q - ref) < 4 * 2.0 ** -q)
 
 
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_shifted_deposit_is_half_cell_translation(order):
+    """The interlacing grid (shift 2^31 in raw * M) equals depositing the
+    particles moved by +1/2 cell, exact when 2^32 / (2M) is an integer."""
+    M = 16
+    x, y, z = rand_u32(3000, 61), rand_u32(3000, 62), rand_u32(3000, 63)
+    half = np.uint32(2 ** 31 // M)
+    a = O.deposit_shifted(x, y, z, M, order, 1 << 31)
+    b = O.deposit(x + half, y + half, z + half, M, order)
+    assert np.array_equal(a, b)
+
+
+def test_interlaced_phi_smooth_mode_unchanged():
+    """For a band-limited density both grids sample the same field, so the
+    interlaced combination reproduces the single-grid potential."""
+    M = 16
+    j = np.arange(M)
+    f = lambda s: np.cos(2 * np.pi * (j + s) / M)[:, None, None] * np.cos(2 * np.pi * 2 * (j + s) / M)[None, :, None]
+    rho0 = (f(0.0) * np.ones((1, 1, M))).astype(np.float32).reshape(-1)
+    rho1 = (f(-0.5) * np.ones((1, 1, M))).astype(np.float32).reshape(-1)  # node j at (j - 1/2) / M
+    t = O.green_table(M, 1.25 / M, 2)
+    a = O.pm_phi(M, rho0, t)
+    b = O.pm_phi_interlaced(M, rho0, rho1, t)
+    assert rel_rms(b, a) < 1e-5
+
+
 def test_single_particle_tsc_at_node():
     M = 8
     raw = 3 * 2 ** 32 // M  # exactly on node 3
@@ -373,13 +399,16 @@ def test_ewald_structure_factor_equals_direct():
 
 
 # ------------------------------------------------------------ whole kick
-def test_cpu_kick_meets_ewald_tolerance():
-    """The oracle's full TreePM kick (TSC) is within 1e-3 rms of FP64 Ewald."""
+@pytest.mark.parametrize("order,interlace", [(3, 0), (2, 1)])
+def test_cpu_kick_meets_ewald_tolerance(order, interlace):
+    """The oracle's full TreePM kick (TSC, or CIC with the interlaced
+    deposit) is within 1e-3 rms of FP64 Ewald."""
     n_side = 24
     n = n_side ** 3
     x, y, z = rand_u32(n, 51), rand_u32(n, 52), rand_u32(n, 53)
     rs = 1.25 / n_side
-    p = O.make_params(n_side, int(1 / (4.5 * rs)), 3, rs, 4.5 * rs, 2 / (25 * n_side), 1.0 / n, 0.1, n)
+    p = O.make_params(n_side, int(1 / (4.5 * rs)), order, rs, 4.5 * rs, 2 / (25 * n_side), 1.0 / n, 0.1, n, 1,
+                      interlace)
     zf = np.zeros(n, np.float32)
     r = O.kick_full(p, x, y, z, zf, zf, zf, np.zeros(n, np.int8))
     s = np.sort(np.random.default_rng(5).choice(n, 200, replace=False))

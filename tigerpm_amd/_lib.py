@@ -25,7 +25,7 @@ class TpmError(RuntimeError):
 
 class TpmParams(ctypes.Structure):
     _fields_ = [("mesh_dim", ctypes.c_int), ("chain_dim", ctypes.c_int), ("order", ctypes.c_int),
-                ("do_phi", ctypes.c_int), ("long_type", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("do_phi", ctypes.c_int), ("long_type", ctypes.c_int), ("interlace", ctypes.c_int),
                 ("rs", ctypes.c_double), ("rcut", ctypes.c_double),
                 ("hsoft", ctypes.c_double), ("GM", ctypes.c_double), ("eta", ctypes.c_double),
                 ("n_total", ctypes.c_double)]

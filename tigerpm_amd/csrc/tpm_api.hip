@@ -37,7 +37,7 @@ extern "C" tpm_status tpm_ctx_destroy(tpm_ctx* ctx) {
                     ctx->gx,    ctx->gy,      ctx->gz,       ctx->pot,       ctx->x2,       ctx->y2,    ctx->z2,
                     ctx->vx2,   ctx->vy2,     ctx->vz2,      ctx->rung2,     ctx->keys,     ctx->keys2, ctx->idx,
                     ctx->perm,  ctx->sort_tmp, ctx->bounds,  ctx->cell_begin, ctx->tile_cnt, ctx->tile_end,
-                    ctx->tiles, ctx->ntiles,  ctx->scan_tmp, ctx->rho_q,     ctx->rho,      ctx->rho_hat,
+                    ctx->tiles, ctx->ntiles,  ctx->scan_tmp, ctx->rho_q,     ctx->rho,      ctx->rho_hat, ctx->rho_hat1,
                     ctx->phi,   ctx->green,   ctx->counters, ctx->dscratch};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -69,7 +69,7 @@ extern "C" tpm_status tpm_set_params(tpm_ctx* ctx, const tpm_params* p) {
     if (q.order < 2 || q.order > 4) return tpm_fail(ctx, TPM_ERR_INVALID, "order must be 2 (CIC), 3 (TSC) or 4");
     if (q.long_type != TPM_LONG_PM && q.long_type != TPM_LONG_PME)
         return tpm_fail(ctx, TPM_ERR_INVALID, "long_type must be TPM_LONG_PM or TPM_LONG_PME");
-    q.pad_ = 0;
+    if (q.interlace != 0 && q.interlace != 1) return tpm_fail(ctx, TPM_ERR_INVALID, "interlace must be 0 or 1");
     if (!(q.rs > 0.0) || !(q.rcut > 0.0) || !(q.hsoft > 0.0) || !(q.rcut < 0.5))
         return tpm_fail(ctx, TPM_ERR_INVALID, "rs, rcut, hsoft must be positive and rcut < 1/2");
     if (!(q.eta > 0.0) || !std::isfinite(q.GM) || !(q.n_total >= 0.0))
@@ -80,7 +80,7 @@ extern "C" tpm_status tpm_set_params(tpm_ctx* ctx, const tpm_params* p) {
     if ((double)q.chain_dim * q.rcut > 1.0)
         return tpm_fail(ctx, TPM_ERR_INVALID, "chain cells smaller than rcut: the 27-cell stencil would miss pairs");
     if (ctx->params_set && (q.mesh_dim != ctx->p.mesh_dim || q.order != ctx->p.order || q.rs != ctx->p.rs ||
-                            q.long_type != ctx->p.long_type))
+                            q.long_type != ctx->p.long_type || q.interlace != ctx->p.interlace))
         ctx->pm_valid = false;
     ctx->p = q;
     ctx->C = q.chain_dim;

@@ -71,6 +71,8 @@ struct tpm_ctx {
     unsigned long long* rho_q = nullptr;  // Q32 fixed-point density
     float* rho = nullptr;                 // FFT real input
     float2* rho_hat = nullptr;            // M*M*(M/2+1)
+    float2* rho_hat1 = nullptr;           // transform of the half-cell-shifted deposit
+    int64_t rho_hat1_cap = 0;
     float* phi = nullptr;                 // potential mesh
     float* green = nullptr;
     int green_M = 0, green_order = 0;
@@ -166,9 +168,10 @@ __device__ __forceinline__ int tpm_chain_coord(uint32_t raw, int C, const uint32
 
 // One-axis integer mass-assignment weights (see the CPU restatement for the
 // derivation): CIC/TSC in Q21, cloud4 in Q16.  Returns the stencil width.
-__device__ __forceinline__ int tpm_weights_1d(int order, uint32_t raw, int M, int* base, uint64_t w[4],
-                                              int* qbits) {
-    const uint64_t P = (uint64_t)raw * (uint64_t)M;
+// shift = 2^31 deposits onto the grid displaced by half a cell (interlacing).
+__device__ __forceinline__ int tpm_weights_1d(int order, uint32_t raw, int M, uint64_t shift, int* base,
+                                              uint64_t w[4], int* qbits) {
+    const uint64_t P = (uint64_t)raw * (uint64_t)M + shift;
     if (order == 2) {
         const uint64_t t = (uint32_t)P >> 11;
         *base = (int)(P >> 32);

@@ -8,6 +8,10 @@
 // bit-exact and deterministic), the 3-D transforms run in rocFFT, and the
 // deposit factor 4 pi M, the PME filter, the assignment deconvolution and the
 // 1/M^3 normalisation live in one FP32 Green's table built in FP64.
+// Optionally (interlace) a second deposit onto the grid displaced by half a
+// cell is combined in k space, cancelling the odd aliases of the assignment.
+#include <algorithm>
+
 #include "tpm_internal.hpp"
 
 namespace {
@@ -23,36 +27,46 @@ __device__ __forceinline__ int wrapi(int i, int M) {
     return i;
 }
 
-// One thread per particle: 1-D integer weights per axis, 3-D products in
-// Q32 with the truncation deficit added to the centre node, 64-bit atomics.
+// 3-D integer weights of one particle: 1-D weights per axis, products in
+// Q32 with the truncation deficit added to the centre node, so that every
+// particle deposits exactly 2^32.
+template <int ORDER>
+__device__ __forceinline__ void weights_3d(uint32_t x, uint32_t y, uint32_t z, int M, uint64_t shift, int* bx,
+                                           int* by, int* bz, uint64_t w3[ORDER * ORDER * ORDER]) {
+    constexpr int NP = ORDER;
+    uint64_t wx[4], wy[4], wz[4];
+    int q;
+    tpm_weights_1d(ORDER, x, M, shift, bx, wx, &q);
+    tpm_weights_1d(ORDER, y, M, shift, by, wy, &q);
+    tpm_weights_1d(ORDER, z, M, shift, bz, wz, &q);
+    const int s = 3 * q - 32;
+    uint64_t sum = 0;
+#pragma unroll
+    for (int a = 0; a < NP; a++)
+#pragma unroll
+        for (int b = 0; b < NP; b++)
+#pragma unroll
+            for (int c = 0; c < NP; c++) {
+                const uint64_t v = (wx[a] * wy[b] * wz[c]) >> s;
+                w3[(a * NP + b) * NP + c] = v;
+                sum += v;
+            }
+    constexpr int centre = (ORDER == 2) ? 0 : ((1 * NP + 1) * NP + 1);
+    w3[centre] += (1ull << 32) - sum;
+}
+
+// One thread per particle, ORDER^3 global 64-bit atomics (any particle order).
 template <int ORDER>
 __global__ __launch_bounds__(256) void deposit_kernel(int64_t n, const uint32_t* __restrict__ x,
                                                       const uint32_t* __restrict__ y,
-                                                      const uint32_t* __restrict__ z, int M,
+                                                      const uint32_t* __restrict__ z, int M, uint64_t shift,
                                                       unsigned long long* __restrict__ grid) {
     constexpr int NP = ORDER;
     const int64_t M2 = (int64_t)M * M;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t wx[4], wy[4], wz[4];
-        int bx, by, bz, q;
-        tpm_weights_1d(ORDER, x[i], M, &bx, wx, &q);
-        tpm_weights_1d(ORDER, y[i], M, &by, wy, &q);
-        tpm_weights_1d(ORDER, z[i], M, &bz, wz, &q);
-        const int shift = 3 * q - 32;
+        int bx, by, bz;
         uint64_t w3[NP * NP * NP];
-        uint64_t sum = 0;
-#pragma unroll
-        for (int a = 0; a < NP; a++)
-#pragma unroll
-            for (int b = 0; b < NP; b++)
-#pragma unroll
-                for (int c = 0; c < NP; c++) {
-                    const uint64_t v = (wx[a] * wy[b] * wz[c]) >> shift;
-                    w3[(a * NP + b) * NP + c] = v;
-                    sum += v;
-                }
-        constexpr int centre = (ORDER == 2) ? 0 : ((1 * NP + 1) * NP + 1);
-        w3[centre] += (1ull << 32) - sum;
+        weights_3d<ORDER>(x[i], y[i], z[i], M, shift, &bx, &by, &bz, w3);
 #pragma unroll
         for (int a = 0; a < NP; a++) {
             const int64_t ix = wrapi(bx + a, M);
@@ -81,16 +95,15 @@ struct BlockedDepositArgs {
     const int32_t* __restrict__ cell_begin;
     const uint32_t* __restrict__ bounds;  // C split planes
     unsigned long long* __restrict__ grid;
+    uint64_t shift;
     int C, M, B, nb;                      // nb = ceil(C / B) blocks per axis
-    int wmax;                             // LDS footprint capacity per axis
 };
 
 // Mesh nodes [lo, lo + w) touched by particles of chain cells [c0, c1].
-__device__ __forceinline__ void block_nodes(const uint32_t* b, int C, int M, int c0, int c1, int* lo, int* w) {
-    const uint64_t xlo = b[c0];
-    const uint64_t xhi = (c1 + 1 < C ? (uint64_t)b[c1 + 1] : (1ull << 32)) - 1;
-    const int l = (int)((xlo * (uint64_t)M) >> 32) - 1;
-    const int h = (int)((xhi * (uint64_t)M) >> 32) + 2;
+__host__ __device__ __forceinline__ void block_nodes(uint64_t xlo, uint64_t xhi_incl, int M, uint64_t shift, int* lo,
+                                                     int* w) {
+    const int l = (int)((xlo * (uint64_t)M + shift) >> 32) - 1;
+    const int h = (int)((xhi_incl * (uint64_t)M + shift) >> 32) + 2;
     *lo = l;
     *w = h - l + 1;
 }
@@ -102,42 +115,29 @@ __global__ __launch_bounds__(256) void deposit_blocked_kernel(BlockedDepositArgs
     const int C = a.C, M = a.M, B = a.B, nb = a.nb;
     const int bid = blockIdx.x;
     const int bx = bid % nb, by = (bid / nb) % nb, bz = bid / (nb * nb);
-    const int c0x = bx * B, c0y = by * B, c0z = bz * B;
-    const int c1x = min(c0x + B, C) - 1, c1y = min(c0y + B, C) - 1, c1z = min(c0z + B, C) - 1;
-    int lox, loy, loz, wx, wy, wz;
-    block_nodes(a.bounds, C, M, c0x, c1x, &lox, &wx);
-    block_nodes(a.bounds, C, M, c0y, c1y, &loy, &wy);
-    block_nodes(a.bounds, C, M, c0z, c1z, &loz, &wz);
-    const int wtot = wx * wy * wz;
+    const int c0[3] = {bx * B, by * B, bz * B};
+    int lo[3], w[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int c1 = min(c0[d] + B, C) - 1;
+        const uint64_t xhi = (c1 + 1 < C ? (uint64_t)a.bounds[c1 + 1] : (1ull << 32)) - 1;
+        block_nodes(a.bounds[c0[d]], xhi, M, a.shift, &lo[d], &w[d]);
+    }
+    const int c1x = min(c0[0] + B, C) - 1, c1y = min(c0[1] + B, C) - 1, c1z = min(c0[2] + B, C) - 1;
+    const int wy = w[1], wz = w[2];
+    const int wtot = w[0] * wy * wz;
     for (int l = threadIdx.x; l < wtot; l += blockDim.x) sg[l] = 0ull;
     __syncthreads();
-    for (int cz = c0z; cz <= c1z; cz++) {
-        for (int cy = c0y; cy <= c1y; cy++) {
+    for (int cz = c0[2]; cz <= c1z; cz++) {
+        for (int cy = c0[1]; cy <= c1y; cy++) {
             // cells c0x..c1x of one (cy, cz) row are contiguous in memory
-            const int rb = a.cell_begin[(cz * C + cy) * C + c0x];
+            const int rb = a.cell_begin[(cz * C + cy) * C + c0[0]];
             const int re = a.cell_begin[(cz * C + cy) * C + c1x + 1];
             for (int i = rb + threadIdx.x; i < re; i += blockDim.x) {
-                uint64_t wxv[4], wyv[4], wzv[4];
-                int bxn, byn, bzn, q;
-                tpm_weights_1d(ORDER, a.x[i], M, &bxn, wxv, &q);
-                tpm_weights_1d(ORDER, a.y[i], M, &byn, wyv, &q);
-                tpm_weights_1d(ORDER, a.z[i], M, &bzn, wzv, &q);
-                const int shift = 3 * q - 32;
+                int bxn, byn, bzn;
                 uint64_t w3[NP * NP * NP];
-                uint64_t sum = 0;
-#pragma unroll
-                for (int p = 0; p < NP; p++)
-#pragma unroll
-                    for (int r = 0; r < NP; r++)
-#pragma unroll
-                        for (int s = 0; s < NP; s++) {
-                            const uint64_t v = (wxv[p] * wyv[r] * wzv[s]) >> shift;
-                            w3[(p * NP + r) * NP + s] = v;
-                            sum += v;
-                        }
-                constexpr int centre = (ORDER == 2) ? 0 : ((1 * NP + 1) * NP + 1);
-                w3[centre] += (1ull << 32) - sum;
-                const int ox = bxn - lox, oy = byn - loy, oz = bzn - loz;
+                weights_3d<ORDER>(a.x[i], a.y[i], a.z[i], M, a.shift, &bxn, &byn, &bzn, w3);
+                const int ox = bxn - lo[0], oy = byn - lo[1], oz = bzn - lo[2];
 #pragma unroll
                 for (int p = 0; p < NP; p++)
 #pragma unroll
@@ -154,10 +154,9 @@ __global__ __launch_bounds__(256) void deposit_blocked_kernel(BlockedDepositArgs
     for (int l = threadIdx.x; l < wtot; l += blockDim.x) {
         const unsigned long long v = sg[l];
         if (v == 0ull) continue;
-        int ix = lox + l / (wy * wz), iy = loy + (l / wz) % wy, iz = loz + l % wz;
-        ix = wrapi(ix, M);
-        iy = wrapi(iy, M);
-        iz = wrapi(iz, M);
+        const int ix = wrapi(lo[0] + l / (wy * wz), M);
+        const int iy = wrapi(lo[1] + (l / wz) % wy, M);
+        const int iz = wrapi(lo[2] + l % wz, M);
         atomicAdd(&a.grid[ix * M2 + (int64_t)iy * M + iz], v);
     }
 }
@@ -202,6 +201,29 @@ __global__ void green_apply_kernel(int64_t n, float2* __restrict__ c, const floa
         v.x *= gg;
         v.y *= gg;
         c[i] = v;
+    }
+}
+
+// Interlaced combination fused with the Green's multiply:
+// c0 <- G (c0 + exp(i (kx + ky + kz) / 2) c1) / 2.
+__global__ void interlace_green_kernel(int M, float2* __restrict__ c0, const float2* __restrict__ c1,
+                                       const float* __restrict__ g) {
+    const int Mh = M / 2 + 1;
+    const int64_t total = (int64_t)M * M * Mh;
+    const float cph = (float)(M_PI / M);  // half of 2 pi / M
+    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < total; l += (int64_t)gridDim.x * blockDim.x) {
+        const int iz = (int)(l % Mh);
+        const int iy = (int)((l / Mh) % M);
+        const int ix = (int)(l / ((int64_t)Mh * M));
+        const int ksum = (2 * ix < M ? ix : ix - M) + (2 * iy < M ? iy : iy - M) + (2 * iz < M ? iz : iz - M);
+        float sn, cs;
+        sincosf(cph * (float)ksum, &sn, &cs);
+        const float2 a = c0[l], b = c1[l];
+        const float gg = 0.5f * g[l];
+        float2 v;
+        v.x = (a.x + cs * b.x - sn * b.y) * gg;
+        v.y = (a.y + sn * b.x + cs * b.y) * gg;
+        c0[l] = v;
     }
 }
 
@@ -272,11 +294,12 @@ tpm_status tpm_ensure_mesh(tpm_ctx* ctx, int M) {
     const int64_t M3 = (int64_t)M * M * M;
     const int64_t MC = (int64_t)M * M * (M / 2 + 1);
     if (M3 > ctx->mesh_cap) {
-        if (ctx->rho_q) TPM_HIP(ctx, hipFree(ctx->rho_q));
-        if (ctx->rho) TPM_HIP(ctx, hipFree(ctx->rho));
-        if (ctx->rho_hat) TPM_HIP(ctx, hipFree(ctx->rho_hat));
-        if (ctx->phi) TPM_HIP(ctx, hipFree(ctx->phi));
-        if (ctx->green) TPM_HIP(ctx, hipFree(ctx->green));
+        void** bufs[] = {(void**)&ctx->rho_q, (void**)&ctx->rho, (void**)&ctx->rho_hat, (void**)&ctx->rho_hat1,
+                         (void**)&ctx->phi, (void**)&ctx->green};
+        for (void** b : bufs) {
+            if (*b) TPM_HIP(ctx, hipFree(*b));
+            *b = nullptr;
+        }
         TPM_HIP(ctx, hipMalloc(&ctx->rho_q, sizeof(unsigned long long) * M3));
         TPM_HIP(ctx, hipMalloc(&ctx->rho, sizeof(float) * M3));
         TPM_HIP(ctx, hipMalloc(&ctx->rho_hat, sizeof(float2) * MC));
@@ -288,22 +311,12 @@ tpm_status tpm_ensure_mesh(tpm_ctx* ctx, int M) {
     return tpm_fft_plan(ctx, M);
 }
 
-tpm_status tpm_pm_solve(tpm_ctx* ctx) {
-    if (!ctx->params_set) return tpm_fail(ctx, TPM_ERR_STATE, "tpm_gravity_long_compute: parameters not set");
+// Integer deposit (grid displaced by shift / 2^32 cells) into rho_q and its
+// float image into rho.
+static tpm_status deposit_density(tpm_ctx* ctx, uint64_t shift) {
     const int M = ctx->M;
     const int64_t M3 = (int64_t)M * M * M;
-    const int64_t MC = (int64_t)M * M * (M / 2 + 1);
     hipStream_t s = ctx->stream;
-    TPM_TRY(tpm_ensure_mesh(ctx, M));
-    // GRAVITY_LONG_PM has no Gaussian filter: the table with rs = 0.
-    const double rs_eff = ctx->p.long_type == TPM_LONG_PM ? 0.0 : ctx->p.rs;
-    if (ctx->green_M != M || ctx->green_rs != rs_eff || ctx->green_order != ctx->p.order) {
-        green_kernel<<<grid_for(MC, 256), 256, 0, s>>>(M, rs_eff, ctx->p.order, ctx->green);
-        TPM_HIP(ctx, hipGetLastError());
-        ctx->green_M = M;
-        ctx->green_rs = rs_eff;
-        ctx->green_order = ctx->p.order;
-    }
     TPM_TRY(tpm_fill_u64(ctx, ctx->rho_q, M3));
     // Blocked deposit needs particles in chain-mesh order; pick the largest
     // block edge (cells) whose LDS footprint fits 64 KiB.
@@ -319,9 +332,9 @@ tpm_status tpm_pm_solve(tpm_ctx* ctx) {
             int wm = 0;
             for (int c0 = 0; c0 < C; c0 += cand) {
                 const int c1 = std::min(c0 + cand, C) - 1;
-                const int lo = (int)((b[c0] * (uint64_t)M) >> 32) - 1;
-                const int hi = (int)(((b[c1 + 1] - 1) * (uint64_t)M) >> 32) + 2;
-                wm = std::max(wm, hi - lo + 1);
+                int lo, w;
+                block_nodes(b[c0], b[c1 + 1] - 1, M, shift, &lo, &w);
+                wm = std::max(wm, w);
             }
             if ((int64_t)wm * wm * wm * 8 <= 65536) {
                 B = cand;
@@ -338,11 +351,11 @@ tpm_status tpm_pm_solve(tpm_ctx* ctx) {
         da.cell_begin = ctx->cell_begin;
         da.bounds = ctx->bounds;
         da.grid = ctx->rho_q;
+        da.shift = shift;
         da.C = ctx->C;
         da.M = M;
         da.B = B;
         da.nb = (ctx->C + B - 1) / B;
-        da.wmax = wmax;
         const unsigned nblk = (unsigned)((int64_t)da.nb * da.nb * da.nb);
         const size_t lds = sizeof(unsigned long long) * (size_t)wmax * wmax * wmax;
         switch (ctx->p.order) {
@@ -354,16 +367,46 @@ tpm_status tpm_pm_solve(tpm_ctx* ctx) {
     } else if (ctx->n > 0) {
         const unsigned g = grid_for(ctx->n, 256);
         switch (ctx->p.order) {
-            case 2: deposit_kernel<2><<<g, 256, 0, s>>>(ctx->n, ctx->x, ctx->y, ctx->z, M, ctx->rho_q); break;
-            case 3: deposit_kernel<3><<<g, 256, 0, s>>>(ctx->n, ctx->x, ctx->y, ctx->z, M, ctx->rho_q); break;
-            default: deposit_kernel<4><<<g, 256, 0, s>>>(ctx->n, ctx->x, ctx->y, ctx->z, M, ctx->rho_q); break;
+            case 2: deposit_kernel<2><<<g, 256, 0, s>>>(ctx->n, ctx->x, ctx->y, ctx->z, M, shift, ctx->rho_q); break;
+            case 3: deposit_kernel<3><<<g, 256, 0, s>>>(ctx->n, ctx->x, ctx->y, ctx->z, M, shift, ctx->rho_q); break;
+            default: deposit_kernel<4><<<g, 256, 0, s>>>(ctx->n, ctx->x, ctx->y, ctx->z, M, shift, ctx->rho_q); break;
         }
         TPM_HIP(ctx, hipGetLastError());
     }
     q32_to_float_kernel<<<grid_for(M3, 256), 256, 0, s>>>(M3, ctx->rho_q, ctx->rho);
     TPM_HIP(ctx, hipGetLastError());
+    return TPM_OK;
+}
+
+tpm_status tpm_pm_solve(tpm_ctx* ctx) {
+    if (!ctx->params_set) return tpm_fail(ctx, TPM_ERR_STATE, "tpm_gravity_long_compute: parameters not set");
+    const int M = ctx->M;
+    const int64_t MC = (int64_t)M * M * (M / 2 + 1);
+    hipStream_t s = ctx->stream;
+    TPM_TRY(tpm_ensure_mesh(ctx, M));
+    // GRAVITY_LONG_PM has no Gaussian filter: the table with rs = 0.
+    const double rs_eff = ctx->p.long_type == TPM_LONG_PM ? 0.0 : ctx->p.rs;
+    if (ctx->green_M != M || ctx->green_rs != rs_eff || ctx->green_order != ctx->p.order) {
+        green_kernel<<<grid_for(MC, 256), 256, 0, s>>>(M, rs_eff, ctx->p.order, ctx->green);
+        TPM_HIP(ctx, hipGetLastError());
+        ctx->green_M = M;
+        ctx->green_rs = rs_eff;
+        ctx->green_order = ctx->p.order;
+    }
+    TPM_TRY(deposit_density(ctx, 0));
     TPM_TRY(tpm_fft_forward(ctx, ctx->rho, ctx->rho_hat));
-    green_apply_kernel<<<grid_for(MC, 256), 256, 0, s>>>(MC, ctx->rho_hat, ctx->green);
+    if (ctx->p.interlace) {
+        if (!ctx->rho_hat1 || ctx->rho_hat1_cap < MC) {
+            if (ctx->rho_hat1) TPM_HIP(ctx, hipFree(ctx->rho_hat1));
+            TPM_HIP(ctx, hipMalloc(&ctx->rho_hat1, sizeof(float2) * MC));
+            ctx->rho_hat1_cap = MC;
+        }
+        TPM_TRY(deposit_density(ctx, 1ull << 31));
+        TPM_TRY(tpm_fft_forward(ctx, ctx->rho, ctx->rho_hat1));
+        interlace_green_kernel<<<grid_for(MC, 256), 256, 0, s>>>(M, ctx->rho_hat, ctx->rho_hat1, ctx->green);
+    } else {
+        green_apply_kernel<<<grid_for(MC, 256), 256, 0, s>>>(MC, ctx->rho_hat, ctx->green);
+    }
     TPM_HIP(ctx, hipGetLastError());
     TPM_TRY(tpm_fft_inverse(ctx, ctx->rho_hat, ctx->phi));
     ctx->pm_valid = true;

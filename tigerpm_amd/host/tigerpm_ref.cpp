@@ -76,6 +76,7 @@ tpm_params device_params(int long_type) {
     p.order = o.order;
     p.do_phi = 1;
     p.long_type = long_type;
+    p.interlace = o.interlace;
     p.rs = o.rs;
     p.rcut = o.rcut;
     p.hsoft = o.hsoft;
@@ -152,6 +153,7 @@ bool process_options(int argc, char* argv[]) {
         else if (k == "rms_disp") o.rms_disp = std::stod(v);
         else if (k == "seed") o.seed = std::stoull(v);
         else if (k == "tile_sinks") o.tile_sinks = std::stoi(v);
+        else if (k == "interlace") o.interlace = std::stoi(v);
         else {
             std::fprintf(stderr, "unrecognised option %s\n", s.c_str());
             return false;

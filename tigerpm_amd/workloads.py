@@ -15,19 +15,22 @@ import math
 from ._lib import TpmParams
 
 WORKLOADS = {
-    # name: (n_side, mesh, order, initial conditions, rms displacement [cells], ewald samples)
-    "uniform64_cic": dict(n_side=64, mesh=64, order=2, ic="uniform", rms=0.0, samples=1024, seed=1),
-    "zeldovich256": dict(n_side=256, mesh=256, order=3, ic="zeldovich", rms=0.5, samples=4096, seed=2),
-    "zeldovich512_tsc": dict(n_side=512, mesh=512, order=3, ic="zeldovich", rms=2.0, samples=8192, seed=3),
+    # n_side^3 particles, mesh^3 PM mesh, assignment order, initial conditions,
+    # rms displacement [cells], Ewald samples, seed, interlaced deposit
+    "uniform64_cic": dict(n_side=64, mesh=64, order=2, ic="uniform", rms=0.0, samples=1024, seed=1, interlace=1),
+    "zeldovich256": dict(n_side=256, mesh=256, order=3, ic="zeldovich", rms=0.5, samples=4096, seed=2,
+                         interlace=1),
+    "zeldovich512_tsc": dict(n_side=512, mesh=512, order=3, ic="zeldovich", rms=2.0, samples=8192, seed=3,
+                             interlace=0),
 }
 
 
-def baseline_params(n_side, mesh, order, n_total=None):
+def baseline_params(n_side, mesh, order, n_total=None, interlace=0):
     n = n_total if n_total is not None else n_side ** 3
     rs = 1.25 / mesh
     rcut = 4.5 * rs
     return TpmParams(mesh_dim=int(mesh), chain_dim=int(math.floor(1.0 / rcut)), order=int(order), do_phi=1,
-                     long_type=1,
+                     long_type=1, interlace=int(interlace),
                      rs=rs, rcut=rcut, hsoft=2.0 / (25.0 * n_side), GM=1.0 / n, eta=0.2 / math.sqrt(2.0),
                      n_total=float(n))
 
