@@ -12,6 +12,13 @@ flops = 44 per candidate pair of the 27-cell chain-mesh stencil, divided by
 the kernel's HIP-event duration on its stream), the CPU restatement timed on
 this host's cores on a bounded sample of the same box, and the rms force
 error against the FP64 Ewald sum on sampled particles.
+
+Two more legs (keys "ensemble" and "serving" of the same JSON line):
+64 independent boxes of N log-uniform in [32^3, 128^3] kicked by the plain
+per-box loop and by one batch (autotuned, graph-captured), and a
+continuous-batching server under Poisson arrivals of boxes with N
+log-uniform in [32^3, 256^3], 10 resident kick + drift steps each, with
+p50/p99 latency per arrival rate.
 """
 import argparse
 import json
@@ -127,6 +134,155 @@ def accuracy(ctx, p, nsamp, seed):
             "potential_rms": rel_rms(P, ph * p.GM), "ewald_seconds": t_ew}
 
 
+def ewald_error(ctx, d, nsamp, seed):
+    """Vector rms force error of downloaded state d against the FP64 Ewald
+    sum of the same particles (resident in ctx)."""
+    rng = np.random.default_rng(seed)
+    s = np.sort(rng.choice(len(d["x"]), min(nsamp, len(d["x"])), replace=False))
+    ph, gx, gy, gz = ctx.ewald(s)
+    G = np.stack([d["gx"][s], d["gy"][s], d["gz"][s]], 1).astype(np.float64)
+    E = np.stack([gx, gy, gz], 1) * ctx.params.GM
+    return rel_rms(G, E)
+
+
+def ensemble_leg(T, steps, seed=1234, nbox=64):
+    """64 independent boxes, N log-uniform in [32^3, 128^3] (n_side
+    log-uniform, mesh = n_side), rms displacement U[0.3, 2] cells, own seeds;
+    interlaced TSC.  The same kicks through (a) the plain per-box drop-in
+    loop -- one resident context per box, tpm_kick_treepm and a wait per box
+    -- and (b) one tpm_batch: autotuned launch shape, the step captured as a
+    HIP graph and replayed."""
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(np.log(32), np.log(128), nbox))).astype(int)
+    rms = rng.uniform(0.3, 2.0, nbox)
+    total = int((ns.astype(np.int64) ** 3).sum())
+    specs = [T.box_spec(T.baseline_params(int(n), int(n), 3, interlace=1), t0=1.0) for n in ns]
+    bt = T.Batch(total, total, nbox, 3, 1)
+    t = time.perf_counter()
+    ids = [bt.add_zeldovich(sp, int(n), 1000 + k, float(r)) for k, (sp, n, r) in enumerate(zip(specs, ns, rms))]
+    bt.sync()
+    log("ensemble: %d boxes, %d particles, initial conditions %.2f s" % (nbox, total, time.perf_counter() - t))
+    ctxs = []
+    for i in ids:
+        c = T.Context(0)
+        bt.to_ctx(i, c)
+        ctxs.append(c)
+    # (a) the plain loop
+    for c in ctxs:
+        c.kick_treepm(0, 1.0, 1.0, True, stats=False)
+        c.sync()
+    t = time.perf_counter()
+    for k in range(steps):
+        for c in ctxs:
+            c.kick_treepm(0, 1.0, 1.0, False, stats=False)
+            c.sync()
+    plain = total * steps / (time.perf_counter() - t)
+    t = time.perf_counter()
+    for k in range(steps):
+        for c in ctxs:
+            c.kick_treepm(0, 1.0, 1.0, False, stats=False)
+    for c in ctxs:
+        c.sync()
+    plain_async = total * steps / (time.perf_counter() - t)
+    # the drop-in boundary with host-resident boxes: upload, kick, download
+    host = [c.download() for c in ctxs]
+    t = time.perf_counter()
+    for c, h in zip(ctxs, host):
+        c.upload(h["x"], h["y"], h["z"], h["vx"], h["vy"], h["vz"], h["rung"])
+        c.kick_treepm(0, 1.0, 1.0, False, stats=False)
+        c.download()
+    plain_host = total / (time.perf_counter() - t)
+    del host
+    for c in ctxs:
+        c.close()
+    log("ensemble plain loop: %.4g kicks/s (wait per box), %.4g (no wait), host-resident %.4g"
+        % (plain, plain_async, plain_host))
+    # (b) the batch
+    t = time.perf_counter()
+    best, ms_best = bt.autotune(reps=2)
+    t_tune = time.perf_counter() - t
+    best["graph"] = 1
+    bt.set_tuning(**best)
+    bt.step(0, stats=False)
+    bt.sync()
+    t = time.perf_counter()
+    for k in range(steps):
+        bt.step(0, stats=False)
+    bt.sync()
+    batch = total * steps / (time.perf_counter() - t)
+    lat = []
+    for k in range(steps):
+        t = time.perf_counter()
+        bt.step(0, stats=False)
+        bt.sync()
+        lat.append(time.perf_counter() - t)
+    st = bt.step(0, stats=True)
+    log("ensemble batch: %.4g kicks/s, tuning %s (autotune %.1f s), step %.1f ms"
+        % (batch, best, t_tune, st.ms_step))
+    # accuracy of four boxes against FP64 Ewald
+    order = np.argsort(ns)
+    errs = {}
+    for k in (order[0], order[nbox // 3], order[2 * nbox // 3], order[-1]):
+        with T.Context(0) as c:
+            bt.to_ctx(ids[k], c)
+            errs["n_side_%d" % ns[k]] = ewald_error(c, bt.download(ids[k]), 512, int(k))
+    bt.close()
+    log("ensemble accuracy", errs)
+    return {
+        "boxes": nbox, "n_particles": total, "n_side_min": int(ns.min()), "n_side_max": int(ns.max()),
+        "assignment": "TSC interlaced", "steps": steps,
+        "plain_loop_kicks_per_s": plain, "plain_loop_no_wait_kicks_per_s": plain_async,
+        "plain_loop_host_resident_kicks_per_s": plain_host,
+        "batch_kicks_per_s": batch, "speedup_vs_plain_loop": batch / plain,
+        "speedup_vs_plain_loop_no_wait": batch / plain_async,
+        "speedup_vs_host_resident_loop": batch / plain_host,
+        "tuning": best, "autotune_s": t_tune, "step_ms_p50": 1e3 * float(np.percentile(lat, 50)),
+        "step_ms_p99": 1e3 * float(np.percentile(lat, 99)),
+        "candidate_pairs": st.candidate_pairs, "executed_pairs": st.executed_pairs,
+        "pp_credited_tflops": FLOPS_PER_PAIR * st.candidate_pairs / (st.ms_step * 1e-3) / 1e12,
+        "force_rms_vector_vs_ewald": errs,
+    }
+
+
+def serving_leg(T, rates, count, budget_gib, seed=77):
+    """Poisson arrivals at each rate, N log-uniform in [32^3, 256^3], 10
+    kick + drift steps per box with the state resident, admission by the
+    arena budget; latency = arrival to the end of the box's last step."""
+    from tigerpm_amd import serving as S
+    out = {"budget_gib": budget_gib, "kicks_per_request": 10, "rates": []}
+    lim = S.budget_limits(budget_gib * (1 << 30))
+    for rate in rates:
+        reqs = S.poisson_requests(count, rate, seed + int(rate * 10))
+        with T.Batch(**lim) as bt:
+            srv = S.Server(bt, kicks=10)
+            t_prep = srv.prepare(reqs)
+            r = srv.run(reqs)
+        r.pop("records")
+        r["rate_per_s"] = rate
+        r["prepare_s"] = t_prep
+        log("serving rate %g/s: p50 %.3f s, p99 %.3f s, %.4g kicks/s, mean resident %.1f"
+            % (rate, r["latency_p50_s"], r["latency_p99_s"], r["kicks_per_s"], r["mean_resident_boxes"]))
+        out["rates"].append(r)
+    # accuracy after 10 steps for three requests served alone: forces at the
+    # final positions (one more step with the kick and drift off) vs Ewald
+    reqs = S.poisson_requests(count, rates[0], seed)
+    picks = sorted(reqs, key=lambda q: q.n_side)
+    errs = {}
+    for q in (picks[0], picks[len(picks) // 2], picks[-1]):
+        with T.Batch(**lim) as bt:
+            sp = S.request_spec(q)
+            box = bt.add_zeldovich(sp, q.n_side, q.seed, q.rms)
+            for k in range(10):
+                bt.step(T.TPM_BATCH_DRIFT, stats=False)
+            bt.step(T.TPM_SKIP_KICK, stats=False)
+            with T.Context(0) as c:
+                bt.to_ctx(box, c)
+                errs["n_side_%d" % q.n_side] = ewald_error(c, bt.download(box), 256, q.rid)
+    out["force_rms_vector_vs_ewald_after_10_steps"] = errs
+    log("serving accuracy", errs)
+    return out
+
+
 def main():
     import tigerpm_amd as T
     from tigerpm_amd.workloads import WORKLOADS
@@ -140,6 +296,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ewald", action="store_true")
     ap.add_argument("--no-pcie", action="store_true", help="skip the host-buffer (PCIe-inclusive) measurement")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-box batched-ensemble leg")
+    ap.add_argument("--no-serving", action="store_true", help="skip the continuous-batching serving leg")
+    ap.add_argument("--serving-rates", default="4,8,12", help="Poisson arrival rates (requests/s)")
+    ap.add_argument("--serving-requests", type=int, default=40)
+    ap.add_argument("--serving-budget-gib", type=float, default=16.0)
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -239,6 +400,11 @@ def main():
         out["cpu_baseline"] = cpu_baseline(ctx, p, args.cpu_budget, w["seed"])
         log("cpu", out["cpu_baseline"])
     ctx.close()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(T, max(args.steps, 3))
+    if not args.no_serving:
+        rates = [float(r) for r in args.serving_rates.split(",") if r]
+        out["serving"] = serving_leg(T, rates, args.serving_requests, args.serving_budget_gib)
     if args.probe_copy:
         out["copy_probe_bytes"] = copy_probe(1 << 30)
     print(json.dumps(out), flush=True)

@@ -189,6 +189,111 @@ TPM_API tpm_status tpm_ic_zeldovich(tpm_ctx* ctx, int n_side, uint64_t seed, dou
 /* n uniform random positions from a counter-based RNG (seeded). */
 TPM_API tpm_status tpm_ic_uniform(tpm_ctx* ctx, int64_t n, uint64_t seed);
 
+/* ---- Batches of independent boxes (ensemble and serving) ---------------
+ * No reference counterpart: the reference kicks one box per call and
+ * allocates per call (kick_treepm.cu:829, 1078-1082).  A batch keeps many
+ * periodic boxes of different N resident in fixed device arenas sized once
+ * (the HBM budget) and kicks all of them together: one chain-mesh sort, one
+ * deposit, one Green's multiply and one fused PP + interpolation + kick
+ * launch over the (box, tile) work items of every box, plus one pair of
+ * FFTs per box.  Every box gets the same result, bit for bit, as the same
+ * box kicked alone through tpm_kick_treepm with the same tile size.
+ * Boxes join (tpm_batch_add*) and leave (tpm_batch_remove) between steps. */
+typedef struct tpm_batch tpm_batch;
+
+/* Arena capacities, fixed at creation. */
+typedef struct {
+    int64_t max_particles;  /* all resident boxes together */
+    int64_t max_mesh_nodes; /* sum of mesh_dim^3 over resident boxes */
+    int64_t max_cells;      /* sum of chain_dim^3; 0 = max_mesh_nodes / 64 + 4096 */
+    int32_t max_boxes;
+    int32_t order;          /* assignment order shared by the boxes */
+    int32_t interlace;      /* shared interlace setting */
+    int32_t pad_;
+} tpm_batch_limits;
+
+/* One box: its parameters and its time stepping.  A step kicks every
+ * particle of the box with rung >= min_rung (kick epilogue with step t0,
+ * first_call on the box's first step) and then, with TPM_BATCH_DRIFT, moves
+ * every particle by v * t0 (a drift step, which the reference does not have:
+ * repeated steps form a kick-drift-kick leapfrog). */
+typedef struct {
+    tpm_params params;
+    double t0;
+    double scale;
+    int32_t min_rung;
+    int32_t pad_;
+} tpm_box_spec;
+
+/* Launch shape of the batched step (results do not depend on it beyond FP32
+ * summation order):
+ *   tile_sinks:    16, 32 or 64 sinks per PP work item;
+ *   deposit_block: chain cells per deposit block edge, 0 = auto;
+ *   sched:         0 = one workgroup per tile (XCD-aware order),
+ *                  1 = persistent workgroups taking tiles from per-XCD queues;
+ *   graph:         1 = capture the step in a HIP graph and replay it while
+ *                  the set of boxes and the tuning stay the same. */
+typedef struct {
+    int tile_sinks;
+    int deposit_block;
+    int sched;
+    int graph;
+} tpm_batch_tuning;
+
+typedef struct {
+    double candidate_pairs; /* W_sr summed over the boxes */
+    double executed_pairs;
+    double ms_step;         /* HIP-event time of the step on the batch stream */
+    int64_t n_particles;
+    int32_t n_boxes;
+    int32_t max_rung;
+} tpm_batch_stats;
+
+#define TPM_BATCH_DRIFT 8u  /* tpm_batch_step flag: drift after the kick */
+
+TPM_API tpm_status tpm_batch_create(int device, const tpm_batch_limits* limits, tpm_batch** batch);
+TPM_API tpm_status tpm_batch_destroy(tpm_batch* batch);
+TPM_API const char* tpm_batch_last_error(const tpm_batch* batch);
+/* Device bytes the arenas of these limits take. */
+TPM_API tpm_status tpm_batch_arena_bytes(const tpm_batch_limits* limits, int64_t* bytes);
+/* Whether a box of n particles fits the free arena space now (1) or not (0). */
+TPM_API tpm_status tpm_batch_fits(tpm_batch* batch, const tpm_box_spec* spec, int64_t n, int* fits);
+/* Adds a box from host arrays (vx, vy, vz, rung may be NULL) or from
+ * Zel'dovich initial conditions made on the device (as tpm_ic_zeldovich).
+ * TPM_ERR_BUDGET when it does not fit.  *id identifies the box. */
+TPM_API tpm_status tpm_batch_add(tpm_batch* batch, const tpm_box_spec* spec, int64_t n, const uint32_t* x,
+                                 const uint32_t* y, const uint32_t* z, const float* vx, const float* vy,
+                                 const float* vz, const int8_t* rung, int* id);
+TPM_API tpm_status tpm_batch_add_zeldovich(tpm_batch* batch, const tpm_box_spec* spec, int n_side, uint64_t seed,
+                                           double rms_disp, double spectral_index, int* id);
+/* Makes what a box of this spec (and Zel'dovich lattice n_side, 0 = none)
+ * needs before its first step -- the FFT plans of its mesh size and
+ * scratch space for its initial conditions -- so that a server can prepare
+ * the sizes it accepts before it takes requests. */
+TPM_API tpm_status tpm_batch_prepare(tpm_batch* batch, const tpm_box_spec* spec, int n_side);
+/* The box leaves; its arena space is reused after the next step. */
+TPM_API tpm_status tpm_batch_remove(tpm_batch* batch, int id);
+/* One step of every resident box (flags: TPM_SKIP_* as for tpm_kick, and
+ * TPM_BATCH_DRIFT).  Asynchronous unless stats is non-NULL. */
+TPM_API tpm_status tpm_batch_step(tpm_batch* batch, unsigned flags, tpm_batch_stats* stats);
+TPM_API tpm_status tpm_batch_sync(tpm_batch* batch);
+/* Particles and steps taken of a box; its arrays in the box's current order
+ * (chain-mesh order after a step).  Output pointers may be NULL. */
+TPM_API tpm_status tpm_batch_box_info(tpm_batch* batch, int id, int64_t* n, int* steps);
+TPM_API tpm_status tpm_batch_download(tpm_batch* batch, int id, uint32_t* x, uint32_t* y, uint32_t* z, float* vx,
+                                      float* vy, float* vz, int8_t* rung, float* gx, float* gy, float* gz,
+                                      float* pot);
+/* Copies a box's state into a context on the same device (parameters
+ * included), e.g. for tpm_ewald or to kick it alone. */
+TPM_API tpm_status tpm_batch_to_ctx(tpm_batch* batch, int id, tpm_ctx* ctx);
+TPM_API tpm_status tpm_batch_set_tuning(tpm_batch* batch, const tpm_batch_tuning* tuning);
+TPM_API tpm_status tpm_batch_get_tuning(const tpm_batch* batch, tpm_batch_tuning* tuning);
+/* Autotuner: times the step over the resident boxes for every tile size and
+ * schedule (then deposit block edge) with the kick epilogue and the drift
+ * off, so the boxes' state does not advance, keeps the fastest tuning and
+ * returns it with its step time.  reps = timed steps per candidate. */
+TPM_API tpm_status tpm_batch_autotune(tpm_batch* batch, int reps, tpm_batch_tuning* best, double* ms_best);
+
 #ifdef __cplusplus
 }
 #endif

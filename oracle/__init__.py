@@ -83,6 +83,7 @@ def lib():
             "orc_pp_sum": (c_dbl, [ctypes.POINTER(OrcParams), c_i64, u32p, u32p, u32p, i64p, c_i64, i64p,
                                    c_int, f32p, f32p, f32p, f32p]),
             "orc_candidate_pairs": (c_dbl, [c_int, i64p]),
+            "orc_drift": (None, [c_i64, u32p, u32p, u32p, f32p, f32p, f32p, c_dbl]),
             "orc_kick_epilogue": (None, [ctypes.POINTER(OrcParams), c_i64, f32p, f32p, f32p, f32p, f32p, f32p,
                                          f32p, i8p, c_int, c_dbl, c_dbl, c_int]),
             "orc_kick_full": (None, [ctypes.POINTER(OrcParams), c_i64, u32p, u32p, u32p, f32p, f32p, f32p, i8p,
@@ -242,6 +243,14 @@ def kick_epilogue(params, g, pot, v, rung, min_rung, scale, t0, first_call):
     lib().orc_kick_epilogue(ctypes.byref(params), len(gx), gx, gy, gz, pot, vx, vy, vz, rung, min_rung, scale,
                             t0, int(first_call))
     return (gx, gy, gz), pot, (vx, vy, vz), rung
+
+
+def drift(x, y, z, vx, vy, vz, t0):
+    """Fixed-point drift x + v t0 (periodic); returns new arrays."""
+    x, y, z = [np.array(a, np.uint32) for a in (x, y, z)]
+    vx, vy, vz = [np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
+    lib().orc_drift(len(x), x, y, z, vx, vy, vz, float(t0))
+    return x, y, z
 
 
 def kick_full(params, x, y, z, vx, vy, vz, rung, min_rung=0, scale=1.0, t0=1.0, first_call=True, fp64_pp=False):

@@ -863,6 +863,22 @@ ORC_API double orc_candidate_pairs(int C, const int64_t* cell_begin) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Drift step of the batched serving loop (no reference counterpart: the      */
+/* reference only kicks).  x <- x + v t0 in fixed32: the displacement v t0 in */
+/* box units, formed in double, is rounded to the nearest raw unit (llrint,   */
+/* ties to even) and added modulo 2^32, i.e. periodically.                    */
+/* ------------------------------------------------------------------------ */
+ORC_API void orc_drift(int64_t n, uint32_t* x, uint32_t* y, uint32_t* z, const float* vx, const float* vy,
+                       const float* vz, double t0) {
+    const double f = t0 * 4294967296.0;
+    for (int64_t i = 0; i < n; i++) {
+        x[i] += (uint32_t)(uint64_t)llrint((double)vx[i] * f);
+        y[i] += (uint32_t)(uint64_t)llrint((double)vy[i] * f);
+        z[i] += (uint32_t)(uint64_t)llrint((double)vz[i] * f);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* a12: kick epilogue (kick_treepm.cu:716-754; rung_dt fmmpm.cu:18-21)        */
 /* ------------------------------------------------------------------------ */
 

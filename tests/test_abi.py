@@ -69,3 +69,37 @@ def test_driver_fails_cleanly_without_gpu():
     drv = os.path.join(ROOT, "tigerpm_amd", "bin", "tigerpm")
     r = subprocess.run([drv, "--test=treepm", "--parts_dim=60"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "tpm_ctx_create" in r.stderr
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """sizeof / offsetof of every ABI struct, from the C header compiled by
+    gcc, equal the ctypes mirrors."""
+    structs = {"tpm_params": T.TpmParams, "tpm_stats": T.TpmStats, "tpm_tuning": T.TpmTuning,
+               "tpm_batch_limits": T.TpmBatchLimits, "tpm_box_spec": T.TpmBoxSpec,
+               "tpm_batch_tuning": T.TpmBatchTuning, "tpm_batch_stats": T.TpmBatchStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tigerpm_hip.h"', "int main(void) {"]
+    for c, py in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (c, c))
+        for f, _ in py._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (c, f, c, f))
+    lines.append("return 0; }")
+    src = tmp_path / "l.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "l"
+    subprocess.run(["gcc", "-std=c11", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(ln.split() for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                    check=True).stdout.splitlines())
+    import ctypes
+    for c, py in structs.items():
+        assert int(got[c]) == ctypes.sizeof(py), c
+        for f, _ in py._fields_:
+            assert int(got["%s.%s" % (c, f)]) == getattr(py, f).offset, (c, f)
+
+
+def test_batch_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    assert T.batch_arena_bytes(1 << 20, 1 << 20, 8, 3, 1) > 100 * (1 << 20)
+    with pytest.raises(T.TpmError):
+        T.Batch(1 << 16, 1 << 16, 4, 3, 0)

@@ -1,0 +1,184 @@
+"""GPU tests of the batched path (tpm_batch_*): every box of a batch gets the
+result it gets alone, bit for bit, whatever else is resident, in eager and in
+captured-graph mode; the drift matches the oracle exactly; admission follows
+the arena budget.  All calls go through the C ABI (tigerpm_amd.Batch)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import tigerpm_amd as T
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("x", "y", "z", "vx", "vy", "vz", "rung", "gx", "gy", "gz", "pot")
+T0 = 0.05
+
+
+def spec(n_side, order=3, interlace=1, t0=T0):
+    return T.box_spec(T.baseline_params(n_side, n_side, order, interlace=interlace), t0=t0)
+
+
+def new_batch(max_particles=400_000, max_mesh=400_000, boxes=16, order=3, interlace=1):
+    return T.Batch(max_particles, max_mesh, boxes, order, interlace)
+
+
+def assert_same(a, b, what):
+    for k in KEYS:
+        assert np.array_equal(a[k], b[k]), "%s: %s differs (max |diff| %s)" % (
+            what, k, np.max(np.abs(a[k].astype(np.float64) - b[k].astype(np.float64))))
+
+
+def test_batch_equals_plain_kick():
+    """A batch of boxes of different sizes and chain meshes: each box's two
+    steps equal tpm_kick_treepm on the same box, every array bit for bit."""
+    boxes = [(20, 11, 1.5), (24, 12, 0.4), (28, 13, 1.0), (32, 14, 2.0)]
+    with new_batch() as bt:
+        ids = [bt.add_zeldovich(spec(ns), ns, seed, rms) for ns, seed, rms in boxes]
+        ctxs = []
+        for i in ids:
+            c = T.Context(0)
+            bt.to_ctx(i, c)
+            ctxs.append(c)
+        for k in range(2):
+            st = bt.step(0)
+            assert st.n_boxes == len(boxes) and st.n_particles == sum(b[0] ** 3 for b in boxes)
+            assert 0 < st.executed_pairs <= st.candidate_pairs
+            for i, c in zip(ids, ctxs):
+                c.kick_treepm(min_rung=0, scale=1.0, t0=T0, first_call=(k == 0))
+                assert_same(bt.download(i), c.download(), "box %d step %d" % (i, k))
+                assert bt.box_info(i)[1] == k + 1
+        for c in ctxs:
+            c.close()
+
+
+def test_batch_drift_matches_oracle():
+    """Kick skipped, drift on: positions move by round(v t0 2^32) exactly, in
+    the chain-mesh order of the positions before the drift."""
+    rng = np.random.default_rng(5)
+    with new_batch(order=2, interlace=0) as bt:
+        ins = []
+        for ns in (18, 22):
+            n = ns ** 3
+            x, y, z = [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+            v = [(rng.standard_normal(n) * 3e-3).astype(np.float32) for _ in range(3)]
+            sp = spec(ns, order=2, interlace=0, t0=0.125)
+            ins.append((bt.add(sp, x, y, z, *v), sp, x, y, z, v))
+        bt.step(T.TPM_SKIP_KICK | T.TPM_BATCH_DRIFT)
+        for i, sp, x, y, z, v in ins:
+            perm, _ = O.chainmesh_sort(x, y, z, sp.params.chain_dim)
+            ex = O.drift(x[perm], y[perm], z[perm], v[0][perm], v[1][perm], v[2][perm], sp.t0)
+            d = bt.download(i)
+            for a, b in zip((d["x"], d["y"], d["z"]), ex):
+                assert np.array_equal(a, b)
+            assert np.array_equal(d["vx"], v[0][perm])
+            assert bt.box_info(i)[1] == 0  # no kick: the box did not advance
+
+
+def test_alone_equals_mix_with_graph_replays():
+    """Box A alone (eager) and A in a changing mix (captured graphs: boxes
+    join and leave, A moves in the arena; later steps replay both buffer
+    parities): after every kick + drift step A's state is the same bit for
+    bit."""
+    sa = spec(24)
+    steps = 7
+    alone = []
+    with new_batch() as bt:
+        a = bt.add_zeldovich(sa, 24, 7, 1.0)
+        for k in range(steps):
+            bt.step(T.TPM_BATCH_DRIFT, stats=False)
+            alone.append(bt.download(a))
+    with new_batch() as bt:
+        bt.set_tuning(tile_sinks=16, graph=1)
+        b = bt.add_zeldovich(spec(28), 28, 8, 0.5)
+        a = bt.add_zeldovich(sa, 24, 7, 1.0)
+        for k in range(steps):
+            if k == 1:
+                bt.add_zeldovich(spec(20), 20, 9, 1.5)
+            if k == 2:
+                bt.remove(b)
+            bt.step(T.TPM_BATCH_DRIFT, stats=False)
+            assert_same(bt.download(a), alone[k], "step %d" % k)
+        assert bt.box_info(a) == (24 ** 3, steps)
+
+
+@pytest.mark.parametrize("sched", [0, 1])
+def test_batch_kick_vs_ewald(sched):
+    """Interlaced TSC boxes kicked in a batch meet the 1e-3 rms force error
+    against the FP64 Ewald sum (both schedules of the PP launch)."""
+    with new_batch() as bt:
+        bt.set_tuning(tile_sinks=16, sched=sched)
+        ids = [bt.add_zeldovich(spec(32), 32, 21, 0.5), bt.add_zeldovich(spec(26), 26, 22, 1.5)]
+        bt.step(0)
+        for i in ids:
+            d = bt.download(i)
+            with T.Context(0) as c:
+                bt.to_ctx(i, c)
+                s = np.sort(np.random.default_rng(i).choice(len(d["x"]), 384, replace=False))
+                ph, gx, gy, gz = c.ewald(s)
+                G = np.stack([d["gx"][s], d["gy"][s], d["gz"][s]], 1).astype(np.float64)
+                E = np.stack([gx, gy, gz], 1) * c.params.GM
+                err = np.sqrt(((G - E) ** 2).sum() / (E ** 2).sum())
+                assert err < 1e-3, err
+
+
+def test_schedules_and_tiles_agree():
+    """Persistent and one-workgroup-per-tile launches give the same bits;
+    other tile sizes agree to FP32 summation order."""
+    ref = None
+    for ts, sched in ((16, 0), (16, 1), (32, 0), (64, 1)):
+        with new_batch() as bt:
+            bt.set_tuning(tile_sinks=ts, sched=sched)
+            i = bt.add_zeldovich(spec(30), 30, 31, 1.2)
+            bt.add_zeldovich(spec(22), 22, 32, 0.8)
+            bt.step(0)
+            d = bt.download(i)
+        if ref is None:
+            ref = d
+            continue
+        assert np.array_equal(d["x"], ref["x"])
+        G = np.stack([d["gx"], d["gy"], d["gz"]], 1).astype(np.float64)
+        R = np.stack([ref["gx"], ref["gy"], ref["gz"]], 1).astype(np.float64)
+        err = np.sqrt(((G - R) ** 2).sum() / (R ** 2).sum())
+        if ts == 16:
+            assert err == 0.0
+        else:
+            assert err < 1e-6, err
+
+
+def test_autotune_leaves_state():
+    with new_batch() as bt:
+        ids = [bt.add_zeldovich(spec(ns), ns, 40 + ns, 1.0) for ns in (20, 26, 30)]
+        bt.step(0)
+        before = [bt.download(i) for i in ids]
+        best, ms = bt.autotune(reps=1)
+        assert best["tile_sinks"] in (16, 32, 64) and best["sched"] in (0, 1) and ms > 0
+        assert bt.get_tuning() == best
+        for i, b in zip(ids, before):
+            d = bt.download(i)
+            for k in ("x", "y", "z", "vx", "vy", "vz", "rung"):
+                assert np.array_equal(d[k], b[k])
+            assert bt.box_info(i)[1] == 1
+
+
+def test_admission_by_budget():
+    n24 = 24 ** 3
+    with new_batch(max_particles=2 * n24 + 100, max_mesh=3 * n24, boxes=4) as bt:
+        a = bt.add_zeldovich(spec(24), 24, 1, 1.0)
+        b = bt.add_zeldovich(spec(24), 24, 2, 1.0)
+        assert not bt.fits(spec(24), n24)
+        with pytest.raises(T.TpmError) as e:
+            bt.add_zeldovich(spec(24), 24, 3, 1.0)
+        assert e.value.status == -5
+        bt.step(0)
+        bt.remove(a)
+        assert bt.fits(spec(24), n24)
+        c = bt.add_zeldovich(spec(24), 24, 3, 1.0)  # compacts the arena first
+        st = bt.step(0)
+        assert st.n_boxes == 2 and st.n_particles == 2 * n24
+        assert bt.box_info(b)[1] == 2 and bt.box_info(c)[1] == 1
+        with pytest.raises(T.TpmError):
+            bt.box_info(a)
+        u = np.arange(50, dtype=np.uint32) * np.uint32(85899345)
+        with pytest.raises(T.TpmError) as e:
+            bt.add(spec(18, order=2), u, u, u)  # order differs from the batch's
+        assert e.value.status == -2

@@ -415,3 +415,38 @@ def test_cpu_kick_meets_ewald_tolerance(order, interlace):
     phi, gx, gy, gz = O.ewald_sf(r["x"], r["y"], r["z"], s, p.hsoft, 9.6)
     G = np.stack([r["gx"][s], r["gy"][s], r["gz"][s]], 1)
     assert rel_rms(G, np.stack([gx, gy, gz], 1) / n) < 1e-3
+
+
+def test_drift_closed_forms():
+    """The fixed-point drift: exact integer displacements, periodic wrap,
+    round-to-nearest-even of half units, zero velocity, and two half-step
+    drifts equal one full drift when every displacement is an integer."""
+    rng = np.random.default_rng(17)
+    n = 4096
+    x, y, z = [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    zero = np.zeros(n, np.float32)
+    for a, b in zip(O.drift(x, y, z, zero, zero, zero, 0.37), (x, y, z)):
+        assert np.array_equal(a, b)
+    t0 = 2.0 ** -3
+    k = rng.integers(-2 ** 20, 2 ** 20, n)
+    v = (k.astype(np.float64) * 2.0 ** -29).astype(np.float32)  # v t0 = k raw units exactly
+    nx, _, _ = O.drift(x, y, z, v, zero, zero, t0)
+    assert np.array_equal(nx, ((x.astype(np.int64) + k) % 2 ** 32).astype(np.uint32))
+    # wrap at both ends
+    xe = np.array([2 ** 32 - 5, 3], np.uint32)
+    ve = np.array([10, -10], np.float32) * np.float32(2.0 ** -29)
+    we, _, _ = O.drift(xe, xe, xe, ve, np.zeros(2, np.float32), np.zeros(2, np.float32), t0)
+    assert we.tolist() == [5, 2 ** 32 - 7]
+    # half units round to even
+    h = np.array([0.5, 1.5, 2.5, -0.5, -1.5], np.float64)
+    vh = (h * 2.0 ** -29).astype(np.float32)
+    xh = np.full(5, 1000, np.uint32)
+    rh, _, _ = O.drift(xh, xh, xh, vh, np.zeros(5, np.float32), np.zeros(5, np.float32), t0)
+    assert rh.tolist() == [1000, 1002, 1002, 1000, 998]
+    # composition with even integer displacements
+    k2 = 2 * rng.integers(-2 ** 19, 2 ** 19, n)
+    v2 = (k2.astype(np.float64) * 2.0 ** -29).astype(np.float32)
+    one = O.drift(x, y, z, v2, v2, v2, t0)
+    half = O.drift(*O.drift(x, y, z, v2, v2, v2, t0 / 2), v2, v2, v2, t0 / 2)
+    for a, b in zip(one, half):
+        assert np.array_equal(a, b)

@@ -8,5 +8,6 @@ missing or no GPU is visible.
 from ._lib import (  # noqa: F401
     TPM_CIC, TPM_TSC, TPM_CLOUD4, TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK,
     Context, TpmError, TpmParams, TpmStats, TpmTuning, lib, lib_path, exported_symbols, DECLARED_SYMBOLS,
+    TPM_BATCH_DRIFT, Batch, TpmBatchLimits, TpmBoxSpec, TpmBatchTuning, TpmBatchStats, box_spec, batch_arena_bytes,
 )
 from .workloads import baseline_params, reference_params  # noqa: F401

@@ -48,6 +48,42 @@ class TpmTuning(ctypes.Structure):
     _fields_ = [("tile_sinks", ctypes.c_int), ("deposit_block", ctypes.c_int)]
 
 
+TPM_BATCH_DRIFT = 8
+
+
+class TpmBatchLimits(ctypes.Structure):
+    _fields_ = [("max_particles", ctypes.c_int64), ("max_mesh_nodes", ctypes.c_int64),
+                ("max_cells", ctypes.c_int64), ("max_boxes", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("interlace", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class TpmBoxSpec(ctypes.Structure):
+    _fields_ = [("params", TpmParams), ("t0", ctypes.c_double), ("scale", ctypes.c_double),
+                ("min_rung", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class TpmBatchTuning(ctypes.Structure):
+    _fields_ = [("tile_sinks", ctypes.c_int), ("deposit_block", ctypes.c_int), ("sched", ctypes.c_int),
+                ("graph", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TpmBatchStats(ctypes.Structure):
+    _fields_ = [("candidate_pairs", ctypes.c_double), ("executed_pairs", ctypes.c_double),
+                ("ms_step", ctypes.c_double), ("n_particles", ctypes.c_int64), ("n_boxes", ctypes.c_int32),
+                ("max_rung", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def box_spec(params, t0=1.0, scale=1.0, min_rung=0):
+    """A tpm_box_spec: the box's parameters and its time stepping."""
+    return TpmBoxSpec(params=params, t0=float(t0), scale=float(scale), min_rung=int(min_rung))
+
+
 _lib = None
 _P = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -115,6 +151,25 @@ def lib():
         "tpm_get_tuning": (_int, [vp, ctypes.POINTER(TpmTuning)]),
         "tpm_profile": (_int, [vp, _int]),
         "tpm_profile_read": (_int, [vp, _f64p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+        "tpm_batch_create": (_int, [_int, ctypes.POINTER(TpmBatchLimits), ctypes.POINTER(vp)]),
+        "tpm_batch_destroy": (_int, [vp]),
+        "tpm_batch_last_error": (ctypes.c_char_p, [vp]),
+        "tpm_batch_arena_bytes": (_int, [ctypes.POINTER(TpmBatchLimits), ctypes.POINTER(_i64)]),
+        "tpm_batch_fits": (_int, [vp, ctypes.POINTER(TpmBoxSpec), _i64, ctypes.POINTER(_int)]),
+        "tpm_batch_add": (_int, [vp, ctypes.POINTER(TpmBoxSpec), _i64, vp, vp, vp, vp, vp, vp, vp,
+                                 ctypes.POINTER(_int)]),
+        "tpm_batch_add_zeldovich": (_int, [vp, ctypes.POINTER(TpmBoxSpec), _int, ctypes.c_uint64, _dbl, _dbl,
+                                           ctypes.POINTER(_int)]),
+        "tpm_batch_remove": (_int, [vp, _int]),
+        "tpm_batch_prepare": (_int, [vp, ctypes.POINTER(TpmBoxSpec), _int]),
+        "tpm_batch_step": (_int, [vp, ctypes.c_uint, ctypes.POINTER(TpmBatchStats)]),
+        "tpm_batch_sync": (_int, [vp]),
+        "tpm_batch_box_info": (_int, [vp, _int, ctypes.POINTER(_i64), ctypes.POINTER(_int)]),
+        "tpm_batch_download": (_int, [vp, _int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "tpm_batch_to_ctx": (_int, [vp, _int, vp]),
+        "tpm_batch_set_tuning": (_int, [vp, ctypes.POINTER(TpmBatchTuning)]),
+        "tpm_batch_get_tuning": (_int, [vp, ctypes.POINTER(TpmBatchTuning)]),
+        "tpm_batch_autotune": (_int, [vp, _int, ctypes.POINTER(TpmBatchTuning), ctypes.POINTER(_dbl)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -275,3 +330,124 @@ class Context:
 
     def sync(self):
         self._chk(self._L.tpm_sync(self._h))
+
+
+def batch_arena_bytes(max_particles, max_mesh_nodes, max_boxes, order, interlace=0, max_cells=0):
+    """Device bytes of the arenas of a batch with these limits."""
+    lim = TpmBatchLimits(max_particles=int(max_particles), max_mesh_nodes=int(max_mesh_nodes),
+                         max_cells=int(max_cells), max_boxes=int(max_boxes), order=int(order),
+                         interlace=int(interlace))
+    b = _i64()
+    s = lib().tpm_batch_arena_bytes(ctypes.byref(lim), ctypes.byref(b))
+    if s != 0:
+        raise TpmError(s, "tpm_batch_arena_bytes")
+    return b.value
+
+
+class Batch:
+    """Many independent boxes resident on one device, kicked together
+    (tpm_batch)."""
+
+    def __init__(self, max_particles, max_mesh_nodes, max_boxes, order, interlace=0, max_cells=0, device=0):
+        L = lib()
+        self._L = L
+        self._h = None
+        self.limits = TpmBatchLimits(max_particles=int(max_particles), max_mesh_nodes=int(max_mesh_nodes),
+                                     max_cells=int(max_cells), max_boxes=int(max_boxes), order=int(order),
+                                     interlace=int(interlace))
+        h = ctypes.c_void_p()
+        s = L.tpm_batch_create(int(device), ctypes.byref(self.limits), ctypes.byref(h))
+        if s != 0:
+            raise TpmError(s, "tpm_batch_create failed (no GPU visible, or the arenas do not fit)")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.tpm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, s):
+        if s != 0:
+            raise TpmError(s, self._L.tpm_batch_last_error(self._h).decode())
+
+    def fits(self, spec, n):
+        f = _int()
+        self._chk(self._L.tpm_batch_fits(self._h, ctypes.byref(spec), int(n), ctypes.byref(f)))
+        return bool(f.value)
+
+    def add(self, spec, x, y, z, vx=None, vy=None, vz=None, rung=None):
+        x, y, z = [np.ascontiguousarray(a, np.uint32) for a in (x, y, z)]
+        vs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (vx, vy, vz)]
+        r = None if rung is None else np.ascontiguousarray(rung, np.int8)
+        i = _int()
+        self._chk(self._L.tpm_batch_add(self._h, ctypes.byref(spec), len(x), _ptr(x), _ptr(y), _ptr(z),
+                                        _ptr(vs[0]), _ptr(vs[1]), _ptr(vs[2]), _ptr(r), ctypes.byref(i)))
+        return i.value
+
+    def add_zeldovich(self, spec, n_side, seed, rms_disp, spectral_index=-2.0):
+        i = _int()
+        self._chk(self._L.tpm_batch_add_zeldovich(self._h, ctypes.byref(spec), int(n_side), int(seed),
+                                                  float(rms_disp), float(spectral_index), ctypes.byref(i)))
+        return i.value
+
+    def prepare(self, spec, n_side=0):
+        self._chk(self._L.tpm_batch_prepare(self._h, ctypes.byref(spec), int(n_side)))
+
+    def remove(self, box):
+        self._chk(self._L.tpm_batch_remove(self._h, int(box)))
+
+    def step(self, flags=0, stats=True):
+        st = TpmBatchStats()
+        self._chk(self._L.tpm_batch_step(self._h, int(flags), ctypes.byref(st) if stats else None))
+        return st if stats else None
+
+    def sync(self):
+        self._chk(self._L.tpm_batch_sync(self._h))
+
+    def box_info(self, box):
+        n, steps = _i64(), _int()
+        self._chk(self._L.tpm_batch_box_info(self._h, int(box), ctypes.byref(n), ctypes.byref(steps)))
+        return n.value, steps.value
+
+    def download(self, box):
+        n, _ = self.box_info(box)
+        out = dict(x=np.empty(n, np.uint32), y=np.empty(n, np.uint32), z=np.empty(n, np.uint32),
+                   vx=np.empty(n, np.float32), vy=np.empty(n, np.float32), vz=np.empty(n, np.float32),
+                   rung=np.empty(n, np.int8), gx=np.empty(n, np.float32), gy=np.empty(n, np.float32),
+                   gz=np.empty(n, np.float32), pot=np.empty(n, np.float32))
+        k = ["x", "y", "z", "vx", "vy", "vz", "rung", "gx", "gy", "gz", "pot"]
+        self._chk(self._L.tpm_batch_download(self._h, int(box), *[_ptr(out[a]) for a in k]))
+        return out
+
+    def to_ctx(self, box, ctx):
+        self._chk(self._L.tpm_batch_to_ctx(self._h, int(box), ctx._h))
+        q = TpmParams()
+        ctx._chk(ctx._L.tpm_get_params(ctx._h, ctypes.byref(q)))
+        ctx.params = q
+
+    def set_tuning(self, tile_sinks=16, deposit_block=0, sched=0, graph=0):
+        t = TpmBatchTuning(int(tile_sinks), int(deposit_block), int(sched), int(graph))
+        self._chk(self._L.tpm_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = TpmBatchTuning()
+        self._chk(self._L.tpm_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, reps=2):
+        t = TpmBatchTuning()
+        ms = _dbl()
+        self._chk(self._L.tpm_batch_autotune(self._h, int(reps), ctypes.byref(t), ctypes.byref(ms)))
+        return t.as_dict(), ms.value

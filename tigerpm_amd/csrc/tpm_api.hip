@@ -61,24 +61,33 @@ extern "C" tpm_status tpm_sync(tpm_ctx* ctx) {
 // an inconsistent parts/chain ratio) plus what the single-device kernels
 // need: the 27-cell stencil must cover rcut (cell >= rcut) and hold 27
 // distinct cells (C >= 3).
+tpm_status tpm_check_params(std::string& err, tpm_params* p) {
+    tpm_params& q = *p;
+    auto bad = [&](const char* m) {
+        err = m;
+        return TPM_ERR_INVALID;
+    };
+    if (q.mesh_dim < 4 || q.mesh_dim > 2048) return bad("mesh_dim out of range [4, 2048]");
+    if (q.order < 2 || q.order > 4) return bad("order must be 2 (CIC), 3 (TSC) or 4");
+    if (q.long_type != TPM_LONG_PM && q.long_type != TPM_LONG_PME)
+        return bad("long_type must be TPM_LONG_PM or TPM_LONG_PME");
+    if (q.interlace != 0 && q.interlace != 1) return bad("interlace must be 0 or 1");
+    if (!(q.rs > 0.0) || !(q.rcut > 0.0) || !(q.hsoft > 0.0) || !(q.rcut < 0.5))
+        return bad("rs, rcut, hsoft must be positive and rcut < 1/2");
+    if (!(q.eta > 0.0) || !std::isfinite(q.GM) || !(q.n_total >= 0.0))
+        return bad("eta must be positive, GM finite, n_total >= 0");
+    if (q.chain_dim == 0) q.chain_dim = (int)std::floor(1.0 / q.rcut);
+    if (q.chain_dim < 3 || q.chain_dim > TPM_MAX_CHAIN) return bad("chain_dim must be in [3, 406]");
+    if ((double)q.chain_dim * q.rcut > 1.0)
+        return bad("chain cells smaller than rcut: the 27-cell stencil would miss pairs");
+    return TPM_OK;
+}
+
 extern "C" tpm_status tpm_set_params(tpm_ctx* ctx, const tpm_params* p) {
     if (!ctx || !p) return TPM_ERR_INVALID;
     (void)hipSetDevice(ctx->device);
     tpm_params q = *p;
-    if (q.mesh_dim < 4 || q.mesh_dim > 2048) return tpm_fail(ctx, TPM_ERR_INVALID, "mesh_dim out of range [4, 2048]");
-    if (q.order < 2 || q.order > 4) return tpm_fail(ctx, TPM_ERR_INVALID, "order must be 2 (CIC), 3 (TSC) or 4");
-    if (q.long_type != TPM_LONG_PM && q.long_type != TPM_LONG_PME)
-        return tpm_fail(ctx, TPM_ERR_INVALID, "long_type must be TPM_LONG_PM or TPM_LONG_PME");
-    if (q.interlace != 0 && q.interlace != 1) return tpm_fail(ctx, TPM_ERR_INVALID, "interlace must be 0 or 1");
-    if (!(q.rs > 0.0) || !(q.rcut > 0.0) || !(q.hsoft > 0.0) || !(q.rcut < 0.5))
-        return tpm_fail(ctx, TPM_ERR_INVALID, "rs, rcut, hsoft must be positive and rcut < 1/2");
-    if (!(q.eta > 0.0) || !std::isfinite(q.GM) || !(q.n_total >= 0.0))
-        return tpm_fail(ctx, TPM_ERR_INVALID, "eta must be positive, GM finite, n_total >= 0");
-    if (q.chain_dim == 0) q.chain_dim = (int)std::floor(1.0 / q.rcut);
-    if (q.chain_dim < 3 || q.chain_dim > TPM_MAX_CHAIN)
-        return tpm_fail(ctx, TPM_ERR_INVALID, "chain_dim must be in [3, 406]");
-    if ((double)q.chain_dim * q.rcut > 1.0)
-        return tpm_fail(ctx, TPM_ERR_INVALID, "chain cells smaller than rcut: the 27-cell stencil would miss pairs");
+    TPM_TRY(tpm_check_params(ctx->err, &q));
     if (ctx->params_set && (q.mesh_dim != ctx->p.mesh_dim || q.order != ctx->p.order || q.rs != ctx->p.rs ||
                             q.long_type != ctx->p.long_type || q.interlace != ctx->p.interlace))
         ctx->pm_valid = false;

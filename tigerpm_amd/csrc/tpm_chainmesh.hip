@@ -191,3 +191,275 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
     ctx->chain_tile = ts;
     return TPM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Batched chain mesh over every box of a tpm_batch.
+//
+// The same order as tpm_chainmesh_build (cells of the box z-major, in a cell
+// by sub-cell Morton code then previous position), found without a radix
+// sort so that the step can be captured in a HIP graph (no memset nodes):
+//   key + count: per particle its global cell (box cell offset + cell id)
+//                and an atomic slot in that cell's count; particles of
+//                departed boxes go to a trash cell after the last one;
+//   scan:        exclusive scan of the counts -> cell_begin;
+//   scatter:     particle -> cell_begin[cell] + slot (order within a cell
+//                still arbitrary);
+//   cell order:  inside each cell, the rank of every particle by (Morton
+//                code, previous position) -> its final place;
+//   gather:      every particle array into the second buffer set.
+// The counts and ranks are exact integers, so the result is deterministic.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
+
+__global__ __launch_bounds__(256) void bkey_kernel(int64_t n, int nbox, const tpm_box_dev* __restrict__ boxes,
+                                                   int32_t trash, const uint32_t* __restrict__ x,
+                                                   const uint32_t* __restrict__ y, const uint32_t* __restrict__ z,
+                                                   uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
+                                                   int32_t* __restrict__ cnt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = tpm_upper_index(nbox, i, [&](int k) { return boxes[k].p0; });
+        uint32_t gc = (uint32_t)trash, m = 0;
+        if (b >= 0 && boxes[b].live && i < boxes[b].p0 + boxes[b].n) {
+            const int C = boxes[b].C;
+            const double ninv = boxes[b].ninv;
+            int sx, sy, sz;
+            const int cx = tpm_chain_coord_f(x[i], C, ninv, &sx);
+            const int cy = tpm_chain_coord_f(y[i], C, ninv, &sy);
+            const int cz = tpm_chain_coord_f(z[i], C, ninv, &sz);
+            gc = (uint32_t)boxes[b].cell0 + ((uint32_t)cz * C + cy) * C + cx;
+            m = ((sz >> 1) << 5) | ((sy >> 1) << 4) | ((sx >> 1) << 3) | ((sz & 1) << 2) | ((sy & 1) << 1) | (sx & 1);
+        }
+        key[i] = gc * 64u + m;
+        slot[i] = (uint32_t)atomicAdd(&cnt[gc], 1);
+    }
+}
+
+__global__ void bfill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t v, int32_t* sh, int32_t* total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < SCAN_THREADS; o <<= 1) {
+        const int32_t a = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    const int32_t incl = sh[t];
+    *total = sh[SCAN_THREADS - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// Phase 1: the sum of every SCAN_CHUNK-item chunk.
+__global__ __launch_bounds__(SCAN_THREADS) void scan_partials_kernel(const int32_t* __restrict__ in, int64_t n,
+                                                                     int32_t* __restrict__ part) {
+    __shared__ int32_t sh[SCAN_THREADS];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_CHUNK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) s += base + k < n ? in[base + k] : 0;
+    int32_t tot;
+    block_exclusive_scan(s, sh, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// Phase 2 (one block): exclusive scan of the chunk sums in place; out[n] =
+// the grand total.
+__global__ __launch_bounds__(SCAN_THREADS) void scan_part_kernel(int32_t* __restrict__ part, int64_t nchunk,
+                                                                 int32_t* __restrict__ out_total) {
+    __shared__ int32_t sh[SCAN_THREADS];
+    const int64_t per = (nchunk + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int64_t b0 = (int64_t)threadIdx.x * per;
+    int32_t s = 0;
+    for (int64_t k = b0; k < b0 + per && k < nchunk; k++) s += part[k];
+    int32_t tot;
+    int32_t run = block_exclusive_scan(s, sh, &tot);
+    for (int64_t k = b0; k < b0 + per && k < nchunk; k++) {
+        const int32_t v = part[k];
+        part[k] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) *out_total = tot;
+}
+
+// Phase 3: chunk-local exclusive scan plus the chunk offset.
+__global__ __launch_bounds__(SCAN_THREADS) void scan_apply_kernel(const int32_t* __restrict__ in, int64_t n,
+                                                                  const int32_t* __restrict__ part,
+                                                                  int32_t* __restrict__ out) {
+    __shared__ int32_t sh[SCAN_THREADS];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_CHUNK + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int32_t v[SCAN_ITEMS];
+    int32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        s += v[k];
+    }
+    int32_t tot;
+    int32_t run = block_exclusive_scan(s, sh, &tot) + part[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+__global__ void bscatter_kernel(int64_t n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
+                                const int32_t* __restrict__ cell_begin, uint32_t* __restrict__ tmp) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        tmp[cell_begin[key[i] >> 6] + slot[i]] = (uint32_t)i;
+}
+
+// One wave per cell (grid-stride over the cells): the rank of each particle
+// of the cell by (Morton code, previous position); every (code, position)
+// pair is distinct, so the ranks are a permutation of the cell's slots.
+__global__ __launch_bounds__(256) void bcell_order_kernel(int32_t ncell, const int32_t* __restrict__ cell_begin,
+                                                          const uint32_t* __restrict__ tmp,
+                                                          const uint32_t* __restrict__ key,
+                                                          uint32_t* __restrict__ perm) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int nwave = (int)(((int64_t)gridDim.x * blockDim.x) >> 6);
+    for (int c = wave; c < ncell; c += nwave) {
+        const int b = cell_begin[c];
+        const int n = cell_begin[c + 1] - b;
+        if (n == 1 && lane == 0) perm[b] = tmp[b];
+        if (n <= 1) continue;
+        for (int g = 0; g < n; g += 64) {
+            const bool valid = g + lane < n;
+            const uint32_t me = valid ? tmp[b + g + lane] : 0u;
+            const uint64_t mine = valid ? (((uint64_t)(key[me] & 63u) << 32) | me) : ~0ull;
+            int rank = 0;
+            for (int h = 0; h < n; h += 64) {
+                const uint32_t other = h + lane < n ? tmp[b + h + lane] : 0u;
+                const uint64_t oc = h + lane < n ? (((uint64_t)(key[other] & 63u) << 32) | other) : ~0ull;
+                const unsigned olo = (unsigned)oc, ohi = (unsigned)(oc >> 32);
+                const int cntq = min(64, n - h);
+                for (int q = 0; q < cntq; q++) {
+                    const uint64_t cq = ((uint64_t)(unsigned)__builtin_amdgcn_readlane((int)ohi, q) << 32) |
+                                        (unsigned)__builtin_amdgcn_readlane((int)olo, q);
+                    rank += cq < mine ? 1 : 0;
+                }
+            }
+            if (valid) perm[b + rank] = me;
+        }
+    }
+}
+
+struct GatherArgs {
+    const uint32_t* perm;
+    const uint32_t *x, *y, *z;
+    const float *vx, *vy, *vz;
+    const int8_t* rung;
+    uint32_t *x2, *y2, *z2;
+    float *vx2, *vy2, *vz2;
+    int8_t* rung2;
+};
+
+__global__ void bgather_kernel(int64_t n, GatherArgs a) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = a.perm[i];
+        a.x2[i] = a.x[j];
+        a.y2[i] = a.y[j];
+        a.z2[i] = a.z[j];
+        a.vx2[i] = a.vx[j];
+        a.vy2[i] = a.vy[j];
+        a.vz2[i] = a.vz[j];
+        a.rung2[i] = a.rung[j];
+    }
+}
+
+__global__ void btiles_count_kernel(int32_t ncell, int ts, const int32_t* __restrict__ cell_begin,
+                                    int32_t* __restrict__ nt) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x)
+        nt[c] = (cell_begin[c + 1] - cell_begin[c] + ts - 1) / ts;
+}
+
+// Tiles (global cell, begin, end, box) in cell order.
+__global__ void btiles_write_kernel(int32_t ncell, int ts, int nbox, const tpm_box_dev* __restrict__ boxes,
+                                    const int32_t* __restrict__ cell_begin, const int32_t* __restrict__ tile_off,
+                                    int4* __restrict__ tiles) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
+        const int box = tpm_upper_index(nbox, c, [&](int k) { return (int64_t)boxes[k].cell0; });
+        const int32_t b = cell_begin[c], e = cell_begin[c + 1];
+        const int32_t t0 = tile_off[c];
+        for (int32_t t = 0; b + t * ts < e; t++) {
+            const int32_t tb = b + t * ts;
+            tiles[t0 + t] = make_int4((int)c, tb, min(tb + ts, e), box);
+        }
+    }
+}
+
+}  // namespace
+
+// Exclusive scan of in[0, n) into out[0, n] (out[n] = total) with the
+// three kernels above; part holds ceil(n / SCAN_CHUNK) ints.
+static tpm_status scan_i32(std::string& err, hipStream_t s, const int32_t* in, int64_t n, int32_t* part,
+                           int32_t* out) {
+    const int64_t nchunk = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (nchunk < 1 || nchunk > INT32_MAX) {
+        err = "scan: bad size";
+        return TPM_ERR_INVALID;
+    }
+    scan_partials_kernel<<<(unsigned)nchunk, SCAN_THREADS, 0, s>>>(in, n, part);
+    scan_part_kernel<<<1, SCAN_THREADS, 0, s>>>(part, nchunk, out + n);
+    scan_apply_kernel<<<(unsigned)nchunk, SCAN_THREADS, 0, s>>>(in, n, part, out);
+    TPM_HIPE(err, hipGetLastError());
+    return TPM_OK;
+}
+
+tpm_status tpm_batch_chainmesh(tpm_batch* bt, const tpm_batch_layout& L) {
+    hipStream_t s = bt->stream;
+    const int32_t ncell = L.ncell;
+    const int64_t n = L.n_arena;
+    const int ts = bt->tune.tile_sinks;
+    // counts of the live cells and of the trash cell ncell
+    bfill_i32_kernel<<<grid_for(ncell + 1, 256), 256, 0, s>>>(bt->cnt, ncell + 1, 0);
+    if (n > 0)
+        bkey_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, L.nbox, bt->d_boxes, ncell, bt->x, bt->y, bt->z, bt->key,
+                                                     bt->slot, bt->cnt);
+    TPM_HIP(bt, hipGetLastError());
+    TPM_TRY(scan_i32(bt->err, s, bt->cnt, ncell + 1, bt->scan_part, bt->cell_begin));
+    if (n > 0) {
+        bscatter_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, bt->key, bt->slot, bt->cell_begin, bt->tmp);
+        const int64_t waves = std::min<int64_t>(std::max<int32_t>(ncell, 1), 65536);
+        bcell_order_kernel<<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(ncell, bt->cell_begin, bt->tmp, bt->key,
+                                                                        bt->perm);
+        TPM_HIP(bt, hipGetLastError());
+    }
+    if (L.n_live > 0) {
+        GatherArgs g{bt->perm, bt->x,  bt->y,  bt->z,  bt->vx,  bt->vy,  bt->vz, bt->rung,
+                     bt->x2,   bt->y2, bt->z2, bt->vx2, bt->vy2, bt->vz2, bt->rung2};
+        bgather_kernel<<<grid_for(L.n_live, 256), 256, 0, s>>>(L.n_live, g);
+        TPM_HIP(bt, hipGetLastError());
+    }
+    std::swap(bt->x, bt->x2);
+    std::swap(bt->y, bt->y2);
+    std::swap(bt->z, bt->z2);
+    std::swap(bt->vx, bt->vx2);
+    std::swap(bt->vy, bt->vy2);
+    std::swap(bt->vz, bt->vz2);
+    std::swap(bt->rung, bt->rung2);
+    // tiles of <= ts sinks per cell
+    if (ncell > 0) {
+        btiles_count_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ts, bt->cell_begin, bt->tile_cnt);
+        TPM_HIP(bt, hipGetLastError());
+        TPM_TRY(scan_i32(bt->err, s, bt->tile_cnt, ncell, bt->scan_part, bt->tile_off));
+        btiles_write_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ts, L.nbox, bt->d_boxes, bt->cell_begin,
+                                                                 bt->tile_off, bt->tiles);
+    } else {
+        bfill_i32_kernel<<<1, 64, 0, s>>>(bt->tile_off, 1, 0);
+    }
+    TPM_HIP(bt, hipGetLastError());
+    return TPM_OK;
+}

@@ -10,6 +10,8 @@
 // scaled so that the rms 3-D displacement equals the requested number of
 // lattice cells.  Positions are (i + 1/2 + psi) / n_side, wrapped into
 // [0, 1) and stored as fixed32.
+#include <algorithm>
+
 #include "tpm_internal.hpp"
 
 namespace {
@@ -69,7 +71,19 @@ __global__ void sumsq_kernel(int64_t n, const float* __restrict__ a, const float
     for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x)
         acc += (double)a[l] * a[l] + (double)b[l] * b[l] + (double)c[l] * c[l];
     for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
-    if ((threadIdx.x & 63) == 0) atomicAdd(out, acc);
+    __shared__ double part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Sum of the per-block partials in a fixed order (deterministic).
+__global__ void sum_parts_kernel(int nparts, double* __restrict__ parts) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < nparts; k++) s += parts[k];
+        parts[nparts] = s;
+    }
 }
 
 __device__ __forceinline__ uint32_t to_fixed(double pos) {
@@ -137,8 +151,15 @@ tpm_status tpm_ic_zeldovich_run(tpm_ctx* ctx, int n_side, uint64_t seed, double 
     // Scratch: noise (rho), its transform (rho_hat), one component spectrum
     // (green), and three displacement grids (phi, gx, gy of the particle
     // store, sized n = M^3).
-    float2* spec = nullptr;
-    TPM_HIP(ctx, hipMalloc(&spec, sizeof(float2) * MC));
+    if (!ctx->rho_hat1 || ctx->rho_hat1_cap < MC) {
+        if (ctx->rho_hat1) TPM_HIP(ctx, hipFree(ctx->rho_hat1));
+        ctx->rho_hat1 = nullptr;
+        TPM_HIP(ctx, hipMalloc(&ctx->rho_hat1, sizeof(float2) * MC));
+        ctx->rho_hat1_cap = MC;
+    }
+    float2* spec = ctx->rho_hat1;
+    constexpr unsigned NPART = 2048;  // sum-of-squares blocks
+    if (!ctx->dscratch) TPM_HIP(ctx, hipMalloc(&ctx->dscratch, sizeof(double) * (NPART + 8)));
     noise_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, seed, ctx->rho);
     TPM_HIP(ctx, hipGetLastError());
     TPM_TRY(tpm_fft_forward(ctx, ctx->rho, ctx->rho_hat));
@@ -148,22 +169,21 @@ tpm_status tpm_ic_zeldovich_run(tpm_ctx* ctx, int n_side, uint64_t seed, double 
         TPM_HIP(ctx, hipGetLastError());
         TPM_TRY(tpm_fft_inverse(ctx, spec, comp[d]));
     }
-    double* d_sum = nullptr;
-    TPM_HIP(ctx, hipMalloc(&d_sum, sizeof(double)));
-    TPM_HIP(ctx, hipMemsetAsync(d_sum, 0, sizeof(double), s));
-    sumsq_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, comp[0], comp[1], comp[2], d_sum);
+    double* parts = ctx->dscratch;
+    const unsigned nb = std::min<unsigned>(grid_for(n, 256), NPART);
+    sumsq_kernel<<<nb, 256, 0, s>>>(n, comp[0], comp[1], comp[2], parts);
+    sum_parts_kernel<<<1, 64, 0, s>>>((int)nb, parts);
+    double* d_sum = parts + nb;
     TPM_HIP(ctx, hipGetLastError());
     double sum = 0.0;
     TPM_HIP(ctx, hipMemcpyAsync(&sum, d_sum, sizeof(double), hipMemcpyDeviceToHost, s));
     TPM_HIP(ctx, hipStreamSynchronize(s));
-    TPM_HIP(ctx, hipFree(d_sum));
     const double rms = sqrt(sum / (double)n);
     const double scale = rms > 0.0 ? rms_disp / rms : 0.0;
     lattice_kernel<<<grid_for(n, 256), 256, 0, s>>>(M, scale, comp[0], comp[1], comp[2], ctx->x, ctx->y, ctx->z,
                                                     ctx->vx, ctx->vy, ctx->vz, ctx->rung);
     TPM_HIP(ctx, hipGetLastError());
     TPM_HIP(ctx, hipStreamSynchronize(s));
-    TPM_HIP(ctx, hipFree(spec));
     ctx->n = n;
     ctx->chain_valid = false;
     ctx->pm_valid = false;  // phi was used as scratch

@@ -6,6 +6,8 @@
 #include <rocfft/rocfft.h>
 
 #include <array>
+#include <iterator>
+#include <map>
 #include <cstdint>
 #include <string>
 #include <utility>
@@ -15,15 +17,56 @@
 
 #define TPM_MAX_CHAIN 406    // C^3 * 64 must fit the 32-bit sort key
 
+// rocFFT plans of one cubic size (single precision, out of place), shared by
+// every context and batch on the device: plans are made once per (device, M)
+// and kept until the process ends.
+struct tpm_fft_plans {
+    rocfft_plan fwd = nullptr;  // R2C
+    rocfft_plan inv = nullptr;  // C2R
+    size_t work_bytes = 0;
+};
+// Returns nullptr (and sets *err) when rocFFT cannot plan the size.
+const tpm_fft_plans* tpm_fft_plans_get(int device, int M, std::string* err);
+
+// What one caller needs to execute cached plans on its stream: execution
+// infos bound to the stream and a work buffer large enough for every plan it
+// has used.
 struct tpm_fft {
     int M = 0;
-    rocfft_plan fwd = nullptr;
-    rocfft_plan inv = nullptr;
-    rocfft_execution_info info_fwd = nullptr;
-    rocfft_execution_info info_inv = nullptr;
+    const tpm_fft_plans* plans = nullptr;
+    rocfft_execution_info info = nullptr;
     void* work = nullptr;
     size_t work_bytes = 0;
 };
+
+// Per-box constants of the fused PP + interpolation + kick kernel, and the
+// box's place in the batched arenas.  One entry per box: the single-box path
+// passes it by value, the batched path keeps a table in device memory.
+struct tpm_box_dev {
+    // layout
+    int64_t p0, n;       // particle range in the arena (before this step's sort)
+    int64_t q0;          // first particle after the sort
+    int64_t mesh0;       // offset of the box's real meshes (M^3 nodes)
+    int64_t k0;          // offset of the box's spectra (M^2 (M/2+1) modes)
+    int64_t green0;      // offset of the box's Green's table
+    int32_t cell0;       // first global chain cell of the box
+    int32_t C, M;        // chain_dim, mesh_dim
+    int32_t live;        // 0: particles leave the arena at the next sort
+    double ninv;         // 1.0 / C (split planes fixed32(k * ninv))
+    int32_t dep0;        // first deposit work item of the box
+    int32_t B, nb;       // deposit block edge (chain cells) and blocks per axis
+    int32_t min_rung, first_call, do_phi, cull;
+    // pair constants in 2^-32 box units
+    float rcut2s, cull2s, inv2rs_s, tsp, h2s, h2inv_s, hinv_s, h3inv_s;
+    float e2s;  // -log2(e) / (4 rs^2) in 2^-64 units: exp(-x^2) = exp2(r2 * e2s)
+    // kick epilogue
+    float GM, phi0, eta, t0, scale, hsoft;
+    double drift;  // t0 of the drift step (batched path)
+};
+
+// Fills the pair, epilogue and layout constants of a box from its
+// parameters (compute_pp_interaction constants, kick_treepm.cu:833-853).
+void tpm_box_fill(tpm_box_dev* b, const tpm_params& p, int min_rung, int first_call, double t0, double scale);
 
 struct tpm_ctx {
     int device = 0;
@@ -103,14 +146,15 @@ static inline hipEvent_t tpm_prof_event(tpm_ctx* ctx) {
     return ctx->prof_pool[ctx->prof_used++];
 }
 
-#define TPM_HIP(ctx, call)                                                              \
+#define TPM_HIPE(errstr, call)                                                          \
     do {                                                                                \
         hipError_t e_ = (call);                                                         \
         if (e_ != hipSuccess) {                                                         \
-            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);             \
+            (errstr) = std::string(#call) + ": " + hipGetErrorString(e_);               \
             return TPM_ERR_HIP;                                                         \
         }                                                                               \
     } while (0)
+#define TPM_HIP(ctx, call) TPM_HIPE((ctx)->err, call)
 
 #define TPM_TRY(call)                     \
     do {                                          \
@@ -118,7 +162,8 @@ static inline hipEvent_t tpm_prof_event(tpm_ctx* ctx) {
         if (s_ != TPM_OK) return s_;              \
     } while (0)
 
-static inline tpm_status tpm_fail(tpm_ctx* ctx, tpm_status s, const std::string& msg) {
+template <class T>
+static inline tpm_status tpm_fail(T* ctx, tpm_status s, const std::string& msg) {
     ctx->err = msg;
     return s;
 }
@@ -209,12 +254,185 @@ __device__ __forceinline__ int tpm_weights_1d(int order, uint32_t raw, int M, ui
     }
 }
 
+// Split plane k of a chain mesh of C cells: fixed32(k * (1.0 / C)), the
+// reference's chainmesh.cpp:198-199 in double, identical on host and device.
+__host__ __device__ __forceinline__ uint32_t tpm_split_plane(int k, double ninv) {
+    return (uint32_t)(4294967296.0 * ((double)k * ninv));
+}
+
+// tpm_chain_coord with the split planes computed instead of looked up.
+__device__ __forceinline__ int tpm_chain_coord_f(uint32_t raw, int C, double ninv, int* sub) {
+    const uint64_t P = (uint64_t)raw * (uint64_t)C;
+    int k = (int)(P >> 32);
+    int s = (int)((uint32_t)P >> 30);
+    if (k + 1 < C && raw >= tpm_split_plane(k + 1, ninv)) {
+        k++;
+        s = 0;
+    } else if (raw < tpm_split_plane(k, ninv)) {
+        k--;
+        s = 3;
+    }
+    *sub = s;
+    return k;
+}
+
+// Index of the last entry with start(e) <= v in a table sorted by start
+// (ties: the last one, which is the only one of them whose range is not
+// empty).  Returns -1 when v is below every start.
+template <class F>
+__host__ __device__ __forceinline__ int tpm_upper_index(int n, int64_t v, F start) {
+    int lo = 0, hi = n;  // first entry with start > v is in [lo, hi]
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (start(mid) <= v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo - 1;
+}
+
+// ---------------------------------------------------------------------------
+// Batch of independent boxes resident on one device (tpm_batch_* API).
+// ---------------------------------------------------------------------------
+
+// First-fit allocator of offsets in [0, cap) (host side; the arenas are
+// allocated once, boxes take and return ranges of them).
+struct tpm_range_alloc {
+    int64_t cap = 0;
+    std::map<int64_t, int64_t> free_;  // offset -> size, coalesced
+    void reset(int64_t c) {
+        cap = c;
+        free_.clear();
+        if (c > 0) free_[0] = c;
+    }
+    int64_t alloc(int64_t size, int64_t align) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            const int64_t off = (it->first + align - 1) / align * align;
+            const int64_t end = it->first + it->second;
+            if (off + size > end) continue;
+            const int64_t b0 = it->first;
+            free_.erase(it);
+            if (off > b0) free_[b0] = off - b0;
+            if (end > off + size) free_[off + size] = end - (off + size);
+            return off;
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t size) {
+        auto it = free_.emplace(off, size).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    // One past the highest offset in use.
+    int64_t high_water() const {
+        if (free_.empty()) return cap;
+        auto last = std::prev(free_.end());
+        return last->first + last->second == cap ? last->first : cap;
+    }
+};
+
+// Spectrum region of one live box, the batch keeps them sorted by k0.
+struct tpm_kregion {
+    int64_t k0, nk;
+    int32_t M, pad_;
+};
+
+struct tpm_batch_box {
+    int id = -1;
+    tpm_box_spec spec{};
+    int64_t n = 0;
+    int64_t p0 = 0;  // particle range [p0, p0 + n) in the arena
+    int steps = 0;
+    bool live = true;
+    int C = 0, M = 0;
+    int64_t mesh0 = -1, k0 = -1;  // arena regions: M^3 real nodes, M^2 (M/2+1) modes
+    int B = 0, nb = 0, wmax = 0;  // blocked deposit
+};
+
+struct tpm_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    tpm_batch_limits lim{};
+    int64_t pcap = 0, mcap = 0, kcap = 0, ccap = 0, tcap = 0;
+
+    // particle arena (SoA) and the buffers the chain-mesh gather writes into
+    uint32_t *x = nullptr, *y = nullptr, *z = nullptr, *x2 = nullptr, *y2 = nullptr, *z2 = nullptr;
+    float *vx = nullptr, *vy = nullptr, *vz = nullptr, *vx2 = nullptr, *vy2 = nullptr, *vz2 = nullptr;
+    int8_t *rung = nullptr, *rung2 = nullptr;
+    float *gx = nullptr, *gy = nullptr, *gz = nullptr, *pot = nullptr;
+    uint32_t *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr;
+    // chain cells and tiles of all live boxes
+    int32_t *cnt = nullptr, *cell_begin = nullptr, *tile_cnt = nullptr, *tile_off = nullptr, *scan_part = nullptr;
+    int4* tiles = nullptr;
+    // mesh arenas: real nodes share mesh0, spectra and Green's tables share k0
+    unsigned long long* rho_q = nullptr;
+    float *rho = nullptr, *phi = nullptr, *green = nullptr;
+    float2 *rho_hat = nullptr, *rho_hat1 = nullptr;
+    tpm_range_alloc mesh_alloc, k_alloc;
+
+    std::vector<tpm_batch_box> boxes;  // arena order
+    int next_id = 0;
+    int64_t n_arena = 0;  // particles in the arena, departed boxes included until the next sort
+    tpm_box_dev* d_boxes = nullptr;   // device table (arena order)
+    tpm_box_dev* h_boxes = nullptr;   // pinned staging of the table
+    tpm_kregion* d_kreg = nullptr;    // live boxes' spectrum regions by k0
+    tpm_kregion* h_kreg = nullptr;
+    hipEvent_t table_ev = nullptr;    // the last table upload
+    int64_t table_cap = 0;
+
+    tpm_fft fft;
+    tpm_ctx* scratch = nullptr;  // initial conditions are generated here
+    unsigned long long* counters = nullptr;  // [0] executed pairs [1] max rung [2] candidates [8..15] PP queues
+    tpm_batch_tuning tune{16, 0, 0, 0};
+    int persist_grid = 0;  // workgroups of the persistent PP launch
+
+    // captured steps, one per buffer parity (which of the two particle
+    // buffer sets holds the state when the step starts)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    uint64_t graph_version[2] = {0, 0};
+    uint64_t layout_version = 1;  // changes with the set of boxes and the tuning
+    int parity = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t copy_ev = nullptr;  // the last copy into the arena from the scratch context
+};
+
+// Layout of one step, fixed by the host before the kernels are enqueued.
+struct tpm_batch_layout {
+    int nbox = 0;          // entries of the device table (departed boxes included)
+    int64_t n_arena = 0;   // particles the key kernel reads
+    int64_t n_live = 0;    // particles after the sort
+    int32_t ncell = 0;     // cells of live boxes (the trash cell is ncell)
+    int32_t ndep = 0;      // deposit work items
+    int64_t tiles_max = 0;
+    int64_t mesh_hw = 0, k_hw = 0;
+    int nkreg = 0;
+    int order = 3, interlace = 0, lds = 0;
+};
+
 // Internal entry points implemented in the .hip files.
+tpm_status tpm_check_params(std::string& err, tpm_params* p);
 tpm_status tpm_chainmesh_build(tpm_ctx* ctx);
 tpm_status tpm_pm_solve(tpm_ctx* ctx);
 tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                        tpm_stats* stats);
 tpm_status tpm_fft_plan(tpm_ctx* ctx, int M);
+tpm_status tpm_fft_use(std::string& err, tpm_fft& f, int device, hipStream_t s, int M);
+tpm_status tpm_fft_r2c(std::string& err, tpm_fft& f, float* in, float2* out);
+tpm_status tpm_fft_c2r(std::string& err, tpm_fft& f, float2* in, float* out);
+void tpm_fft_free(tpm_fft& f);
 tpm_status tpm_fft_forward(tpm_ctx* ctx, float* in, float2* out);
 tpm_status tpm_fft_inverse(tpm_ctx* ctx, float2* in, float* out);
 tpm_status tpm_ensure_particles(tpm_ctx* ctx, int64_t n);
@@ -224,3 +442,16 @@ tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, doubl
 tpm_status tpm_ic_zeldovich_run(tpm_ctx* ctx, int n_side, uint64_t seed, double rms_disp, double spectral_index);
 tpm_status tpm_ic_uniform_run(tpm_ctx* ctx, int64_t n, uint64_t seed);
 tpm_status tpm_fill_u64(tpm_ctx* ctx, unsigned long long* p, int64_t n);
+
+// Batched stages (tpm_chainmesh.hip, tpm_pm.hip, tpm_pp.hip).
+tpm_status tpm_batch_chainmesh(tpm_batch* bt, const tpm_batch_layout& L);
+tpm_status tpm_batch_pm(tpm_batch* bt, const tpm_batch_layout& L);
+tpm_status tpm_batch_pp(tpm_batch* bt, const tpm_batch_layout& L, unsigned flags);
+tpm_status tpm_batch_drift(tpm_batch* bt, const tpm_batch_layout& L);
+tpm_status tpm_batch_candidates(tpm_batch* bt, const tpm_batch_layout& L);
+tpm_status tpm_batch_fill_u64(tpm_batch* bt, unsigned long long* p, int64_t n);
+int tpm_pp_persistent_grid(int device, int ts);
+tpm_status tpm_green_table(std::string& err, hipStream_t s, int M, double rs, int order, float* table);
+// Deposit blocking of one box: block edge B (chain cells) and the widest
+// footprint in mesh nodes, so that w^3 int64 nodes fit 64 KiB of LDS.
+void tpm_deposit_blocking(int C, int M, int interlace, int want, int* B, int* wmax);

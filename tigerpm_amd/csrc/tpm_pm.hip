@@ -11,6 +11,8 @@
 // Optionally (interlace) a second deposit onto the grid displaced by half a
 // cell is combined in k space, cancelling the odd aliases of the assignment.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "tpm_internal.hpp"
 
@@ -108,20 +110,23 @@ __host__ __device__ __forceinline__ void block_nodes(uint64_t xlo, uint64_t xhi_
     *w = h - l + 1;
 }
 
-template <int ORDER>
-__global__ __launch_bounds__(256) void deposit_blocked_kernel(BlockedDepositArgs a) {
-    extern __shared__ unsigned long long sg[];
+// The deposit of block bid (of nb^3) of one box; plane(k) gives split plane
+// k < C of the box's chain mesh, cell0 its first global cell.
+template <int ORDER, class Plane>
+__device__ __forceinline__ void deposit_block(const uint32_t* __restrict__ x, const uint32_t* __restrict__ y,
+                                              const uint32_t* __restrict__ z, const int32_t* __restrict__ cell_begin,
+                                              int cell0, unsigned long long* __restrict__ grid, uint64_t shift, int C,
+                                              int M, int B, int nb, int bid, Plane plane,
+                                              unsigned long long* __restrict__ sg) {
     constexpr int NP = ORDER;
-    const int C = a.C, M = a.M, B = a.B, nb = a.nb;
-    const int bid = blockIdx.x;
     const int bx = bid % nb, by = (bid / nb) % nb, bz = bid / (nb * nb);
     const int c0[3] = {bx * B, by * B, bz * B};
     int lo[3], w[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const int c1 = min(c0[d] + B, C) - 1;
-        const uint64_t xhi = (c1 + 1 < C ? (uint64_t)a.bounds[c1 + 1] : (1ull << 32)) - 1;
-        block_nodes(a.bounds[c0[d]], xhi, M, a.shift, &lo[d], &w[d]);
+        const uint64_t xhi = (c1 + 1 < C ? (uint64_t)plane(c1 + 1) : (1ull << 32)) - 1;
+        block_nodes(plane(c0[d]), xhi, M, shift, &lo[d], &w[d]);
     }
     const int c1x = min(c0[0] + B, C) - 1, c1y = min(c0[1] + B, C) - 1, c1z = min(c0[2] + B, C) - 1;
     const int wy = w[1], wz = w[2];
@@ -131,21 +136,21 @@ __global__ __launch_bounds__(256) void deposit_blocked_kernel(BlockedDepositArgs
     for (int cz = c0[2]; cz <= c1z; cz++) {
         for (int cy = c0[1]; cy <= c1y; cy++) {
             // cells c0x..c1x of one (cy, cz) row are contiguous in memory
-            const int rb = a.cell_begin[(cz * C + cy) * C + c0[0]];
-            const int re = a.cell_begin[(cz * C + cy) * C + c1x + 1];
+            const int rb = cell_begin[cell0 + (cz * C + cy) * C + c0[0]];
+            const int re = cell_begin[cell0 + (cz * C + cy) * C + c1x + 1];
             for (int i = rb + threadIdx.x; i < re; i += blockDim.x) {
                 int bxn, byn, bzn;
                 uint64_t w3[NP * NP * NP];
-                weights_3d<ORDER>(a.x[i], a.y[i], a.z[i], M, a.shift, &bxn, &byn, &bzn, w3);
+                weights_3d<ORDER>(x[i], y[i], z[i], M, shift, &bxn, &byn, &bzn, w3);
                 const int ox = bxn - lo[0], oy = byn - lo[1], oz = bzn - lo[2];
 #pragma unroll
                 for (int p = 0; p < NP; p++)
 #pragma unroll
                     for (int r = 0; r < NP; r++)
 #pragma unroll
-                        for (int s = 0; s < NP; s++)
-                            atomicAdd(&sg[((ox + p) * wy + (oy + r)) * wz + (oz + s)],
-                                      (unsigned long long)w3[(p * NP + r) * NP + s]);
+                        for (int q = 0; q < NP; q++)
+                            atomicAdd(&sg[((ox + p) * wy + (oy + r)) * wz + (oz + q)],
+                                      (unsigned long long)w3[(p * NP + r) * NP + q]);
             }
         }
     }
@@ -157,8 +162,32 @@ __global__ __launch_bounds__(256) void deposit_blocked_kernel(BlockedDepositArgs
         const int ix = wrapi(lo[0] + l / (wy * wz), M);
         const int iy = wrapi(lo[1] + (l / wz) % wy, M);
         const int iz = wrapi(lo[2] + l % wz, M);
-        atomicAdd(&a.grid[ix * M2 + (int64_t)iy * M + iz], v);
+        atomicAdd(&grid[ix * M2 + (int64_t)iy * M + iz], v);
     }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256) void deposit_blocked_kernel(BlockedDepositArgs a) {
+    extern __shared__ unsigned long long sg[];
+    const uint32_t* bounds = a.bounds;
+    deposit_block<ORDER>(a.x, a.y, a.z, a.cell_begin, 0, a.grid, a.shift, a.C, a.M, a.B, a.nb, (int)blockIdx.x,
+                         [&](int k) { return bounds[k]; }, sg);
+}
+
+// Batched: work item -> (box, block) through the boxes' first items dep0.
+template <int ORDER>
+__global__ __launch_bounds__(256) void bdeposit_kernel(const uint32_t* __restrict__ x, const uint32_t* __restrict__ y,
+                                                       const uint32_t* __restrict__ z,
+                                                       const int32_t* __restrict__ cell_begin, int nbox,
+                                                       const tpm_box_dev* __restrict__ boxes,
+                                                       unsigned long long* __restrict__ grid, uint64_t shift) {
+    extern __shared__ unsigned long long sg[];
+    const int item = (int)blockIdx.x;
+    const int b = tpm_upper_index(nbox, item, [&](int k) { return (int64_t)boxes[k].dep0; });
+    const tpm_box_dev& bx = boxes[b];
+    const double ninv = bx.ninv;
+    deposit_block<ORDER>(x, y, z, cell_begin, bx.cell0, grid + bx.mesh0, shift, bx.C, bx.M, bx.B, bx.nb,
+                         item - bx.dep0, [&](int k) { return tpm_split_plane(k, ninv); }, sg);
 }
 
 // Unit masses per node: float(double(sum) * 2^-32).
@@ -227,7 +256,107 @@ __global__ void interlace_green_kernel(int M, float2* __restrict__ c0, const flo
     }
 }
 
+// Batched Green's multiply over the spectrum arena [0, n): each mode finds
+// its box's region; modes outside every region are left alone.  With c1 the
+// interlaced combination of interlace_green_kernel.
+__global__ void bgreen_apply_kernel(int64_t n, int nreg, const tpm_kregion* __restrict__ reg,
+                                    float2* __restrict__ c0, const float2* __restrict__ c1,
+                                    const float* __restrict__ g) {
+    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < n; l += (int64_t)gridDim.x * blockDim.x) {
+        const int r = tpm_upper_index(nreg, l, [&](int k) { return reg[k].k0; });
+        if (r < 0 || l >= reg[r].k0 + reg[r].nk) continue;
+        const float gg = g[l];
+        float2 a = c0[l];
+        if (c1) {
+            const int M = reg[r].M;
+            const int Mh = M / 2 + 1;
+            const int64_t q = l - reg[r].k0;
+            const int iz = (int)(q % Mh);
+            const int iy = (int)((q / Mh) % M);
+            const int ix = (int)(q / ((int64_t)Mh * M));
+            const int ksum = (2 * ix < M ? ix : ix - M) + (2 * iy < M ? iy : iy - M) + (2 * iz < M ? iz : iz - M);
+            float sn, cs;
+            sincosf((float)(M_PI / M) * (float)ksum, &sn, &cs);
+            const float2 b = c1[l];
+            const float h = 0.5f * gg;
+            a = make_float2((a.x + cs * b.x - sn * b.y) * h, (a.y + sn * b.x + cs * b.y) * h);
+        } else {
+            a.x *= gg;
+            a.y *= gg;
+        }
+        c0[l] = a;
+    }
+}
+
 }  // namespace
+
+tpm_status tpm_green_table(std::string& err, hipStream_t s, int M, double rs, int order, float* table) {
+    const int64_t MC = (int64_t)M * M * (M / 2 + 1);
+    green_kernel<<<grid_for(MC, 256), 256, 0, s>>>(M, rs, order, table);
+    TPM_HIPE(err, hipGetLastError());
+    return TPM_OK;
+}
+
+tpm_status tpm_batch_fill_u64(tpm_batch* bt, unsigned long long* p, int64_t n) {
+    fill_u64_kernel<<<grid_for(n, 256), 256, 0, bt->stream>>>(p, n);
+    TPM_HIP(bt, hipGetLastError());
+    return TPM_OK;
+}
+
+// Deposit (grid shift) of every live box into its region of rho_q, then the
+// float image of the used part of the arena.
+static tpm_status batch_deposit(tpm_batch* bt, const tpm_batch_layout& L, uint64_t shift) {
+    hipStream_t s = bt->stream;
+    TPM_TRY(tpm_batch_fill_u64(bt, bt->rho_q, L.mesh_hw));
+    if (L.ndep > 0) {
+        const size_t lds = (size_t)L.lds;
+        switch (L.order) {
+            case 2:
+                bdeposit_kernel<2><<<(unsigned)L.ndep, 256, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
+                                                                       bt->d_boxes, bt->rho_q, shift);
+                break;
+            case 3:
+                bdeposit_kernel<3><<<(unsigned)L.ndep, 256, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
+                                                                       bt->d_boxes, bt->rho_q, shift);
+                break;
+            default:
+                bdeposit_kernel<4><<<(unsigned)L.ndep, 256, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
+                                                                       bt->d_boxes, bt->rho_q, shift);
+                break;
+        }
+        TPM_HIP(bt, hipGetLastError());
+    }
+    q32_to_float_kernel<<<grid_for(L.mesh_hw, 256), 256, 0, s>>>(L.mesh_hw, bt->rho_q, bt->rho);
+    TPM_HIP(bt, hipGetLastError());
+    return TPM_OK;
+}
+
+tpm_status tpm_batch_pm(tpm_batch* bt, const tpm_batch_layout& L) {
+    if (L.mesh_hw == 0) return TPM_OK;
+    TPM_TRY(batch_deposit(bt, L, 0));
+    for (const tpm_batch_box& b : bt->boxes) {
+        if (!b.live) continue;
+        TPM_TRY(tpm_fft_use(bt->err, bt->fft, bt->device, bt->stream, b.M));
+        TPM_TRY(tpm_fft_r2c(bt->err, bt->fft, bt->rho + b.mesh0, bt->rho_hat + b.k0));
+    }
+    if (L.interlace) {
+        TPM_TRY(batch_deposit(bt, L, 1ull << 31));
+        for (const tpm_batch_box& b : bt->boxes) {
+            if (!b.live) continue;
+            TPM_TRY(tpm_fft_use(bt->err, bt->fft, bt->device, bt->stream, b.M));
+            TPM_TRY(tpm_fft_r2c(bt->err, bt->fft, bt->rho + b.mesh0, bt->rho_hat1 + b.k0));
+        }
+    }
+    bgreen_apply_kernel<<<grid_for(L.k_hw, 256), 256, 0, bt->stream>>>(
+        L.k_hw, L.nkreg, bt->d_kreg, bt->rho_hat, L.interlace ? bt->rho_hat1 : nullptr, bt->green);
+    TPM_HIP(bt, hipGetLastError());
+    for (const tpm_batch_box& b : bt->boxes) {
+        if (!b.live) continue;
+        TPM_TRY(tpm_fft_use(bt->err, bt->fft, bt->device, bt->stream, b.M));
+        TPM_TRY(tpm_fft_c2r(bt->err, bt->fft, bt->rho_hat + b.k0, bt->phi + b.mesh0));
+    }
+    return TPM_OK;
+}
 
 tpm_status tpm_fill_u64(tpm_ctx* ctx, unsigned long long* p, int64_t n) {
     fill_u64_kernel<<<grid_for(n, 256), 256, 0, ctx->stream>>>(p, n);
@@ -235,60 +364,94 @@ tpm_status tpm_fill_u64(tpm_ctx* ctx, unsigned long long* p, int64_t n) {
     return TPM_OK;
 }
 
-static tpm_status fft_destroy(tpm_ctx* ctx) {
-    tpm_fft& f = ctx->fft;
-    if (f.fwd) rocfft_plan_destroy(f.fwd);
-    if (f.inv) rocfft_plan_destroy(f.inv);
-    if (f.info_fwd) rocfft_execution_info_destroy(f.info_fwd);
-    if (f.info_inv) rocfft_execution_info_destroy(f.info_inv);
-    if (f.work) TPM_HIP(ctx, hipFree(f.work));
-    f = tpm_fft{};
-    return TPM_OK;
+// ---------------------------------------------------------------------------
+// rocFFT plans: one pair per (device, M), shared by every caller.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_plan_mu;
+std::map<std::pair<int, int>, tpm_fft_plans*> g_plans;
+}  // namespace
+
+const tpm_fft_plans* tpm_fft_plans_get(int device, int M, std::string* err) {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find({device, M});
+    if (it != g_plans.end()) return it->second;
+    auto* pl = new tpm_fft_plans();
+    const size_t len[3] = {(size_t)M, (size_t)M, (size_t)M};  // fastest first; cubic
+    if (rocfft_plan_create(&pl->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                           rocfft_precision_single, 3, len, 1, nullptr) != rocfft_status_success ||
+        rocfft_plan_create(&pl->inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                           rocfft_precision_single, 3, len, 1, nullptr) != rocfft_status_success) {
+        if (pl->fwd) rocfft_plan_destroy(pl->fwd);
+        delete pl;
+        if (err) *err = "rocfft_plan_create failed for M = " + std::to_string(M);
+        return nullptr;
+    }
+    size_t w1 = 0, w2 = 0;
+    rocfft_plan_get_work_buffer_size(pl->fwd, &w1);
+    rocfft_plan_get_work_buffer_size(pl->inv, &w2);
+    pl->work_bytes = std::max(w1, w2);
+    g_plans[{device, M}] = pl;
+    return pl;
 }
 
-tpm_status tpm_fft_plan(tpm_ctx* ctx, int M) {
-    tpm_fft& f = ctx->fft;
-    if (f.M == M && f.fwd) return TPM_OK;
-    TPM_TRY(fft_destroy(ctx));
-    const size_t len[3] = {(size_t)M, (size_t)M, (size_t)M};  // fastest first; cubic
-    if (rocfft_plan_create(&f.fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
-                           rocfft_precision_single, 3, len, 1, nullptr) != rocfft_status_success)
-        return tpm_fail(ctx, TPM_ERR_FFT, "rocfft_plan_create (forward) failed");
-    if (rocfft_plan_create(&f.inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
-                           rocfft_precision_single, 3, len, 1, nullptr) != rocfft_status_success)
-        return tpm_fail(ctx, TPM_ERR_FFT, "rocfft_plan_create (inverse) failed");
-    size_t w1 = 0, w2 = 0;
-    rocfft_plan_get_work_buffer_size(f.fwd, &w1);
-    rocfft_plan_get_work_buffer_size(f.inv, &w2);
-    f.work_bytes = std::max(w1, w2);
-    if (f.work_bytes) TPM_HIP(ctx, hipMalloc(&f.work, f.work_bytes));
-    rocfft_execution_info_create(&f.info_fwd);
-    rocfft_execution_info_create(&f.info_inv);
-    rocfft_execution_info_set_stream(f.info_fwd, ctx->stream);
-    rocfft_execution_info_set_stream(f.info_inv, ctx->stream);
-    if (f.work_bytes) {
-        rocfft_execution_info_set_work_buffer(f.info_fwd, f.work, f.work_bytes);
-        rocfft_execution_info_set_work_buffer(f.info_inv, f.work, f.work_bytes);
+tpm_status tpm_fft_use(std::string& err, tpm_fft& f, int device, hipStream_t s, int M) {
+    if (f.M == M && f.plans) return TPM_OK;
+    const tpm_fft_plans* pl = tpm_fft_plans_get(device, M, &err);
+    if (!pl) return TPM_ERR_FFT;
+    if (!f.info) {
+        if (rocfft_execution_info_create(&f.info) != rocfft_status_success ||
+            rocfft_execution_info_set_stream(f.info, s) != rocfft_status_success) {
+            err = "rocfft_execution_info_create failed";
+            return TPM_ERR_FFT;
+        }
+    }
+    if (pl->work_bytes > f.work_bytes) {
+        if (f.work) TPM_HIPE(err, hipFree(f.work));
+        f.work = nullptr;
+        TPM_HIPE(err, hipMalloc(&f.work, pl->work_bytes));
+        f.work_bytes = pl->work_bytes;
+        if (rocfft_execution_info_set_work_buffer(f.info, f.work, f.work_bytes) != rocfft_status_success) {
+            err = "rocfft_execution_info_set_work_buffer failed";
+            return TPM_ERR_FFT;
+        }
     }
     f.M = M;
+    f.plans = pl;
     return TPM_OK;
 }
 
-tpm_status tpm_fft_forward(tpm_ctx* ctx, float* in, float2* out) {
+tpm_status tpm_fft_r2c(std::string& err, tpm_fft& f, float* in, float2* out) {
     void* ip[1] = {in};
     void* op[1] = {out};
-    if (rocfft_execute(ctx->fft.fwd, ip, op, ctx->fft.info_fwd) != rocfft_status_success)
-        return tpm_fail(ctx, TPM_ERR_FFT, "rocfft_execute (forward) failed");
+    if (rocfft_execute(f.plans->fwd, ip, op, f.info) != rocfft_status_success) {
+        err = "rocfft_execute (forward) failed";
+        return TPM_ERR_FFT;
+    }
     return TPM_OK;
 }
 
-tpm_status tpm_fft_inverse(tpm_ctx* ctx, float2* in, float* out) {
+tpm_status tpm_fft_c2r(std::string& err, tpm_fft& f, float2* in, float* out) {
     void* ip[1] = {in};
     void* op[1] = {out};
-    if (rocfft_execute(ctx->fft.inv, ip, op, ctx->fft.info_inv) != rocfft_status_success)
-        return tpm_fail(ctx, TPM_ERR_FFT, "rocfft_execute (inverse) failed");
+    if (rocfft_execute(f.plans->inv, ip, op, f.info) != rocfft_status_success) {
+        err = "rocfft_execute (inverse) failed";
+        return TPM_ERR_FFT;
+    }
     return TPM_OK;
 }
+
+void tpm_fft_free(tpm_fft& f) {
+    if (f.info) rocfft_execution_info_destroy(f.info);
+    if (f.work) (void)hipFree(f.work);
+    f = tpm_fft{};
+}
+
+tpm_status tpm_fft_plan(tpm_ctx* ctx, int M) { return tpm_fft_use(ctx->err, ctx->fft, ctx->device, ctx->stream, M); }
+
+tpm_status tpm_fft_forward(tpm_ctx* ctx, float* in, float2* out) { return tpm_fft_r2c(ctx->err, ctx->fft, in, out); }
+
+tpm_status tpm_fft_inverse(tpm_ctx* ctx, float2* in, float* out) { return tpm_fft_c2r(ctx->err, ctx->fft, in, out); }
 
 tpm_status tpm_ensure_mesh(tpm_ctx* ctx, int M) {
     const int64_t M3 = (int64_t)M * M * M;
@@ -311,6 +474,32 @@ tpm_status tpm_ensure_mesh(tpm_ctx* ctx, int M) {
     return tpm_fft_plan(ctx, M);
 }
 
+void tpm_deposit_blocking(int C, int M, int interlace, int want, int* Bout, int* wout) {
+    // the largest block edge (cells, <= want or 3) whose footprint of int64
+    // nodes fits 64 KiB of LDS, for both grid shifts in use
+    const double ninv = 1.0 / C;
+    std::vector<uint64_t> b(C + 1);
+    for (int k = 0; k < C; k++) b[k] = tpm_split_plane(k, ninv);
+    b[C] = 1ull << 32;
+    *Bout = 0;
+    *wout = 0;
+    for (int cand = want > 0 ? want : 3; cand >= 1; cand--) {
+        int wm = 0;
+        for (int sh = 0; sh <= (interlace ? 1 : 0); sh++)
+            for (int c0 = 0; c0 < C; c0 += cand) {
+                const int c1 = std::min(c0 + cand, C) - 1;
+                int lo, w;
+                block_nodes(b[c0], b[c1 + 1] - 1, M, sh ? (1ull << 31) : 0ull, &lo, &w);
+                wm = std::max(wm, w);
+            }
+        if ((int64_t)wm * wm * wm * 8 <= 65536) {
+            *Bout = cand;
+            *wout = wm;
+            return;
+        }
+    }
+}
+
 // Integer deposit (grid displaced by shift / 2^32 cells) into rho_q and its
 // float image into rho.
 static tpm_status deposit_density(tpm_ctx* ctx, uint64_t shift) {
@@ -318,31 +507,10 @@ static tpm_status deposit_density(tpm_ctx* ctx, uint64_t shift) {
     const int64_t M3 = (int64_t)M * M * M;
     hipStream_t s = ctx->stream;
     TPM_TRY(tpm_fill_u64(ctx, ctx->rho_q, M3));
-    // Blocked deposit needs particles in chain-mesh order; pick the largest
-    // block edge (cells) whose LDS footprint fits 64 KiB.
+    // Blocked deposit needs particles in chain-mesh order.
     int B = 0, wmax = 0;
-    if (ctx->n > 0 && ctx->chain_valid && ctx->chain_C == ctx->C && ctx->deposit_block >= 0) {
-        const int C = ctx->C;
-        std::vector<uint64_t> b(C + 1);
-        const double ninv = 1.0 / C;
-        for (int k = 0; k < C; k++) b[k] = (uint32_t)(4294967296.0 * ((double)k * ninv));
-        b[C] = 1ull << 32;
-        const int want = ctx->deposit_block > 0 ? ctx->deposit_block : 3;
-        for (int cand = want; cand >= 1; cand--) {
-            int wm = 0;
-            for (int c0 = 0; c0 < C; c0 += cand) {
-                const int c1 = std::min(c0 + cand, C) - 1;
-                int lo, w;
-                block_nodes(b[c0], b[c1 + 1] - 1, M, shift, &lo, &w);
-                wm = std::max(wm, w);
-            }
-            if ((int64_t)wm * wm * wm * 8 <= 65536) {
-                B = cand;
-                wmax = wm;
-                break;
-            }
-        }
-    }
+    if (ctx->n > 0 && ctx->chain_valid && ctx->chain_C == ctx->C && ctx->deposit_block >= 0)
+        tpm_deposit_blocking(ctx->C, M, ctx->p.interlace, ctx->deposit_block, &B, &wmax);
     if (B > 0) {
         BlockedDepositArgs da;
         da.x = ctx->x;
@@ -413,4 +581,7 @@ tpm_status tpm_pm_solve(tpm_ctx* ctx) {
     return TPM_OK;
 }
 
-tpm_status tpm_fft_release(tpm_ctx* ctx) { return fft_destroy(ctx); }
+tpm_status tpm_fft_release(tpm_ctx* ctx) {
+    tpm_fft_free(ctx->fft);
+    return TPM_OK;
+}

@@ -30,6 +30,11 @@
 
 #define SBUF 256  // staged sources per flush (4 KiB LDS)
 
+// Every fused multiply-add in this file is written out (fmaf) and the
+// compiler may fuse nothing else, so the results come out the same bits in
+// every instantiation of the kernel (single box, batch, persistent).
+#pragma clang fp contract(off)
+
 namespace {
 
 struct PPArgs {
@@ -47,16 +52,12 @@ struct PPArgs {
     const int32_t* __restrict__ cell_begin;
     const int4* __restrict__ tiles;
     const int32_t* __restrict__ ntiles;
-    const float* __restrict__ phi;
+    const float* __restrict__ phi;  // mesh arena (box offset mesh0), nullptr: no mesh term
     unsigned long long* __restrict__ counters;
-    int C, M;
-    int min_rung, first_call, do_phi, cull;
+    int* __restrict__ queue;        // 8 per-XCD tile queues (persistent launch)
+    const tpm_box_dev* __restrict__ boxes;  // batched: per-box table, tile.w = box
+    tpm_box_dev box0;                        // single box
     unsigned flags;
-    // pair constants in 2^-32 box units
-    float rcut2s, cull2s, inv2rs_s, tsp, h2s, h2inv_s, hinv_s, h3inv_s;
-    float e2s;  // -log2(e) / (4 rs^2), 2^-64 units: exp(-x^2) = exp2(r2 * e2s)
-    // epilogue
-    float GM, phi0, eta, t0, scale, hsoft;
 };
 
 __device__ __forceinline__ unsigned lane_id() {
@@ -81,7 +82,7 @@ __device__ __forceinline__ int wave_max(int v) {
 // compute_pp_interaction (kick_treepm.cu:242-283) with erfcexp
 // (util.hpp:36-59, Abramowitz & Stegun 7.1.26), in 2^-32 units.  self is
 // the sink's own particle index: that pair is excluded here.
-__device__ __forceinline__ void pp_pair(const PPArgs& a, int ox, int oy, int oz, int self, int4 s, float& gx,
+__device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, int oz, int self, int4 s, float& gx,
                                         float& gy, float& gz, float& ph) {
     const float dx = (float)(int)((unsigned)ox - (unsigned)s.x);
     const float dy = (float)(int)((unsigned)oy - (unsigned)s.y);
@@ -114,13 +115,13 @@ __device__ __forceinline__ void pp_pair(const PPArgs& a, int ox, int oy, int oz,
         d1 = fmaf(d1, q2, -21.0f / 4.0f);
         d1 = fmaf(d1, q2, 35.0f / 8.0f);
         d1 *= a.h3inv_s;
-        rinv3 = ((erfc0 - 1.0f) + tre) * ri * ri * ri + d1;
+        rinv3 = fmaf(((erfc0 - 1.0f) + tre) * ri * ri, ri, d1);
         float d0 = -5.0f / 16.0f;
         d0 = fmaf(d0, q2, 21.0f / 16.0f);
         d0 = fmaf(d0, q2, -35.0f / 16.0f);
         d0 = fmaf(d0, q2, 35.0f / 16.0f);
         d0 *= a.hinv_s;
-        rinv1 = (erfc0 - 1.0f) * ri + d0;
+        rinv1 = fmaf(erfc0 - 1.0f, ri, d0);
     }
     gx = fmaf(-dx, rinv3, gx);
     gy = fmaf(-dy, rinv3, gy);
@@ -144,213 +145,270 @@ __device__ __forceinline__ void interp6_weights(float x, float w[6], float dw[6]
     dw[5] = x * x * ((21.f / 24.f) + x * (-2.0f + x * (25.f / 24.f)));
 }
 
-template <int TS>
+// The fused kernel.  Work item = one tile: TS sinks of one chain cell
+// against the culled sources of the 27 neighbour cells, then mesh
+// interpolation and the kick epilogue.
+//   PERSIST = false: one workgroup per tile.  The hardware deals workgroups
+//     round-robin over the 8 XCDs, so each XCD gets one contiguous
+//     (spatially coherent) run of tiles and its L2 keeps the shared
+//     neighbour cells.
+//   PERSIST = true: resident workgroups take tiles from 8 queues, one
+//     contiguous run per XCD; a workgroup drains its own XCD's queue first
+//     and then helps the others.  Every queue ends, so every workgroup exits.
+// BATCH = true: tile.w names the box whose constants apply (device table).
+template <int TS, bool BATCH, bool PERSIST>
 __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
     constexpr int SPLIT = 64 / TS;  // source streams per wave
     __shared__ int4 sbuf[SBUF];
-    // XCD-aware tile order: the hardware deals workgroups round-robin over
-    // the 8 XCDs, so give each XCD one contiguous (spatially coherent) run of
-    // tiles and let its L2 keep the shared neighbour cells.
     const int nt = *a.ntiles;
     const int per_xcd = (nt + 7) >> 3;
-    const int k = blockIdx.x >> 3;
-    if (k >= per_xcd) return;
-    const int t = (int)(blockIdx.x & 7) * per_xcd + k;
-    if (t >= nt) return;
-
-    const int4 tile = a.tiles[t];
-    const int lane = (int)lane_id();
-    const int slot = lane % TS;
-    const int stream = lane / TS;
-    const int i = tile.y + slot;
-    const bool valid = i < tile.z;
-    const uint32_t rx = a.x[tile.y], ry = a.y[tile.y], rz = a.z[tile.y];
-    uint32_t px = rx, py = ry, pz = rz;
-    int r = 0;
-    if (valid) {
-        px = a.x[i];
-        py = a.y[i];
-        pz = a.z[i];
-        r = a.rung[i];
+    const int xcd = (int)(blockIdx.x & 7);
+    int q = xcd, qi = 0;  // persistent: current queue and queues visited
+    int t;
+    if constexpr (PERSIST) {
+        t = INT_MAX;
+    } else {
+        const int k = blockIdx.x >> 3;
+        if (k >= per_xcd) return;
+        t = xcd * per_xcd + k;
+        if (t >= nt) return;
     }
-    const bool active = valid && r >= a.min_rung;
-    const unsigned long long amask = __ballot(active);
-    if (amask == 0ull) return;
-    const int ox = (int)(px - rx), oy = (int)(py - ry), oz = (int)(pz - rz);
-
-    float gx = 0.f, gy = 0.f, gz = 0.f, ph = 0.f;
-    if (!(a.flags & TPM_SKIP_PP)) {
-        // Tile bounding box as floats.  float(int32) rounds by <= 64 raw
-        // units, so box distances below carry <= 192 units of error, inside
-        // the 512-unit margin of cull2s: the test never drops a source with
-        // r < rcut to any sink.
-        const float lox = (float)wave_min(active ? ox : INT_MAX), hix = (float)wave_max(active ? ox : INT_MIN);
-        const float loy = (float)wave_min(active ? oy : INT_MAX), hiy = (float)wave_max(active ? oy : INT_MIN);
-        const float loz = (float)wave_min(active ? oz : INT_MAX), hiz = (float)wave_max(active ? oz : INT_MIN);
-        const int C = a.C;
-        const int cid = tile.x;
-        const int cx = cid % C, cy = (cid / C) % C, cz = cid / (C * C);
-        int nbuf = 0;
-        unsigned long long staged = 0ull;
-        // Far dummy that pads the list to a multiple of SPLIT: r >= rcut.
-        const int4 far = make_int4(0x40000000, 0x40000000, 0x40000000, -1);
-        auto flush = [&]() {
-            const int padded = (nbuf + SPLIT - 1) / SPLIT * SPLIT;
-            if (lane < padded - nbuf) sbuf[nbuf + lane] = far;
-            __syncthreads();
-            int4 s = sbuf[stream];
+    for (;;) {
+        if constexpr (PERSIST) {
+            // next tile: own queue first, then the others
+            for (;;) {
+                int k = 0;
+                if (lane_id() == 0) k = atomicAdd(&a.queue[q], 1);
+                k = __builtin_amdgcn_readfirstlane(__shfl(k, 0));
+                t = q * per_xcd + k;
+                if (t < min((q + 1) * per_xcd, nt)) break;
+                if (++qi == 8) return;
+                q = (xcd + qi) & 7;
+            }
+        }
+        const int4 tile = a.tiles[t];
+        const tpm_box_dev& B = BATCH ? a.boxes[__builtin_amdgcn_readfirstlane(tile.w)] : a.box0;
+        const int lane = (int)lane_id();
+        const int slot = lane % TS;
+        const int stream = lane / TS;
+        const int i = tile.y + slot;
+        const bool valid = i < tile.z;
+        const uint32_t rx = a.x[tile.y], ry = a.y[tile.y], rz = a.z[tile.y];
+        uint32_t px = rx, py = ry, pz = rz;
+        int r = 0;
+        if (valid) {
+            px = a.x[i];
+            py = a.y[i];
+            pz = a.z[i];
+            r = a.rung[i];
+        }
+        const bool active = valid && r >= B.min_rung;
+        const unsigned long long amask = __ballot(active);
+        if (amask == 0ull) {
+            if constexpr (PERSIST)
+                continue;
+            else
+                return;
+        }
+        {
+            const int ox = (int)(px - rx), oy = (int)(py - ry), oz = (int)(pz - rz);
+            float gx = 0.f, gy = 0.f, gz = 0.f, ph = 0.f;
+            if (!(a.flags & TPM_SKIP_PP)) {
+                // Tile bounding box as floats.  float(int32) rounds by <= 64
+                // raw units, so box distances below carry <= 192 units of
+                // error, inside the 512-unit margin of cull2s: the test never
+                // drops a source with r < rcut to any sink.
+                const float lox = (float)wave_min(active ? ox : INT_MAX), hix = (float)wave_max(active ? ox : INT_MIN);
+                const float loy = (float)wave_min(active ? oy : INT_MAX), hiy = (float)wave_max(active ? oy : INT_MIN);
+                const float loz = (float)wave_min(active ? oz : INT_MAX), hiz = (float)wave_max(active ? oz : INT_MIN);
+                const int C = B.C;
+                const int cid = tile.x - B.cell0;
+                const int cx = cid % C, cy = (cid / C) % C, cz = cid / (C * C);
+                const bool cull = B.cull != 0;
+                const float cull2s = B.cull2s;
+                int nbuf = 0;
+                unsigned long long staged = 0ull;
+                // Far dummy that pads the list to a multiple of SPLIT: r >= rcut.
+                const int4 far = make_int4(0x40000000, 0x40000000, 0x40000000, -1);
+                auto flush = [&]() {
+                    const int padded = (nbuf + SPLIT - 1) / SPLIT * SPLIT;
+                    if (lane < padded - nbuf) sbuf[nbuf + lane] = far;
+                    __syncthreads();
+                    int4 s = sbuf[stream];
 #pragma unroll 2
-            for (int q = 0; q < padded; q += SPLIT) {
-                const int4 sn = sbuf[min(q + SPLIT, padded - SPLIT) + stream];
-                pp_pair(a, ox, oy, oz, i, s, gx, gy, gz, ph);
-                s = sn;
-            }
-            staged += (unsigned long long)nbuf;
-            nbuf = 0;
-            __syncthreads();
-        };
-        for (int nb = 0; nb < 27; nb++) {
-            int jx = cx + (nb % 3) - 1, jy = cy + (nb / 3) % 3 - 1, jz = cz + nb / 9 - 1;
-            jx += (jx < 0) ? C : 0;
-            jx -= (jx >= C) ? C : 0;
-            jy += (jy < 0) ? C : 0;
-            jy -= (jy >= C) ? C : 0;
-            jz += (jz < 0) ? C : 0;
-            jz -= (jz >= C) ? C : 0;
-            const int c2 = (jz * C + jy) * C + jx;
-            const int sb = a.cell_begin[c2], se = a.cell_begin[c2 + 1];
-            for (int j0 = sb; j0 < se; j0 += 64) {
-                const int j = j0 + lane;
-                bool keep = false;
-                int4 sv = make_int4(0, 0, 0, j);
-                if (j < se) {
-                    sv.x = (int)(a.x[j] - rx);
-                    sv.y = (int)(a.y[j] - ry);
-                    sv.z = (int)(a.z[j] - rz);
-                    const float fx = (float)sv.x, fy = (float)sv.y, fz = (float)sv.z;
-                    const float ex = fmaxf(fmaxf(lox - fx, fx - hix), 0.f);
-                    const float ey = fmaxf(fmaxf(loy - fy, fy - hiy), 0.f);
-                    const float ez = fmaxf(fmaxf(loz - fz, fz - hiz), 0.f);
-                    keep = !a.cull || fmaf(ex, ex, fmaf(ey, ey, ez * ez)) < a.cull2s;
+                    for (int qq = 0; qq < padded; qq += SPLIT) {
+                        const int4 sn = sbuf[min(qq + SPLIT, padded - SPLIT) + stream];
+                        pp_pair(B, ox, oy, oz, i, s, gx, gy, gz, ph);
+                        s = sn;
+                    }
+                    staged += (unsigned long long)nbuf;
+                    nbuf = 0;
+                    __syncthreads();
+                };
+                for (int nb = 0; nb < 27; nb++) {
+                    int jx = cx + (nb % 3) - 1, jy = cy + (nb / 3) % 3 - 1, jz = cz + nb / 9 - 1;
+                    jx += (jx < 0) ? C : 0;
+                    jx -= (jx >= C) ? C : 0;
+                    jy += (jy < 0) ? C : 0;
+                    jy -= (jy >= C) ? C : 0;
+                    jz += (jz < 0) ? C : 0;
+                    jz -= (jz >= C) ? C : 0;
+                    const int c2 = B.cell0 + (jz * C + jy) * C + jx;
+                    const int sb = a.cell_begin[c2], se = a.cell_begin[c2 + 1];
+                    for (int j0 = sb; j0 < se; j0 += 64) {
+                        const int j = j0 + lane;
+                        bool keep = false;
+                        int4 sv = make_int4(0, 0, 0, j);
+                        if (j < se) {
+                            sv.x = (int)(a.x[j] - rx);
+                            sv.y = (int)(a.y[j] - ry);
+                            sv.z = (int)(a.z[j] - rz);
+                            const float fx = (float)sv.x, fy = (float)sv.y, fz = (float)sv.z;
+                            const float ex = fmaxf(fmaxf(lox - fx, fx - hix), 0.f);
+                            const float ey = fmaxf(fmaxf(loy - fy, fy - hiy), 0.f);
+                            const float ez = fmaxf(fmaxf(loz - fz, fz - hiz), 0.f);
+                            keep = !cull || fmaf(ex, ex, fmaf(ey, ey, ez * ez)) < cull2s;
+                        }
+                        const unsigned long long m = __ballot(keep);
+                        if (keep) sbuf[nbuf + (int)count_below(m)] = sv;
+                        nbuf += __popcll(m);
+                        if (nbuf > SBUF - 2 * 64) flush();
+                    }
                 }
-                const unsigned long long m = __ballot(keep);
-                if (keep) sbuf[nbuf + (int)count_below(m)] = sv;
-                nbuf += __popcll(m);
-                if (nbuf > SBUF - 2 * 64) flush();
+                if (nbuf > 0) flush();
+                // sum the source streams
+#pragma unroll
+                for (int o = TS; o < 64; o <<= 1) {
+                    gx += __shfl_xor(gx, o);
+                    gy += __shfl_xor(gy, o);
+                    gz += __shfl_xor(gz, o);
+                    ph += __shfl_xor(ph, o);
+                }
+                if (lane == 0 && a.counters)
+                    atomicAdd(&a.counters[0],
+                              staged * (unsigned long long)__popcll(amask & ((TS == 64) ? ~0ull : ((1ull << TS) - 1))));
+                // the self pair (kick_treepm.cu:263-276 at r = 0): -35/16 h^-1
+                ph -= (35.0f / 16.0f) * B.hinv_s;
+                // back to box units: g in 2^-64, phi in 2^-32
+                gx *= 0x1p64f;
+                gy *= 0x1p64f;
+                gz *= 0x1p64f;
+                ph *= 0x1p32f;
             }
-        }
-        if (nbuf > 0) flush();
-        // sum the source streams
+            if constexpr (!PERSIST) {
+                if (!active || stream != 0) return;
+            } else {
+                if (!active || stream != 0) goto next_tile;
+            }
+            {
+                // Mesh force and potential (kick_treepm.cu:505-573) on the
+                // periodic mesh.
+                float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
+                if (a.phi && !(a.flags & TPM_SKIP_PM)) {
+                    const int M = B.M;
+                    const float* phi = a.phi + B.mesh0;
+                    const uint32_t pr[3] = {px, py, pz};
+                    int I0[3];
+                    float w[3][6], dw[3][6];
 #pragma unroll
-        for (int o = TS; o < 64; o <<= 1) {
-            gx += __shfl_xor(gx, o);
-            gy += __shfl_xor(gy, o);
-            gz += __shfl_xor(gz, o);
-            ph += __shfl_xor(ph, o);
-        }
-        if (lane == 0 && a.counters)
-            atomicAdd(&a.counters[0], staged * (unsigned long long)__popcll(amask & ((TS == 64) ? ~0ull : ((1ull << TS) - 1))));
-        // the self pair (kick_treepm.cu:263-276 at r = 0): -35/16 h^-1
-        ph -= (35.0f / 16.0f) * a.hinv_s;
-        // back to box units: g in 2^-64, phi in 2^-32
-        gx *= 0x1p64f;
-        gy *= 0x1p64f;
-        gz *= 0x1p64f;
-        ph *= 0x1p32f;
-    }
-    if (!active || stream != 0) return;
+                    for (int d = 0; d < 3; d++) {
+                        const uint64_t P = (uint64_t)pr[d] * (uint64_t)M;
+                        I0[d] = (int)(P >> 32) - 2 + M;
+                        const float X = (float)((uint32_t)P >> 8) * 0x1p-24f;
+                        interp6_weights(X, w[d], dw[d]);
+                    }
+                    const float Mf = (float)M;
+                    for (int aa = 0; aa < 6; aa++) {
+                        int jx = I0[0] + aa;
+                        jx -= (jx >= M) ? M : 0;
+                        jx -= (jx >= M) ? M : 0;
+                        for (int bb = 0; bb < 6; bb++) {
+                            int jy = I0[1] + bb;
+                            jy -= (jy >= M) ? M : 0;
+                            jy -= (jy >= M) ? M : 0;
+                            const float* row = phi + ((int64_t)jx * M + jy) * M;
+                            const float wxy = w[0][aa] * w[1][bb];
+                            const float dxy = dw[0][aa] * w[1][bb];
+                            const float xdy = w[0][aa] * dw[1][bb];
+#pragma unroll
+                            for (int cc = 0; cc < 6; cc++) {
+                                int jz = I0[2] + cc;
+                                jz -= (jz >= M) ? M : 0;
+                                jz -= (jz >= M) ? M : 0;
+                                const float p0 = row[jz];
+                                const float p1 = p0 * Mf;
+                                pgx = fmaf(-dxy * w[2][cc], p1, pgx);
+                                pgy = fmaf(-xdy * w[2][cc], p1, pgy);
+                                pgz = fmaf(-wxy * dw[2][cc], p1, pgz);
+                                pph = fmaf(wxy * w[2][cc], p0, pph);
+                            }
+                        }
+                    }
+                }
 
-    // Mesh force and potential (kick_treepm.cu:505-573) on the periodic mesh.
-    float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
-    if (a.phi && !(a.flags & TPM_SKIP_PM)) {
-        const int M = a.M;
-        const uint32_t pr[3] = {px, py, pz};
-        int I0[3];
-        float w[3][6], dw[3][6];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            const uint64_t P = (uint64_t)pr[d] * (uint64_t)M;
-            I0[d] = (int)(P >> 32) - 2 + M;
-            const float X = (float)((uint32_t)P >> 8) * 0x1p-24f;
-            interp6_weights(X, w[d], dw[d]);
-        }
-        const float Mf = (float)M;
-        for (int aa = 0; aa < 6; aa++) {
-            int jx = I0[0] + aa;
-            jx -= (jx >= M) ? M : 0;
-            jx -= (jx >= M) ? M : 0;
-            for (int bb = 0; bb < 6; bb++) {
-                int jy = I0[1] + bb;
-                jy -= (jy >= M) ? M : 0;
-                jy -= (jy >= M) ? M : 0;
-                const float* row = a.phi + ((int64_t)jx * M + jy) * M;
-                const float wxy = w[0][aa] * w[1][bb];
-                const float dxy = dw[0][aa] * w[1][bb];
-                const float xdy = w[0][aa] * dw[1][bb];
-#pragma unroll
-                for (int cc = 0; cc < 6; cc++) {
-                    int jz = I0[2] + cc;
-                    jz -= (jz >= M) ? M : 0;
-                    jz -= (jz >= M) ? M : 0;
-                    const float p0 = row[jz];
-                    const float p1 = p0 * Mf;
-                    pgx = fmaf(-dxy * w[2][cc], p1, pgx);
-                    pgy = fmaf(-xdy * w[2][cc], p1, pgy);
-                    pgz = fmaf(-wxy * dw[2][cc], p1, pgz);
-                    pph = fmaf(wxy * w[2][cc], p0, pph);
+                // Kick epilogue (kick_treepm.cu:716-754); rung_dt[r] = 2^-r.
+                const float GM = B.GM;
+                const float g0 = (pgx + gx) * GM;
+                const float g1 = (pgy + gy) * GM;
+                const float g2 = (pgz + gz) * GM;
+                a.gx[i] = g0;
+                a.gy[i] = g1;
+                a.gz[i] = g2;
+                a.pot[i] = B.do_phi ? (B.phi0 + pph + ph) * GM : 0.f;
+                if (!(a.flags & TPM_SKIP_KICK)) {
+                    const float t0 = B.t0;
+                    float vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
+                    float dt = 0.5f * ldexpf(1.0f, -r) * t0;
+                    if (!B.first_call) {
+                        vx = fmaf(g0, dt, vx);
+                        vy = fmaf(g1, dt, vy);
+                        vz = fmaf(g2, dt, vz);
+                    }
+                    const float gg = fmaf(g0, g0, fmaf(g1, g1, g2 * g2));
+                    const float factor = B.eta * sqrtf(B.scale * B.hsoft);
+                    dt = fminf(factor * (1.0f / sqrtf(sqrtf(gg))), t0);
+                    float nr = fmaxf(ceilf(log2f(t0) - log2f(dt)), (float)(r - 1));
+                    nr = fminf(fmaxf(nr, 0.f), 31.f);
+                    r = (int)nr;
+                    dt = 0.5f * ldexpf(1.0f, -r) * t0;
+                    vx = fmaf(g0, dt, vx);
+                    vy = fmaf(g1, dt, vy);
+                    vz = fmaf(g2, dt, vz);
+                    a.vx[i] = vx;
+                    a.vy[i] = vy;
+                    a.vz[i] = vz;
+                    a.rung[i] = (int8_t)r;
+                    if (a.counters) atomicMax(&a.counters[1], (unsigned long long)r);
                 }
             }
         }
+    next_tile:
+        if constexpr (!PERSIST) return;
     }
-
-    // Kick epilogue (kick_treepm.cu:716-754); rung_dt[r] = 2^-r.
-    const float GM = a.GM;
-    const float g0 = (pgx + gx) * GM;
-    const float g1 = (pgy + gy) * GM;
-    const float g2 = (pgz + gz) * GM;
-    const float pt = a.do_phi ? (a.phi0 + pph + ph) * GM : 0.f;
-    a.gx[i] = g0;
-    a.gy[i] = g1;
-    a.gz[i] = g2;
-    a.pot[i] = pt;
-    if (a.flags & TPM_SKIP_KICK) return;
-    float vx = a.vx[i], vy = a.vy[i], vz = a.vz[i];
-    float dt = 0.5f * ldexpf(1.0f, -r) * a.t0;
-    if (!a.first_call) {
-        vx = fmaf(g0, dt, vx);
-        vy = fmaf(g1, dt, vy);
-        vz = fmaf(g2, dt, vz);
-    }
-    const float gg = fmaf(g0, g0, fmaf(g1, g1, g2 * g2));
-    const float factor = a.eta * sqrtf(a.scale * a.hsoft);
-    dt = fminf(factor * (1.0f / sqrtf(sqrtf(gg))), a.t0);
-    float nr = fmaxf(ceilf(log2f(a.t0) - log2f(dt)), (float)(r - 1));
-    nr = fminf(fmaxf(nr, 0.f), 31.f);
-    r = (int)nr;
-    dt = 0.5f * ldexpf(1.0f, -r) * a.t0;
-    vx = fmaf(g0, dt, vx);
-    vy = fmaf(g1, dt, vy);
-    vz = fmaf(g2, dt, vz);
-    a.vx[i] = vx;
-    a.vy[i] = vy;
-    a.vz[i] = vz;
-    a.rung[i] = (int8_t)r;
-    if (a.counters) atomicMax(&a.counters[1], (unsigned long long)r);
 }
 
-// W_sr = sum_c n_c * sum_{27 neighbours} n_c' (exact, 64-bit integer).
-__global__ void candidate_pairs_kernel(int C, const int32_t* __restrict__ cell_begin,
-                                       unsigned long long* __restrict__ out) {
-    const int64_t ncell = (int64_t)C * C * C;
+// W_sr = sum_c n_c * sum_{27 neighbours} n_c' (exact, 64-bit integer) over
+// the global cells [0, ncell); with a box table, cell c belongs to the last
+// box whose cell0 <= c.
+__global__ void candidate_pairs_kernel(int64_t ncell, int nbox, const tpm_box_dev* __restrict__ boxes, int C0,
+                                       const int32_t* __restrict__ cell_begin, unsigned long long* __restrict__ out) {
     unsigned long long acc = 0;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
-        const int cx = (int)(c % C), cy = (int)((c / C) % C), cz = (int)(c / ((int64_t)C * C));
+        int C = C0;
+        int64_t base = 0;
+        if (boxes) {
+            const int b = tpm_upper_index(nbox, c, [&](int k) { return (int64_t)boxes[k].cell0; });
+            C = boxes[b].C;
+            base = boxes[b].cell0;
+        }
+        const int64_t l = c - base;
+        const int cx = (int)(l % C), cy = (int)((l / C) % C), cz = (int)(l / ((int64_t)C * C));
         const int64_t nc = cell_begin[c + 1] - cell_begin[c];
         int64_t s = 0;
         for (int nb = 0; nb < 27; nb++) {
             const int jx = (cx + nb % 3 - 1 + C) % C, jy = (cy + (nb / 3) % 3 - 1 + C) % C,
                       jz = (cz + nb / 9 - 1 + C) % C;
-            const int64_t c2 = ((int64_t)jz * C + jy) * C + jx;
+            const int64_t c2 = base + ((int64_t)jz * C + jy) * C + jx;
             s += cell_begin[c2 + 1] - cell_begin[c2];
         }
         acc += (unsigned long long)(nc * s);
@@ -358,7 +416,73 @@ __global__ void candidate_pairs_kernel(int C, const int32_t* __restrict__ cell_b
     atomicAdd(out, acc);
 }
 
+// Drift of every live particle by v * t0 of its box (fixed32, periodic):
+// raw += round(v t0 2^32) mod 2^32.
+__global__ void drift_kernel(int64_t n, int nbox, const tpm_box_dev* __restrict__ boxes, uint32_t* __restrict__ x,
+                             uint32_t* __restrict__ y, uint32_t* __restrict__ z, const float* __restrict__ vx,
+                             const float* __restrict__ vy, const float* __restrict__ vz) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = tpm_upper_index(nbox, i, [&](int k) { return boxes[k].q0; });
+        const double f = boxes[b].drift * 4294967296.0;
+        x[i] += (uint32_t)(uint64_t)llrint((double)vx[i] * f);
+        y[i] += (uint32_t)(uint64_t)llrint((double)vy[i] * f);
+        z[i] += (uint32_t)(uint64_t)llrint((double)vz[i] * f);
+    }
+}
+
+template <bool BATCH>
+static void pp_launch(int ts, bool persistent, unsigned grid, hipStream_t s, const PPArgs& a) {
+    if (persistent) {
+        switch (ts) {
+            case 16: pp_kick_kernel<16, BATCH, true><<<grid, 64, 0, s>>>(a); break;
+            case 64: pp_kick_kernel<64, BATCH, true><<<grid, 64, 0, s>>>(a); break;
+            default: pp_kick_kernel<32, BATCH, true><<<grid, 64, 0, s>>>(a); break;
+        }
+    } else {
+        switch (ts) {
+            case 16: pp_kick_kernel<16, BATCH, false><<<grid, 64, 0, s>>>(a); break;
+            case 64: pp_kick_kernel<64, BATCH, false><<<grid, 64, 0, s>>>(a); break;
+            default: pp_kick_kernel<32, BATCH, false><<<grid, 64, 0, s>>>(a); break;
+        }
+    }
+}
+
 }  // namespace
+
+void tpm_box_fill(tpm_box_dev* b, const tpm_params& p, int min_rung, int first_call, double t0, double scale) {
+    b->C = p.chain_dim;
+    b->M = p.mesh_dim;
+    b->ninv = 1.0 / p.chain_dim;
+    b->min_rung = min_rung;
+    b->first_call = first_call ? 1 : 0;
+    b->do_phi = p.do_phi;
+    // Constants of compute_pp_interaction (kick_treepm.cu:833-853), scaled
+    // to 2^-32 units: lengths x 2^32, squares x 2^64.
+    const float rcut = (float)p.rcut, rs = (float)p.rs, h = (float)p.hsoft;
+    const float S = 0x1p32f;
+    b->rcut2s = (rcut * rcut) * (S * S);
+    // Culling keeps a margin of 512 raw units over rcut (float rounding of
+    // the box distance stays below ~200); off for C < 5, where a source's
+    // offset from the tile origin can wrap to the far image.
+    const double cull = (double)p.rcut * 4294967296.0 + 512.0;
+    b->e2s = (float)(-1.4426950408889634 / (4.0 * p.rs * p.rs) / (4294967296.0 * 4294967296.0));
+    b->cull2s = (float)(cull * cull);
+    b->cull = p.chain_dim >= 5 ? 1 : 0;
+    b->inv2rs_s = (1.0f / rs / 2.0f) / S;
+    b->tsp = 2.0f / sqrtf((float)M_PI);
+    b->h2s = (h * h) * (S * S);
+    const float hinv = 1.f / h;
+    b->hinv_s = hinv / S;
+    b->h2inv_s = (hinv * hinv) / (S * S);
+    b->h3inv_s = ((hinv * hinv * hinv) / (S * S)) / S;
+    b->GM = (float)p.GM;
+    b->phi0 = (float)(p.n_total * 4.0 * M_PI * p.rs * p.rs + (35.0 / 16.0) / p.hsoft);
+    b->eta = (float)p.eta;
+    b->t0 = (float)t0;
+    b->scale = (float)scale;
+    b->hsoft = (float)p.hsoft;
+    b->drift = t0;
+}
 
 tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                        tpm_stats* stats) {
@@ -369,11 +493,10 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
         return tpm_fail(ctx, TPM_ERR_STATE, "tpm_kick: PM mesh not computed (call tpm_gravity_long_compute)");
     if (t0 <= 0.0) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: t0 must be positive");
     hipStream_t s = ctx->stream;
-    const tpm_params& p = ctx->p;
     if (!ctx->counters) TPM_HIP(ctx, hipMalloc(&ctx->counters, sizeof(unsigned long long) * 4));
     TPM_TRY(tpm_fill_u64(ctx, ctx->counters, 4));
 
-    PPArgs a;
+    PPArgs a{};
     a.x = ctx->x;
     a.y = ctx->y;
     a.z = ctx->z;
@@ -390,37 +513,12 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     a.ntiles = ctx->ntiles;
     a.phi = use_pm ? ctx->phi : nullptr;
     a.counters = ctx->counters;
-    a.C = ctx->C;
-    a.M = ctx->M;
-    a.min_rung = min_rung;
-    a.first_call = first_call ? 1 : 0;
-    a.do_phi = p.do_phi;
+    a.queue = nullptr;
+    a.boxes = nullptr;
     a.flags = flags;
-    // Constants of compute_pp_interaction (kick_treepm.cu:833-853), scaled
-    // to 2^-32 units: lengths x 2^32, squares x 2^64.
-    const float rcut = (float)p.rcut, rs = (float)p.rs, h = (float)p.hsoft;
-    const float S = 0x1p32f;
-    a.rcut2s = (rcut * rcut) * (S * S);
-    // Culling keeps a margin of 512 raw units over rcut (float rounding of
-    // the box distance stays below ~200); off for C < 5, where a source's
-    // offset from the tile origin can wrap to the far image.
-    const double cull = (double)p.rcut * 4294967296.0 + 512.0;
-    a.e2s = (float)(-1.4426950408889634 / (4.0 * p.rs * p.rs) / (4294967296.0 * 4294967296.0));
-    a.cull2s = (float)(cull * cull);
-    a.cull = ctx->C >= 5 ? 1 : 0;
-    a.inv2rs_s = (1.0f / rs / 2.0f) / S;
-    a.tsp = 2.0f / sqrtf((float)M_PI);
-    a.h2s = (h * h) * (S * S);
-    const float hinv = 1.f / h;
-    a.hinv_s = hinv / S;
-    a.h2inv_s = (hinv * hinv) / (S * S);
-    a.h3inv_s = ((hinv * hinv * hinv) / (S * S)) / S;
-    a.GM = (float)p.GM;
-    a.phi0 = (float)(p.n_total * 4.0 * M_PI * p.rs * p.rs + (35.0 / 16.0) / p.hsoft);
-    a.eta = (float)p.eta;
-    a.t0 = (float)t0;
-    a.scale = (float)scale;
-    a.hsoft = (float)p.hsoft;
+    tpm_box_fill(&a.box0, ctx->p, min_rung, first_call, t0, scale);
+    a.box0.cell0 = 0;
+    a.box0.mesh0 = 0;
 
     const int64_t ncell = (int64_t)ctx->C * ctx->C * ctx->C;
     const int64_t tiles_max = ctx->n / ctx->tile + ncell + 1;
@@ -438,16 +536,13 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     }
     TPM_HIP(ctx, hipEventRecord(e0, s));
     if (ctx->n > 0) {
-        switch (ctx->tile) {
-            case 16: pp_kick_kernel<16><<<(unsigned)blocks, 64, 0, s>>>(a); break;
-            case 64: pp_kick_kernel<64><<<(unsigned)blocks, 64, 0, s>>>(a); break;
-            default: pp_kick_kernel<32><<<(unsigned)blocks, 64, 0, s>>>(a); break;
-        }
+        pp_launch<false>(ctx->tile, false, (unsigned)blocks, s, a);
         TPM_HIP(ctx, hipGetLastError());
     }
     TPM_HIP(ctx, hipEventRecord(e1, s));
     if (stats) {
-        candidate_pairs_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ctx->C, ctx->cell_begin, ctx->counters + 2);
+        candidate_pairs_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, 0, nullptr, ctx->C, ctx->cell_begin,
+                                                                    ctx->counters + 2);
         TPM_HIP(ctx, hipGetLastError());
         unsigned long long hc[4];
         int32_t nt = 0;
@@ -463,4 +558,69 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
         stats->n_tiles = nt;
     }
     return TPM_OK;
+}
+
+// The fused kernel over the tiles of every live box of a batch.
+tpm_status tpm_batch_pp(tpm_batch* bt, const tpm_batch_layout& L, unsigned flags) {
+    if (L.n_live == 0 || L.ncell == 0) return TPM_OK;
+    PPArgs a{};
+    a.x = bt->x;
+    a.y = bt->y;
+    a.z = bt->z;
+    a.vx = bt->vx;
+    a.vy = bt->vy;
+    a.vz = bt->vz;
+    a.rung = bt->rung;
+    a.gx = bt->gx;
+    a.gy = bt->gy;
+    a.gz = bt->gz;
+    a.pot = bt->pot;
+    a.cell_begin = bt->cell_begin;
+    a.tiles = bt->tiles;
+    a.ntiles = bt->tile_off + L.ncell;
+    a.phi = (flags & TPM_SKIP_PM) ? nullptr : bt->phi;
+    a.counters = bt->counters;
+    a.queue = reinterpret_cast<int*>(bt->counters + 8);
+    a.boxes = bt->d_boxes;
+    a.flags = flags & (TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK);
+    const bool persistent = bt->tune.sched == 1;
+    int64_t grid = persistent ? bt->persist_grid : (L.tiles_max + 7) / 8 * 8;
+    if (grid < 8) grid = 8;
+    if (grid > INT_MAX) return tpm_fail(bt, TPM_ERR_INVALID, "tpm_batch_step: too many tiles");
+    pp_launch<true>(bt->tune.tile_sinks, persistent, (unsigned)grid, bt->stream, a);
+    TPM_HIP(bt, hipGetLastError());
+    return TPM_OK;
+}
+
+tpm_status tpm_batch_candidates(tpm_batch* bt, const tpm_batch_layout& L) {
+    if (L.ncell == 0) return TPM_OK;
+    candidate_pairs_kernel<<<grid_for(L.ncell, 256), 256, 0, bt->stream>>>(L.ncell, L.nbox, bt->d_boxes, 0,
+                                                                           bt->cell_begin, bt->counters + 2);
+    TPM_HIP(bt, hipGetLastError());
+    return TPM_OK;
+}
+
+tpm_status tpm_batch_drift(tpm_batch* bt, const tpm_batch_layout& L) {
+    if (L.n_live == 0) return TPM_OK;
+    drift_kernel<<<grid_for(L.n_live, 256), 256, 0, bt->stream>>>(L.n_live, L.nbox, bt->d_boxes, bt->x, bt->y, bt->z,
+                                                                  bt->vx, bt->vy, bt->vz);
+    TPM_HIP(bt, hipGetLastError());
+    return TPM_OK;
+}
+
+// Workgroups of one persistent PP launch: resident workgroups per CU x CUs,
+// a multiple of the 8 XCDs.
+int tpm_pp_persistent_grid(int device, int ts) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
+    int per_cu = 0;
+    hipError_t e;
+    switch (ts) {
+        case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<16, true, true>, 64, 0); break;
+        case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<64, true, true>, 64, 0); break;
+        default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<32, true, true>, 64, 0); break;
+    }
+    if (e != hipSuccess || per_cu < 1) per_cu = 8;
+    const int g = per_cu * prop.multiProcessorCount;
+    return (g + 7) / 8 * 8;
 }
