@@ -201,6 +201,9 @@ def ensemble_leg(T, steps, seed=1234, nbox=64):
     t = time.perf_counter()
     best, ms_best = bt.autotune(reps=2)
     t_tune = time.perf_counter() - t
+    stage = bt.step(0, stats=True)  # eager: stage breakdown
+    log("ensemble batch step (eager) %.1f ms: chain mesh %.1f, PM %.1f, PP %.1f"
+        % (stage.ms_step, stage.ms_chainmesh, stage.ms_pm, stage.ms_pp))
     best["graph"] = 1
     bt.set_tuning(**best)
     bt.step(0, stats=False)
@@ -236,7 +239,10 @@ def ensemble_leg(T, steps, seed=1234, nbox=64):
         "batch_kicks_per_s": batch, "speedup_vs_plain_loop": batch / plain,
         "speedup_vs_plain_loop_no_wait": batch / plain_async,
         "speedup_vs_host_resident_loop": batch / plain_host,
-        "tuning": best, "autotune_s": t_tune, "step_ms_p50": 1e3 * float(np.percentile(lat, 50)),
+        "tuning": best, "autotune_s": t_tune,
+        "eager_step_ms": {"step": stage.ms_step, "chainmesh": stage.ms_chainmesh, "pm": stage.ms_pm,
+                          "pp": stage.ms_pp},
+        "step_ms_p50": 1e3 * float(np.percentile(lat, 50)),
         "step_ms_p99": 1e3 * float(np.percentile(lat, 99)),
         "candidate_pairs": st.candidate_pairs, "executed_pairs": st.executed_pairs,
         "pp_credited_tflops": FLOPS_PER_PAIR * st.candidate_pairs / (st.ms_step * 1e-3) / 1e12,
@@ -252,7 +258,8 @@ def serving_leg(T, rates, count, budget_gib, seed=77):
     out = {"budget_gib": budget_gib, "kicks_per_request": 10, "rates": []}
     lim = S.budget_limits(budget_gib * (1 << 30))
     for rate in rates:
-        reqs = S.poisson_requests(count, rate, seed + int(rate * 10))
+        # the same boxes at every rate; only the arrival times scale
+        reqs = S.poisson_requests(count, rate, seed)
         with T.Batch(**lim) as bt:
             srv = S.Server(bt, kicks=10)
             t_prep = srv.prepare(reqs)
@@ -298,7 +305,9 @@ def main():
     ap.add_argument("--no-pcie", action="store_true", help="skip the host-buffer (PCIe-inclusive) measurement")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-box batched-ensemble leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the continuous-batching serving leg")
-    ap.add_argument("--serving-rates", default="4,8,12", help="Poisson arrival rates (requests/s)")
+    ap.add_argument("--only", choices=["ensemble", "serving"],
+                    help="run only this leg and print its result (for profiling; not the benchmark line)")
+    ap.add_argument("--serving-rates", default="4,8,12,16", help="Poisson arrival rates (requests/s)")
     ap.add_argument("--serving-requests", type=int, default=40)
     ap.add_argument("--serving-budget-gib", type=float, default=16.0)
     ap.add_argument("--probe-copy", action="store_true",
@@ -309,6 +318,14 @@ def main():
               "described in DESIGN.md, not implemented)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.only == "ensemble":
+        print(json.dumps({"ensemble": ensemble_leg(T, max(args.steps, 3))}), flush=True)
+        return
+    if args.only == "serving":
+        rates = [float(r) for r in args.serving_rates.split(",") if r]
+        print(json.dumps({"serving": serving_leg(T, rates, args.serving_requests, args.serving_budget_gib)}),
+              flush=True)
+        return
     w = WORKLOADS[args.workload]
     ctx = T.Context(0)
     p = ctx.set_params(T.baseline_params(w["n_side"], w["mesh"], w["order"], interlace=w["interlace"]))

@@ -92,7 +92,7 @@ typedef struct {
 /* Launch-shape tuning (no reference counterpart: the reference fixes 32-thread
  * blocks and 128-sink buckets, tigerpm.hpp:29-32).  Results do not depend on
  * it beyond FP32 summation order.
- *   tile_sinks:    sinks per PP work item, 16, 32 or 64 (one wave64 walks
+ *   tile_sinks:    sinks per PP work item, 8, 16, 32 or 64 (one wave64 walks
  *                  64/tile_sinks source streams of one culled source list);
  *   deposit_block: chain cells per edge of an LDS-privatised deposit block,
  *                  0 = largest that fits 64 KiB, -1 = per-particle global
@@ -227,7 +227,7 @@ typedef struct {
 
 /* Launch shape of the batched step (results do not depend on it beyond FP32
  * summation order):
- *   tile_sinks:    16, 32 or 64 sinks per PP work item;
+ *   tile_sinks:    8, 16, 32 or 64 sinks per PP work item;
  *   deposit_block: chain cells per deposit block edge, 0 = auto;
  *   sched:         0 = one workgroup per tile (XCD-aware order),
  *                  1 = persistent workgroups taking tiles from per-XCD queues;
@@ -244,6 +244,9 @@ typedef struct {
     double candidate_pairs; /* W_sr summed over the boxes */
     double executed_pairs;
     double ms_step;         /* HIP-event time of the step on the batch stream */
+    double ms_chainmesh;    /* stage times of a step run without a graph; */
+    double ms_pm;           /* -1 when the step was a graph replay        */
+    double ms_pp;
     int64_t n_particles;
     int32_t n_boxes;
     int32_t max_rung;

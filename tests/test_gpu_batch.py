@@ -102,30 +102,45 @@ def test_alone_equals_mix_with_graph_replays():
 
 
 @pytest.mark.parametrize("sched", [0, 1])
-def test_batch_kick_vs_ewald(sched):
-    """Interlaced TSC boxes kicked in a batch meet the 1e-3 rms force error
-    against the FP64 Ewald sum (both schedules of the PP launch)."""
+def test_batch_kick_vs_oracle_and_ewald(sched):
+    """Interlaced TSC boxes kicked in a batch (both schedules of the PP
+    launch): forces equal the CPU oracle's kick of the same particles to
+    1e-5, and the rms error against the FP64 Ewald sum stays below 2e-3.
+    (With the cutoff at 4.5 rs the short-range force is still 1.75% of the
+    Newtonian force at rcut; for weakly clustered boxes of a few 10^4
+    particles the oracle itself is 0.7e-3 to 1.7e-3 off Ewald, and 0.7e-3
+    once rcut = 6 rs, so 1e-3 is a property of the configuration, not a
+    bound every small box meets.)"""
     with new_batch() as bt:
         bt.set_tuning(tile_sinks=16, sched=sched)
-        ids = [bt.add_zeldovich(spec(32), 32, 21, 0.5), bt.add_zeldovich(spec(26), 26, 22, 1.5)]
+        boxes = [(32, 21, 1.5), (26, 22, 2.0)]
+        ids = [bt.add_zeldovich(spec(ns), ns, seed, rms) for ns, seed, rms in boxes]
+        before = [bt.download(i) for i in ids]
         bt.step(0)
-        for i in ids:
+        for i, b0, (ns, _, _) in zip(ids, before, boxes):
             d = bt.download(i)
+            p = spec(ns).params
+            op = O.make_params(p.mesh_dim, p.chain_dim, p.order, p.rs, p.rcut, p.hsoft, p.GM, p.eta, p.n_total,
+                               1, 1)
+            r = O.kick_full(op, b0["x"], b0["y"], b0["z"], b0["vx"], b0["vy"], b0["vz"], b0["rung"], t0=T0)
+            assert np.array_equal(d["x"], r["x"])
+            G = np.stack([d["gx"], d["gy"], d["gz"]], 1).astype(np.float64)
+            R = np.stack([r["gx"], r["gy"], r["gz"]], 1).astype(np.float64)
+            assert np.sqrt(((G - R) ** 2).sum() / (R ** 2).sum()) < 1e-5
             with T.Context(0) as c:
                 bt.to_ctx(i, c)
                 s = np.sort(np.random.default_rng(i).choice(len(d["x"]), 384, replace=False))
                 ph, gx, gy, gz = c.ewald(s)
-                G = np.stack([d["gx"][s], d["gy"][s], d["gz"][s]], 1).astype(np.float64)
                 E = np.stack([gx, gy, gz], 1) * c.params.GM
-                err = np.sqrt(((G - E) ** 2).sum() / (E ** 2).sum())
-                assert err < 1e-3, err
+                err = np.sqrt(((G[s] - E) ** 2).sum() / (E ** 2).sum())
+                assert err < 2e-3, err
 
 
 def test_schedules_and_tiles_agree():
     """Persistent and one-workgroup-per-tile launches give the same bits;
     other tile sizes agree to FP32 summation order."""
     ref = None
-    for ts, sched in ((16, 0), (16, 1), (32, 0), (64, 1)):
+    for ts, sched in ((16, 0), (16, 1), (8, 0), (32, 0), (64, 1)):
         with new_batch() as bt:
             bt.set_tuning(tile_sinks=ts, sched=sched)
             i = bt.add_zeldovich(spec(30), 30, 31, 1.2)
@@ -151,7 +166,7 @@ def test_autotune_leaves_state():
         bt.step(0)
         before = [bt.download(i) for i in ids]
         best, ms = bt.autotune(reps=1)
-        assert best["tile_sinks"] in (16, 32, 64) and best["sched"] in (0, 1) and ms > 0
+        assert best["tile_sinks"] in (8, 16, 32, 64) and best["sched"] in (0, 1) and ms > 0
         assert bt.get_tuning() == best
         for i, b in zip(ids, before):
             d = bt.download(i)

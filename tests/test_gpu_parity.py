@@ -136,7 +136,8 @@ def test_full_kick_interlaced_vs_oracle(gpu_ctx):
     assert rel_rms(G[s], np.stack([gx, gy, gz], 1) * p.GM) < 1e-3
 
 
-@pytest.mark.parametrize("dist,tile", [("uniform", 32), ("clustered", 32), ("clustered", 16), ("uniform", 64)])
+@pytest.mark.parametrize("dist,tile", [("uniform", 32), ("clustered", 32), ("clustered", 16), ("uniform", 64),
+                                       ("clustered", 8)])
 def test_pp_only_vs_oracle(gpu_ctx, dist, tile):
     n_side = 32
     n = n_side ** 3

@@ -72,7 +72,8 @@ class TpmBatchTuning(ctypes.Structure):
 
 class TpmBatchStats(ctypes.Structure):
     _fields_ = [("candidate_pairs", ctypes.c_double), ("executed_pairs", ctypes.c_double),
-                ("ms_step", ctypes.c_double), ("n_particles", ctypes.c_int64), ("n_boxes", ctypes.c_int32),
+                ("ms_step", ctypes.c_double), ("ms_chainmesh", ctypes.c_double), ("ms_pm", ctypes.c_double),
+                ("ms_pp", ctypes.c_double), ("n_particles", ctypes.c_int64), ("n_boxes", ctypes.c_int32),
                 ("max_rung", ctypes.c_int32)]
 
     def as_dict(self):

@@ -106,8 +106,8 @@ extern "C" tpm_status tpm_get_params(const tpm_ctx* ctx, tpm_params* p) {
 
 extern "C" tpm_status tpm_set_tuning(tpm_ctx* ctx, const tpm_tuning* t) {
     if (!ctx || !t) return TPM_ERR_INVALID;
-    if (t->tile_sinks != 16 && t->tile_sinks != 32 && t->tile_sinks != 64)
-        return tpm_fail(ctx, TPM_ERR_INVALID, "tile_sinks must be 16, 32 or 64");
+    if (t->tile_sinks != 8 && t->tile_sinks != 16 && t->tile_sinks != 32 && t->tile_sinks != 64)
+        return tpm_fail(ctx, TPM_ERR_INVALID, "tile_sinks must be 8, 16, 32 or 64");
     if (t->deposit_block < -1 || t->deposit_block > 8)
         return tpm_fail(ctx, TPM_ERR_INVALID, "deposit_block must be in [-1, 8]");
     ctx->tile = t->tile_sinks;

@@ -33,7 +33,7 @@ constexpr int64_t K_ALIGN = 32;     // modes (256 B of float2)
 
 int64_t kmodes(int M) { return (int64_t)M * M * (M / 2 + 1); }
 int64_t cells_of(int C) { return (int64_t)C * C * C; }
-int64_t tiles_of(int64_t n, int C) { return n / 16 + cells_of(C) + 1; }
+int64_t tiles_of(int64_t n, int C) { return n / 8 + cells_of(C) + 1; }  // any tile size
 
 struct Caps {
     int64_t p, m, k, c, t, nbox;
@@ -48,14 +48,14 @@ Caps caps_of(const tpm_batch_limits& l) {
     c.k = l.max_mesh_nodes / 4 * 3 + K_ALIGN * (l.max_boxes + 1);
     c.m += MESH_ALIGN * (l.max_boxes + 1);
     c.c = l.max_cells > 0 ? l.max_cells : l.max_mesh_nodes / 64 + 4096;
-    c.t = c.p / 16 + c.c + l.max_boxes + 64;
+    c.t = c.p / 8 + c.c + l.max_boxes + 64;
     return c;
 }
 
 int64_t arena_bytes(const tpm_batch_limits& l) {
     const Caps c = caps_of(l);
     const int64_t per_p = 12 * 2 + 12 * 2 + 2 + 16 + 16;  // positions, velocities (x2), rungs, g/phi, sort
-    const int64_t per_m = 8 + 4 + 4;                      // Q32 density, float density, phi
+    const int64_t per_m = 8 + 4 + 4 + (l.interlace ? 4 : 0);  // Q32 density, float densities, phi
     const int64_t per_k = 8 + 4 + (l.interlace ? 8 : 0);  // spectrum, Green's, shifted spectrum
     const int64_t per_c = 4 * 4;                          // counts, cell_begin, tiles per cell, tile offsets
     return c.p * per_p + c.m * per_m + c.k * per_k + (c.c + 2) * per_c + c.t * 16 +
@@ -146,10 +146,17 @@ tpm_status build_layout(tpm_batch* bt, tpm_batch_layout* L) {
 // The kernels of one step, in stream order (capturable: kernels and rocFFT
 // executions only).
 tpm_status enqueue_step(tpm_batch* bt, const tpm_batch_layout& L, unsigned flags) {
+    auto mark = [&](int k) -> tpm_status {
+        if (bt->stage_events) TPM_HIP(bt, hipEventRecord(bt->ev_stage[k], bt->stream));
+        return TPM_OK;
+    };
     TPM_TRY(tpm_batch_fill_u64(bt, bt->counters, 16));
     TPM_TRY(tpm_batch_chainmesh(bt, L));
+    TPM_TRY(mark(0));
     if (!(flags & TPM_SKIP_PM)) TPM_TRY(tpm_batch_pm(bt, L));
+    TPM_TRY(mark(1));
     TPM_TRY(tpm_batch_pp(bt, L, flags));
+    TPM_TRY(mark(2));
     if (flags & TPM_BATCH_DRIFT) TPM_TRY(tpm_batch_drift(bt, L));
     return TPM_OK;
 }
@@ -228,9 +235,13 @@ tpm_status run_step(tpm_batch* bt, unsigned flags, tpm_batch_stats* st) {
         TPM_HIP(bt, hipGraphLaunch(bt->graph_exec[par], bt->stream));
         swap_sets(bt);
     } else {
-        TPM_TRY(enqueue_step(bt, L, flags));
+        bt->stage_events = st != nullptr;
+        const tpm_status s = enqueue_step(bt, L, flags);
+        bt->stage_events = false;
+        TPM_TRY(s);
     }
     TPM_HIP(bt, hipEventRecord(bt->ev1, bt->stream));
+    const bool staged = st != nullptr && !(bt->tune.graph && L.n_live > 0);
     after_sort(bt, !(flags & TPM_SKIP_KICK));
     if (st) {
         TPM_TRY(tpm_batch_candidates(bt, L));
@@ -243,6 +254,16 @@ tpm_status run_step(tpm_batch* bt, unsigned flags, tpm_batch_stats* st) {
         st->max_rung = (int32_t)hc[1];
         st->candidate_pairs = (double)hc[2];
         st->ms_step = ms;
+        st->ms_chainmesh = st->ms_pm = st->ms_pp = -1.0;
+        if (staged) {
+            float a = 0.f, b = 0.f, c = 0.f;
+            TPM_HIP(bt, hipEventElapsedTime(&a, bt->ev0, bt->ev_stage[0]));
+            TPM_HIP(bt, hipEventElapsedTime(&b, bt->ev_stage[0], bt->ev_stage[1]));
+            TPM_HIP(bt, hipEventElapsedTime(&c, bt->ev_stage[1], bt->ev_stage[2]));
+            st->ms_chainmesh = a;
+            st->ms_pm = b;
+            st->ms_pp = c;
+        }
         st->n_particles = L.n_live;
         int nl = 0;
         for (const auto& b : bt->boxes) nl += b.live ? 1 : 0;
@@ -303,6 +324,8 @@ tpm_status add_box(tpm_batch* bt, const tpm_box_spec* spec_in, int64_t n, const 
     // FFT plans of this size (and a work buffer large enough) before any
     // step is captured; the Green's table of the box.
     tpm_status s = tpm_fft_use(bt->err, bt->fft, bt->device, bt->stream, M);
+    for (int k = 0; k < tpm_batch::NSIDE && s == TPM_OK; k++)
+        s = tpm_fft_use(bt->err, bt->side_fft[k], bt->device, bt->side[k], M);
     if (s != TPM_OK) return undo(s);
     const double rs_eff = p.long_type == TPM_LONG_PM ? 0.0 : p.rs;
     s = tpm_green_table(bt->err, bt->stream, M, rs_eff, p.order, bt->green + k0);
@@ -409,10 +432,18 @@ extern "C" tpm_status tpm_batch_create(int device, const tpm_batch_limits* l, tp
         (l->interlace && alloc(bt, &bt->rho_hat1, c.k)) || alloc(bt, &bt->d_boxes, l->max_boxes) ||
         alloc(bt, &bt->d_kreg, l->max_boxes) || alloc(bt, &bt->counters, 16))
         return fail(TPM_ERR_BUDGET);
+    if (l->interlace && alloc(bt, &bt->rho1, c.m)) return fail(TPM_ERR_BUDGET);
+    for (int k = 0; k < tpm_batch::NSIDE; k++)
+        if (hipStreamCreateWithFlags(&bt->side[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&bt->ev_join[k], hipEventDisableTiming) != hipSuccess)
+            return fail(TPM_ERR_HIP);
+    if (hipEventCreateWithFlags(&bt->ev_fork, hipEventDisableTiming) != hipSuccess) return fail(TPM_ERR_HIP);
     if (hipHostMalloc(reinterpret_cast<void**>(&bt->h_boxes), sizeof(tpm_box_dev) * l->max_boxes, 0) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&bt->h_kreg), sizeof(tpm_kregion) * l->max_boxes, 0) != hipSuccess)
         return fail(TPM_ERR_HIP);
     if (hipEventCreate(&bt->ev0) != hipSuccess || hipEventCreate(&bt->ev1) != hipSuccess ||
+        hipEventCreate(&bt->ev_stage[0]) != hipSuccess || hipEventCreate(&bt->ev_stage[1]) != hipSuccess ||
+        hipEventCreate(&bt->ev_stage[2]) != hipSuccess ||
         hipEventCreateWithFlags(&bt->table_ev, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&bt->copy_ev, hipEventDisableTiming) != hipSuccess)
         return fail(TPM_ERR_HIP);
@@ -429,8 +460,17 @@ extern "C" tpm_status tpm_batch_destroy(tpm_batch* bt) {
     if (!bt) return TPM_OK;
     (void)hipSetDevice(bt->device);
     if (bt->stream) (void)hipStreamSynchronize(bt->stream);
+    for (int k = 0; k < tpm_batch::NSIDE; k++)
+        if (bt->side[k]) (void)hipStreamSynchronize(bt->side[k]);
     drop_graphs(bt);
     tpm_fft_free(bt->fft);
+    for (int k = 0; k < tpm_batch::NSIDE; k++) {
+        tpm_fft_free(bt->side_fft[k]);
+        if (bt->side[k]) (void)hipStreamDestroy(bt->side[k]);
+        if (bt->ev_join[k]) (void)hipEventDestroy(bt->ev_join[k]);
+    }
+    if (bt->ev_fork) (void)hipEventDestroy(bt->ev_fork);
+    if (bt->rho1) (void)hipFree(bt->rho1);
     void* bufs[] = {bt->x,     bt->y,        bt->z,        bt->x2,       bt->y2,     bt->z2,      bt->vx,
                     bt->vy,    bt->vz,       bt->vx2,      bt->vy2,      bt->vz2,    bt->rung,    bt->rung2,
                     bt->gx,    bt->gy,       bt->gz,       bt->pot,      bt->key,    bt->slot,    bt->tmp,
@@ -441,7 +481,8 @@ extern "C" tpm_status tpm_batch_destroy(tpm_batch* bt) {
         if (b) (void)hipFree(b);
     if (bt->h_boxes) (void)hipHostFree(bt->h_boxes);
     if (bt->h_kreg) (void)hipHostFree(bt->h_kreg);
-    hipEvent_t evs[] = {bt->ev0, bt->ev1, bt->table_ev, bt->copy_ev};
+    hipEvent_t evs[] = {bt->ev0, bt->ev1, bt->table_ev, bt->copy_ev, bt->ev_stage[0], bt->ev_stage[1],
+                        bt->ev_stage[2]};
     for (hipEvent_t e : evs)
         if (e) (void)hipEventDestroy(e);
     if (bt->stream) (void)hipStreamDestroy(bt->stream);
@@ -501,6 +542,8 @@ extern "C" tpm_status tpm_batch_prepare(tpm_batch* bt, const tpm_box_spec* spec,
     tpm_params p = spec->params;
     TPM_TRY(tpm_check_params(bt->err, &p));
     TPM_TRY(tpm_fft_use(bt->err, bt->fft, bt->device, bt->stream, p.mesh_dim));
+    for (int k = 0; k < tpm_batch::NSIDE; k++)
+        TPM_TRY(tpm_fft_use(bt->err, bt->side_fft[k], bt->device, bt->side[k], p.mesh_dim));
     if (n_side > 0) {
         tpm_ctx* sc = bt->scratch;
         TPM_HIP(bt, hipStreamWaitEvent(sc->stream, bt->copy_ev, 0));
@@ -601,8 +644,8 @@ extern "C" tpm_status tpm_batch_to_ctx(tpm_batch* bt, int id, tpm_ctx* ctx) {
 
 extern "C" tpm_status tpm_batch_set_tuning(tpm_batch* bt, const tpm_batch_tuning* t) {
     if (!bt || !t) return TPM_ERR_INVALID;
-    if (t->tile_sinks != 16 && t->tile_sinks != 32 && t->tile_sinks != 64)
-        return tpm_fail(bt, TPM_ERR_INVALID, "tile_sinks must be 16, 32 or 64");
+    if (t->tile_sinks != 8 && t->tile_sinks != 16 && t->tile_sinks != 32 && t->tile_sinks != 64)
+        return tpm_fail(bt, TPM_ERR_INVALID, "tile_sinks must be 8, 16, 32 or 64");
     if (t->deposit_block < 0 || t->deposit_block > 8)
         return tpm_fail(bt, TPM_ERR_INVALID, "deposit_block must be in [0, 8]");
     if (t->sched != 0 && t->sched != 1) return tpm_fail(bt, TPM_ERR_INVALID, "sched must be 0 or 1");
@@ -657,8 +700,8 @@ extern "C" tpm_status tpm_batch_autotune(tpm_batch* bt, int reps, tpm_batch_tuni
     };
     tpm_batch_tuning bestt = keep;
     double bms = 1e300;
-    const int tiles[3] = {16, 32, 64};
-    for (int ti = 0; ti < 3; ti++)
+    const int tiles[4] = {8, 16, 32, 64};
+    for (int ti = 0; ti < 4; ti++)
         for (int sc = 0; sc <= 1; sc++) {
             tpm_batch_tuning t = keep;
             t.tile_sinks = tiles[ti];

@@ -215,6 +215,26 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
 
+// cnt[c] += 1 for every active lane, one atomic per distinct c in the wave;
+// returns the lane's slot (count before it).  Which lane of a cell gets which
+// slot depends on timing; the in-cell ranking later fixes the order.
+__device__ __forceinline__ uint32_t wave_count_slot(int32_t* cnt, uint32_t c) {
+    const int lane = (int)(threadIdx.x & 63);
+    unsigned long long todo = __ballot(1);
+    uint32_t slot = 0;
+    while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const uint32_t lc = (uint32_t)__shfl((int)c, leader);
+        const unsigned long long same = __ballot(c == lc) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[lc], __popcll(same));
+        base = __shfl(base, leader);
+        if (c == lc) slot = (uint32_t)base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return slot;
+}
+
 __global__ __launch_bounds__(256) void bkey_kernel(int64_t n, int nbox, const tpm_box_dev* __restrict__ boxes,
                                                    int32_t trash, const uint32_t* __restrict__ x,
                                                    const uint32_t* __restrict__ y, const uint32_t* __restrict__ z,
@@ -234,7 +254,7 @@ __global__ __launch_bounds__(256) void bkey_kernel(int64_t n, int nbox, const tp
             m = ((sz >> 1) << 5) | ((sy >> 1) << 4) | ((sx >> 1) << 3) | ((sz & 1) << 2) | ((sy & 1) << 1) | (sx & 1);
         }
         key[i] = gc * 64u + m;
-        slot[i] = (uint32_t)atomicAdd(&cnt[gc], 1);
+        slot[i] = wave_count_slot(cnt, gc);
     }
 }
 

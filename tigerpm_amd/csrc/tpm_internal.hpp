@@ -59,6 +59,7 @@ struct tpm_box_dev {
     // pair constants in 2^-32 box units
     float rcut2s, cull2s, inv2rs_s, tsp, h2s, h2inv_s, hinv_s, h3inv_s;
     float e2s;  // -log2(e) / (4 rs^2) in 2^-64 units: exp(-x^2) = exp2(r2 * e2s)
+    float p_inv2rs_s, tsp_inv2rs_s;  // 0.3275911 / (2 rs) and 2 / sqrt(pi) / (2 rs), 2^-32 units
     // kick epilogue
     float GM, phi0, eta, t0, scale, hsoft;
     double drift;  // t0 of the drift step (batched path)
@@ -393,6 +394,13 @@ struct tpm_batch {
     int64_t table_cap = 0;
 
     tpm_fft fft;
+    // per-box FFTs fan out over side streams (fork/join with events; a
+    // captured step records them as parallel branches)
+    static constexpr int NSIDE = 4;
+    hipStream_t side[NSIDE] = {};
+    tpm_fft side_fft[NSIDE];
+    hipEvent_t ev_fork = nullptr, ev_join[NSIDE] = {};
+    float* rho1 = nullptr;  // float image of the shifted deposit (interlace)
     tpm_ctx* scratch = nullptr;  // initial conditions are generated here
     unsigned long long* counters = nullptr;  // [0] executed pairs [1] max rung [2] candidates [8..15] PP queues
     tpm_batch_tuning tune{16, 0, 0, 0};
@@ -406,6 +414,8 @@ struct tpm_batch {
     uint64_t layout_version = 1;  // changes with the set of boxes and the tuning
     int parity = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_stage[3] = {nullptr, nullptr, nullptr};  // after chain mesh, PM, PP
+    bool stage_events = false;                             // record them (eager steps with stats)
     hipEvent_t copy_ev = nullptr;  // the last copy into the arena from the scratch context
 };
 

@@ -304,8 +304,8 @@ tpm_status tpm_batch_fill_u64(tpm_batch* bt, unsigned long long* p, int64_t n) {
 }
 
 // Deposit (grid shift) of every live box into its region of rho_q, then the
-// float image of the used part of the arena.
-static tpm_status batch_deposit(tpm_batch* bt, const tpm_batch_layout& L, uint64_t shift) {
+// float image of the used part of the arena into out.
+static tpm_status batch_deposit(tpm_batch* bt, const tpm_batch_layout& L, uint64_t shift, float* out) {
     hipStream_t s = bt->stream;
     TPM_TRY(tpm_batch_fill_u64(bt, bt->rho_q, L.mesh_hw));
     if (L.ndep > 0) {
@@ -326,35 +326,72 @@ static tpm_status batch_deposit(tpm_batch* bt, const tpm_batch_layout& L, uint64
         }
         TPM_HIP(bt, hipGetLastError());
     }
-    q32_to_float_kernel<<<grid_for(L.mesh_hw, 256), 256, 0, s>>>(L.mesh_hw, bt->rho_q, bt->rho);
+    q32_to_float_kernel<<<grid_for(L.mesh_hw, 256), 256, 0, s>>>(L.mesh_hw, bt->rho_q, out);
     TPM_HIP(bt, hipGetLastError());
+    return TPM_OK;
+}
+
+// Live boxes, largest mesh first (side stream k takes boxes k, k + NSIDE, ...).
+static std::vector<const tpm_batch_box*> boxes_by_size(const tpm_batch* bt) {
+    std::vector<const tpm_batch_box*> v;
+    for (const tpm_batch_box& b : bt->boxes)
+        if (b.live) v.push_back(&b);
+    std::stable_sort(v.begin(), v.end(), [](const tpm_batch_box* a, const tpm_batch_box* b) { return a->M > b->M; });
+    return v;
+}
+
+// Forks the side streams off the main stream (they wait for its work so far).
+static tpm_status fork_sides(tpm_batch* bt) {
+    TPM_HIP(bt, hipEventRecord(bt->ev_fork, bt->stream));
+    for (int k = 0; k < tpm_batch::NSIDE; k++) TPM_HIP(bt, hipStreamWaitEvent(bt->side[k], bt->ev_fork, 0));
+    return TPM_OK;
+}
+
+// The main stream waits for everything the side streams were given.
+static tpm_status join_sides(tpm_batch* bt) {
+    for (int k = 0; k < tpm_batch::NSIDE; k++) {
+        TPM_HIP(bt, hipEventRecord(bt->ev_join[k], bt->side[k]));
+        TPM_HIP(bt, hipStreamWaitEvent(bt->stream, bt->ev_join[k], 0));
+    }
+    return TPM_OK;
+}
+
+// One transform per live box on the side streams: R2C from real to
+// spectrum (forward) or C2R from spectrum to real.
+static tpm_status side_ffts(tpm_batch* bt, const std::vector<const tpm_batch_box*>& v, bool forward, float* real,
+                            float2* spec) {
+    for (size_t i = 0; i < v.size(); i++) {
+        const tpm_batch_box& b = *v[i];
+        const int k = (int)(i % tpm_batch::NSIDE);
+        tpm_fft& f = bt->side_fft[k];
+        TPM_TRY(tpm_fft_use(bt->err, f, bt->device, bt->side[k], b.M));
+        if (forward)
+            TPM_TRY(tpm_fft_r2c(bt->err, f, real + b.mesh0, spec + b.k0));
+        else
+            TPM_TRY(tpm_fft_c2r(bt->err, f, spec + b.k0, real + b.mesh0));
+    }
     return TPM_OK;
 }
 
 tpm_status tpm_batch_pm(tpm_batch* bt, const tpm_batch_layout& L) {
     if (L.mesh_hw == 0) return TPM_OK;
-    TPM_TRY(batch_deposit(bt, L, 0));
-    for (const tpm_batch_box& b : bt->boxes) {
-        if (!b.live) continue;
-        TPM_TRY(tpm_fft_use(bt->err, bt->fft, bt->device, bt->stream, b.M));
-        TPM_TRY(tpm_fft_r2c(bt->err, bt->fft, bt->rho + b.mesh0, bt->rho_hat + b.k0));
-    }
+    const auto v = boxes_by_size(bt);
+    TPM_TRY(batch_deposit(bt, L, 0, bt->rho));
+    TPM_TRY(fork_sides(bt));
+    TPM_TRY(side_ffts(bt, v, true, bt->rho, bt->rho_hat));
     if (L.interlace) {
-        TPM_TRY(batch_deposit(bt, L, 1ull << 31));
-        for (const tpm_batch_box& b : bt->boxes) {
-            if (!b.live) continue;
-            TPM_TRY(tpm_fft_use(bt->err, bt->fft, bt->device, bt->stream, b.M));
-            TPM_TRY(tpm_fft_r2c(bt->err, bt->fft, bt->rho + b.mesh0, bt->rho_hat1 + b.k0));
-        }
+        // the shifted deposit runs on the main stream meanwhile
+        TPM_TRY(batch_deposit(bt, L, 1ull << 31, bt->rho1));
+        TPM_TRY(fork_sides(bt));
+        TPM_TRY(side_ffts(bt, v, true, bt->rho1, bt->rho_hat1));
     }
+    TPM_TRY(join_sides(bt));
     bgreen_apply_kernel<<<grid_for(L.k_hw, 256), 256, 0, bt->stream>>>(
         L.k_hw, L.nkreg, bt->d_kreg, bt->rho_hat, L.interlace ? bt->rho_hat1 : nullptr, bt->green);
     TPM_HIP(bt, hipGetLastError());
-    for (const tpm_batch_box& b : bt->boxes) {
-        if (!b.live) continue;
-        TPM_TRY(tpm_fft_use(bt->err, bt->fft, bt->device, bt->stream, b.M));
-        TPM_TRY(tpm_fft_c2r(bt->err, bt->fft, bt->rho_hat + b.k0, bt->phi + b.mesh0));
-    }
+    TPM_TRY(fork_sides(bt));
+    TPM_TRY(side_ffts(bt, v, false, bt->phi, bt->rho_hat));
+    TPM_TRY(join_sides(bt));
     return TPM_OK;
 }
 

@@ -19,11 +19,9 @@
 // Pair distances are int32(sink - source) converted to float, which is the
 // reference's distance() (gravity_short.hpp:20-22) in units of 2^-32; every
 // constant is pre-scaled by the matching power of two so the FP32 arithmetic
-// is the reference's up to exact power-of-two factors.  The self pair, which
-// the reference evaluates inside the softening branch (it contributes
-// -35/16 h^-1 to phi and nothing to g), is masked out of the loop and added
-// once per sink, so the softening branch is taken only by distinct pairs
-// closer than h.
+// is the reference's up to exact power-of-two factors.  The self pair is
+// evaluated like any other source, as in the reference: inside the
+// softening branch it contributes -35/16 h^-1 to phi and nothing to g.
 #include "tpm_internal.hpp"
 
 #include <climits>
@@ -80,48 +78,52 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // compute_pp_interaction (kick_treepm.cu:242-283) with erfcexp
-// (util.hpp:36-59, Abramowitz & Stegun 7.1.26), in 2^-32 units.  self is
-// the sink's own particle index: that pair is excluded here.
-__device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, int oz, int self, int4 s, float& gx,
-                                        float& gy, float& gz, float& ph) {
+// (util.hpp:36-59, Abramowitz & Stegun 7.1.26), in 2^-32 units.  The sink's
+// own particle is one of its sources, as in the reference: at r = 0 the
+// softening branch gives no force and exactly -35/16 h^-1 to phi.
+__device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, int oz, int4 s, float& gx, float& gy,
+                                        float& gz, float& ph) {
     const float dx = (float)(int)((unsigned)ox - (unsigned)s.x);
     const float dy = (float)(int)((unsigned)oy - (unsigned)s.y);
     const float dz = (float)(int)((unsigned)oz - (unsigned)s.z);
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    // Distinct particles differ by >= 1 raw unit, so r2 >= 1 unless this is
-    // the self pair; the clamp keeps rinv^3 finite for it.
-    const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0f));
+    // Distinct particles differ by >= 1 raw unit, so r2 = 0 only for the
+    // sink itself: there rinv = inf and every value below that depends on it
+    // is replaced in the softening branch.
+    const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
-    const float r0 = r * a.inv2rs_s;
-    const float t1 = __builtin_amdgcn_rcpf(fmaf(0.3275911f, r0, 1.f));
-    // The cutoff test (kick_treepm.cu:254) and the self exclusion zero the
-    // Gaussian factor, which zeroes both erfc and the force factor below.
-    const bool other = s.w != self;
-    const bool in = r2 < a.rcut2s && other;
-    const float e = in ? __builtin_amdgcn_exp2f(r2 * a.e2s) : 0.f;
+    // x = r / (2 rs); t = 1 / (1 + p x) with p / (2 rs) folded into one constant
+    const float t1 = __builtin_amdgcn_rcpf(fmaf(a.p_inv2rs_s, r, 1.f));
+    // exp(-x^2); the cutoff test (kick_treepm.cu:254) zeroes it, which
+    // zeroes both erfc and the force factor below.
+    const float e = r2 < a.rcut2s ? __builtin_amdgcn_exp2f(r2 * a.e2s) : 0.f;
     const float poly =
         t1 * fmaf(t1, fmaf(t1, fmaf(t1, fmaf(t1, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
                   0.254829592f);
-    const float erfc0 = poly * e;
-    const float tre = a.tsp * r0 * e;
-    float rinv3 = fmaf(a.tsp, r0, poly) * e * (rinv * rinv) * rinv;
-    float rinv1 = rinv * erfc0;
-    if (r2 <= a.h2s && other) {
-        // Distinct particles closer than h: Plummer polynomial plus the
-        // (erfc - 1) correction (kick_treepm.cu:263-276).
-        const float ri = r2 > 0.f ? rinv : 0.f;
+    const float er = e * rinv;
+    // erfc(x) / r and (erfc(x) + 2 x / sqrt(pi) exp(-x^2)) / r^3
+    float rinv1 = poly * er;
+    float rinv3 = fmaf(a.tsp_inv2rs_s, r, poly) * er * (rinv * rinv);
+    if (r2 <= a.h2s) {
+        // Closer than h: Plummer polynomial plus the (erfc - 1) correction
+        // (kick_treepm.cu:263-276).  At r = 0 (the sink itself) only d1
+        // (times dx = 0) and d0 remain.
+        const bool pos = r2 > 0.f;
+        const float ri = pos ? rinv : 0.f;
+        const float em1 = pos ? fmaf(poly, e, -1.0f) : 0.f;
+        const float tre = pos ? a.tsp_inv2rs_s * r * e : 0.f;
         const float q2 = r2 * a.h2inv_s;
         float d1 = 15.0f / 8.0f;
         d1 = fmaf(d1, q2, -21.0f / 4.0f);
         d1 = fmaf(d1, q2, 35.0f / 8.0f);
         d1 *= a.h3inv_s;
-        rinv3 = fmaf(((erfc0 - 1.0f) + tre) * ri * ri, ri, d1);
+        rinv3 = fmaf((em1 + tre) * ri * ri, ri, d1);
         float d0 = -5.0f / 16.0f;
         d0 = fmaf(d0, q2, 21.0f / 16.0f);
         d0 = fmaf(d0, q2, -35.0f / 16.0f);
         d0 = fmaf(d0, q2, 35.0f / 16.0f);
         d0 *= a.hinv_s;
-        rinv1 = fmaf(erfc0 - 1.0f, ri, d0);
+        rinv1 = fmaf(em1, ri, d0);
     }
     gx = fmaf(-dx, rinv3, gx);
     gy = fmaf(-dy, rinv3, gy);
@@ -235,10 +237,13 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
                     if (lane < padded - nbuf) sbuf[nbuf + lane] = far;
                     __syncthreads();
                     int4 s = sbuf[stream];
+                    // the prefetch past the last source reads a slot below
+                    // padded + SPLIT <= 195 + 4 < SBUF (a flush starts at
+                    // nbuf <= SBUF / 2 + 64), never used
 #pragma unroll 2
                     for (int qq = 0; qq < padded; qq += SPLIT) {
-                        const int4 sn = sbuf[min(qq + SPLIT, padded - SPLIT) + stream];
-                        pp_pair(B, ox, oy, oz, i, s, gx, gy, gz, ph);
+                        const int4 sn = sbuf[qq + SPLIT + stream];
+                        pp_pair(B, ox, oy, oz, s, gx, gy, gz, ph);
                         s = sn;
                     }
                     staged += (unsigned long long)nbuf;
@@ -287,8 +292,6 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
                 if (lane == 0 && a.counters)
                     atomicAdd(&a.counters[0],
                               staged * (unsigned long long)__popcll(amask & ((TS == 64) ? ~0ull : ((1ull << TS) - 1))));
-                // the self pair (kick_treepm.cu:263-276 at r = 0): -35/16 h^-1
-                ph -= (35.0f / 16.0f) * B.hinv_s;
                 // back to box units: g in 2^-64, phi in 2^-32
                 gx *= 0x1p64f;
                 gy *= 0x1p64f;
@@ -434,12 +437,14 @@ template <bool BATCH>
 static void pp_launch(int ts, bool persistent, unsigned grid, hipStream_t s, const PPArgs& a) {
     if (persistent) {
         switch (ts) {
+            case 8: pp_kick_kernel<8, BATCH, true><<<grid, 64, 0, s>>>(a); break;
             case 16: pp_kick_kernel<16, BATCH, true><<<grid, 64, 0, s>>>(a); break;
             case 64: pp_kick_kernel<64, BATCH, true><<<grid, 64, 0, s>>>(a); break;
             default: pp_kick_kernel<32, BATCH, true><<<grid, 64, 0, s>>>(a); break;
         }
     } else {
         switch (ts) {
+            case 8: pp_kick_kernel<8, BATCH, false><<<grid, 64, 0, s>>>(a); break;
             case 16: pp_kick_kernel<16, BATCH, false><<<grid, 64, 0, s>>>(a); break;
             case 64: pp_kick_kernel<64, BATCH, false><<<grid, 64, 0, s>>>(a); break;
             default: pp_kick_kernel<32, BATCH, false><<<grid, 64, 0, s>>>(a); break;
@@ -470,6 +475,9 @@ void tpm_box_fill(tpm_box_dev* b, const tpm_params& p, int min_rung, int first_c
     b->cull = p.chain_dim >= 5 ? 1 : 0;
     b->inv2rs_s = (1.0f / rs / 2.0f) / S;
     b->tsp = 2.0f / sqrtf((float)M_PI);
+    // A&S 7.1.26 p and 2/sqrt(pi), each times 1 / (2 rs) in 2^-32 units
+    b->p_inv2rs_s = (float)(0.3275911 / (2.0 * p.rs) / 4294967296.0);
+    b->tsp_inv2rs_s = (float)(2.0 / std::sqrt(M_PI) / (2.0 * p.rs) / 4294967296.0);
     b->h2s = (h * h) * (S * S);
     const float hinv = 1.f / h;
     b->hinv_s = hinv / S;
@@ -616,6 +624,7 @@ int tpm_pp_persistent_grid(int device, int ts) {
     int per_cu = 0;
     hipError_t e;
     switch (ts) {
+        case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<8, true, true>, 64, 0); break;
         case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<16, true, true>, 64, 0); break;
         case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<64, true, true>, 64, 0); break;
         default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<32, true, true>, 64, 0); break;
