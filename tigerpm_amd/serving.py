@@ -45,13 +45,13 @@ def poisson_requests(count, rate, seed, n_lo=32, n_hi=256, rms_lo=0.3, rms_hi=2.
     return [Request(i, float(arrivals[i]), int(ns[i]), int(seed) * 100003 + i, float(rms[i])) for i in range(count)]
 
 
-def request_spec(req, order=3, interlace=1, t0=0.01):
+def request_spec(req, order=4, interlace=0, t0=0.01):
     """Box parameters of a request: mesh = n_side, rs = 1.25 cells,
     rcut = 4.5 rs, masses 1/N."""
     return box_spec(baseline_params(req.n_side, req.n_side, order, interlace=interlace), t0=t0)
 
 
-def budget_limits(budget_bytes, n_hi=256, order=3, interlace=1, max_boxes=256):
+def budget_limits(budget_bytes, n_hi=256, order=4, interlace=0, max_boxes=256):
     """Arena capacities for a device-memory budget: particle and mesh slots
     in the ratio of boxes with mesh = n_side, the largest box always fits."""
     from ._lib import batch_arena_bytes
@@ -62,11 +62,13 @@ def budget_limits(budget_bytes, n_hi=256, order=3, interlace=1, max_boxes=256):
 
 
 class Server:
-    def __init__(self, batch, kicks=10, order=3, interlace=1, t0=0.01):
+    """Serves requests with the batch's assignment order and interlacing."""
+
+    def __init__(self, batch, kicks=10, t0=0.01):
         self.batch = batch
         self.kicks = kicks
-        self.order = order
-        self.interlace = interlace
+        self.order = batch.limits.order
+        self.interlace = batch.limits.interlace
         self.t0 = t0
 
     def spec(self, req):
