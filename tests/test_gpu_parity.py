@@ -225,3 +225,54 @@ def test_kick_vs_ewald(gpu_ctx, order, tol):
     G = np.stack([d["gx"][s], d["gy"][s], d["gz"][s]], 1)
     E = np.stack([gx, gy, gz], 1) * p.GM
     assert rel_rms(G, E) < tol
+
+
+def test_edge_positions_vs_oracle(gpu_ctx):
+    """Coincident particles (zero force and the self-like d0 potential,
+    as the reference's r = 0 handling gives), particles at raw 0 and
+    2^32 - 1, particles on and beside every chain-cell split plane, and a
+    half-empty box: full kick and chain mesh against the oracle."""
+    n_side, C = 24, 4
+    rng = np.random.default_rng(123)
+    p = T.baseline_params(n_side, n_side, 3)
+    p.chain_dim = C
+    p.rcut = min(p.rcut, 0.99 / C)
+    planes = [int(4294967296.0 * (k * (1.0 / C))) for k in range(C)]
+    pts = []
+    for b in planes:
+        for d in (-1, 0, 1):
+            pts.append((b + d) % 2 ** 32)
+    pts += [0, 1, 2 ** 32 - 1, 2 ** 32 - 2]
+    edge = np.array(pts, np.uint64)
+    m = len(edge)
+    n_rest = n_side ** 3 - 3 * m - 64
+    # the rest in one half of the box along x
+    xs = rng.integers(0, 2 ** 31, n_rest, dtype=np.uint64)
+    ys = rng.integers(0, 2 ** 32, n_rest, dtype=np.uint64)
+    zs = rng.integers(0, 2 ** 32, n_rest, dtype=np.uint64)
+    ex = np.concatenate([edge, rng.integers(0, 2 ** 32, m, dtype=np.uint64), rng.integers(0, 2 ** 32, m, dtype=np.uint64)])
+    ey = np.concatenate([rng.integers(0, 2 ** 32, m, dtype=np.uint64), edge, rng.integers(0, 2 ** 32, m, dtype=np.uint64)])
+    ez = np.concatenate([rng.integers(0, 2 ** 32, m, dtype=np.uint64), rng.integers(0, 2 ** 32, m, dtype=np.uint64), edge])
+    x = np.concatenate([xs, ex]).astype(np.uint32)
+    y = np.concatenate([ys, ey]).astype(np.uint32)
+    z = np.concatenate([zs, ez]).astype(np.uint32)
+    # 32 coincident pairs
+    k = rng.choice(len(x), 32, replace=False)
+    x, y, z = np.concatenate([x, x[k]]), np.concatenate([y, y[k]]), np.concatenate([z, z[k]])
+    n = len(x)
+    p.n_total = float(n)
+    p.GM = 1.0 / n
+    p = gpu_ctx.set_params(p)
+    gpu_ctx.upload(x, y, z)
+    gpu_ctx.kick_treepm()
+    d = gpu_ctx.download()
+    operm, ocb = O.chainmesh_sort(x, y, z, C)
+    assert np.array_equal(gpu_ctx.chainmesh_perm(), operm)
+    assert np.array_equal(gpu_ctx.chainmesh_cells(), ocb)
+    zf = np.zeros(n, np.float32)
+    r = O.kick_full(oparams(p), x, y, z, zf, zf, zf, np.zeros(n, np.int8))
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1)
+    OG = np.stack([r["gx"], r["gy"], r["gz"]], 1)
+    assert np.all(np.isfinite(G)) and np.all(np.isfinite(d["pot"]))
+    assert rel_rms(G, OG) < 1e-5
+    assert rel_rms(d["pot"], r["pot"]) < 1e-5
