@@ -51,6 +51,10 @@ typedef int tpm_status;
 #define TPM_SKIP_PP 1u   /* no short-range particle-particle term */
 #define TPM_SKIP_PM 2u   /* no mesh interpolation term */
 #define TPM_SKIP_KICK 4u /* store g / phi only: velocities and rungs untouched */
+/* tpm_kick flag: kick_pme_kernel's pair law (kick_pme.cu:280-310) instead of
+ * kick_treepm's -- every pair of the 27 cells with no cutoff test, plain
+ * softened polynomials inside hsoft, phi starting at 0 (no phi0). */
+#define TPM_KICK_PME 16u
 
 /* Physical and numerical parameters.  Replaces the options struct
  * (options.hpp:5-24, derived in options.cpp:81-96) and the scalars that

@@ -34,7 +34,7 @@ class OrcParams(ctypes.Structure):
         ("order", c_int),
         ("do_phi", c_int),
         ("interlace", c_int),
-        ("pad_", c_int),
+        ("pme", c_int),
         ("rs", c_dbl),
         ("rcut", c_dbl),
         ("hsoft", c_dbl),
@@ -102,8 +102,9 @@ def lib():
     return _lib
 
 
-def make_params(mesh_dim, chain_dim, order, rs, rcut, hsoft, GM, eta, n_total, do_phi=1, interlace=0):
-    return OrcParams(int(mesh_dim), int(chain_dim), int(order), int(do_phi), int(interlace), 0, float(rs),
+def make_params(mesh_dim, chain_dim, order, rs, rcut, hsoft, GM, eta, n_total, do_phi=1, interlace=0, pme=0):
+    """pme = 1: kick_pme's pair law (no cutoff, plain softening, no phi0)."""
+    return OrcParams(int(mesh_dim), int(chain_dim), int(order), int(do_phi), int(interlace), int(pme), float(rs),
                      float(rcut), float(hsoft), float(GM), float(eta), float(n_total))
 
 

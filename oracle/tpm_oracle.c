@@ -26,7 +26,7 @@ typedef struct {
     int order;      /* mass assignment: 2 CIC, 3 TSC, 4 cloud4 (cubic B-spline)  */
     int do_phi;     /* accumulate the potential (reference do_phi)               */
     int interlace;  /* second deposit on the half-cell-shifted grid (this build) */
-    int pad_;
+    int pme;        /* 1: kick_pme's pair law (no cutoff, plain softening, no phi0) */
     double rs;      /* split scale, box units                                    */
     double rcut;    /* short-range cutoff, box units                             */
     double hsoft;   /* Plummer-polynomial softening length, box units            */
@@ -669,7 +669,7 @@ static inline float erfcexp(float x, float* e) {
 
 typedef struct {
     float rcut2, inv2rs, twooversqrtpi, h2, hinv, h2inv, h3inv;
-    int do_phi;
+    int do_phi, pme;
 } pp_consts;
 
 static pp_consts make_pp_consts(const orc_params* p) {
@@ -685,6 +685,7 @@ static pp_consts make_pp_consts(const orc_params* p) {
     c.h2inv = c.hinv * c.hinv;
     c.h3inv = c.hinv * c.h2inv;
     c.do_phi = p->do_phi;
+    c.pme = p->pme;
     return c;
 }
 
@@ -722,6 +723,39 @@ static inline void pp_fp32(const pp_consts* c, float dx, float dy, float dz, flo
         g[2] -= dz * rinv3;
         *phi -= rinv;
     }
+}
+
+/* kick_pme_kernel's pair law (kick_pme.cu:280-310), FP32, as written: every
+ * pair of the 27 cells (no cutoff test); outside h the erfc-split force,
+ * inside h the plain Plummer polynomials without the (erfc - 1) terms. */
+static inline void pp_fp32_pme(const pp_consts* c, float dx, float dy, float dz, float* g, float* phi) {
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    float rinv, rinv3;
+    if (r2 > c->h2) {
+        const float r = sqrtf(r2);
+        rinv = 1.0f / sqrtf(r2);
+        const float r0 = r * c->inv2rs;
+        float exp0;
+        const float erfc0 = erfcexp(r0, &exp0);
+        rinv3 = (erfc0 + c->twooversqrtpi * r0 * exp0) * rinv * rinv * rinv;
+        rinv *= erfc0;
+    } else {
+        const float q = sqrtf(r2) * c->hinv;
+        const float q2 = q * q;
+        rinv3 = +15.0f / 8.0f;
+        rinv3 = fmaf(rinv3, q2, -21.0f / 4.0f);
+        rinv3 = fmaf(rinv3, q2, +35.0f / 8.0f);
+        rinv3 *= c->h3inv;
+        rinv = -5.0f / 16.0f;
+        rinv = fmaf(rinv, q2, 21.0f / 16.0f);
+        rinv = fmaf(rinv, q2, -35.0f / 16.0f);
+        rinv = fmaf(rinv, q2, 35.0f / 16.0f);
+        rinv *= c->hinv;
+    }
+    g[0] -= dx * rinv3;
+    g[1] -= dy * rinv3;
+    g[2] -= dz * rinv3;
+    *phi -= rinv;
 }
 
 /* The same interaction in FP64 with libm erfc (the truth the FP32 formula
@@ -777,7 +811,10 @@ ORC_API void orc_pp_pair(const orc_params* p, double dx, double dy, double dz, i
     } else {
         pp_consts c = make_pp_consts(p);
         float g[3] = {0, 0, 0}, ph = 0;
-        pp_fp32(&c, (float)dx, (float)dy, (float)dz, g, &ph);
+        if (c.pme)
+            pp_fp32_pme(&c, (float)dx, (float)dy, (float)dz, g, &ph);
+        else
+            pp_fp32(&c, (float)dx, (float)dy, (float)dz, g, &ph);
         out[0] = g[0];
         out[1] = g[1];
         out[2] = g[2];
@@ -820,8 +857,12 @@ ORC_API double orc_pp_sum(const orc_params* p, int64_t n, const uint32_t* x, con
                             const double dz = (double)(int32_t)(z[i] - z[j]) * 0x1p-32;
                             pp_fp64(p, dx, dy, dz, G, &PH);
                         } else {
-                            pp_fp32(&c, orc_distance(x[i], x[j]), orc_distance(y[i], y[j]),
-                                    orc_distance(z[i], z[j]), g, &ph);
+                            if (c.pme)
+                                pp_fp32_pme(&c, orc_distance(x[i], x[j]), orc_distance(y[i], y[j]),
+                                            orc_distance(z[i], z[j]), g, &ph);
+                            else
+                                pp_fp32(&c, orc_distance(x[i], x[j]), orc_distance(y[i], y[j]),
+                                        orc_distance(z[i], z[j]), g, &ph);
                         }
                     }
                 }
@@ -889,7 +930,8 @@ ORC_API void orc_kick_epilogue(const orc_params* p, int64_t n, float* gx, float*
                                float* vx, float* vy, float* vz, int8_t* rung, int min_rung, double scale, double t0,
                                int first_call) {
     const float GM = (float)p->GM;
-    const float phi0 = (float)(p->n_total * 4.0 * M_PI * p->rs * p->rs + (35.0 / 16.0) / p->hsoft);
+    /* kick_treepm's phi0 (kick_treepm.cu:839); kick_pme starts phi at 0 */
+    const float phi0 = p->pme ? 0.f : (float)(p->n_total * 4.0 * M_PI * p->rs * p->rs + (35.0 / 16.0) / p->hsoft);
     const float eta = (float)p->eta;
     const float hsoft = (float)p->hsoft;
     const float t0f = (float)t0;

@@ -35,3 +35,10 @@ def test_treepm_scenario_reference_defaults():
 def test_kick_pm_scenario():
     out, e = run("--test=kick_pm", "--parts_dim=60")
     assert "Force RMS Error" in out
+
+
+def test_kick_pme_scenario():
+    # kick_pme_test (test.cpp:182-235): random particles, kick_pme_begin
+    out, e = run("--test=kick_pme", "--parts_dim=60")
+    assert "KICK" in out and "COMPARE" in out
+    assert 0.0 < e["Force RMS Error"] < 0.5

@@ -276,3 +276,41 @@ def test_edge_positions_vs_oracle(gpu_ctx):
     assert np.all(np.isfinite(G)) and np.all(np.isfinite(d["pot"]))
     assert rel_rms(G, OG) < 1e-5
     assert rel_rms(d["pot"], r["pot"]) < 1e-5
+
+
+@pytest.mark.parametrize("flags_extra", [T.TPM_SKIP_PM | T.TPM_SKIP_KICK, 0])
+def test_kick_pme_law_vs_oracle(gpu_ctx, flags_extra):
+    """TPM_KICK_PME: kick_pme's pair law (every pair of the 27 cells, plain
+    softening inside h, phi without phi0) against the oracle's restatement,
+    PP alone and the whole kick with mixed rungs."""
+    n_side = 28
+    n = n_side ** 3
+    x, y, z = clustered(n, 101)
+    rng = np.random.default_rng(7)
+    vx, vy, vz = [rng.standard_normal(n).astype(np.float32) * 1e-3 for _ in range(3)]
+    rung = rng.integers(0, 3, n).astype(np.int8)
+    p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, 4))
+    gpu_ctx.upload(x, y, z, vx, vy, vz, rung)
+    gpu_ctx.chainmesh_create()
+    gpu_ctx.gravity_long_compute()
+    st = gpu_ctx.kick(min_rung=1, scale=1.0, t0=0.5, first_call=False, flags=T.TPM_KICK_PME | flags_extra)
+    assert st.executed_pairs == st.candidate_pairs  # no culling without a cutoff
+    d = gpu_ctx.download()
+    op = oparams(p)
+    op.pme = 1
+    operm, ocb = O.chainmesh_sort(x, y, z, p.chain_dim)
+    act = rung[operm] >= 1
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1)[act]
+    if flags_extra:
+        xs, ys, zs = x[operm], y[operm], z[operm]
+        (gx, gy, gz, pot), _ = O.pp_sum(op, xs, ys, zs, ocb, np.arange(n), fp64=False)
+        OG = np.stack([gx, gy, gz], 1)[act] * p.GM
+        assert rel_rms(G, OG) < 1e-5
+        assert rel_rms(d["pot"][act], pot[act] * p.GM) < 1e-5
+    else:
+        r = O.kick_full(op, x, y, z, vx, vy, vz, rung, min_rung=1, scale=1.0, t0=0.5, first_call=False)
+        OG = np.stack([r["gx"], r["gy"], r["gz"]], 1)[act]
+        assert rel_rms(G, OG) < 1e-5
+        assert rel_rms(d["pot"][act], r["pot"][act]) < 1e-5
+        V = np.stack([d["vx"], d["vy"], d["vz"]], 1)
+        assert rel_rms(V[act], np.stack([r["vx"], r["vy"], r["vz"]], 1)[act]) < 1e-5

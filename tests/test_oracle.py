@@ -450,3 +450,31 @@ def test_drift_closed_forms():
     half = O.drift(*O.drift(x, y, z, v2, v2, v2, t0 / 2), v2, v2, v2, t0 / 2)
     for a, b in zip(one, half):
         assert np.array_equal(a, b)
+
+
+def test_pme_pair_law_closed_forms():
+    """kick_pme's pair law (kick_pme.cu:280-310): inside h the plain softened
+    polynomials (no erfc terms), at r = 0 exactly -35/16 h^-1 and no force,
+    beyond h the erfc-split force with no cutoff (checked against scipy's
+    erfc past rcut, where the kick_treepm law gives zero)."""
+    from scipy.special import erfc
+    rs, rcut, h = 0.01, 0.045, 0.002
+    p = O.make_params(32, 6, 3, rs, rcut, h, 1.0, 0.14, 1000.0, pme=1)
+    q = O.make_params(32, 6, 3, rs, rcut, h, 1.0, 0.14, 1000.0, pme=0)
+    g = O.pp_pair(p, 0.0, 0.0, 0.0, False)
+    assert g[0] == 0.0 and g[1] == 0.0 and g[2] == 0.0
+    assert abs(g[3] + 35.0 / 16.0 / h) <= 1e-6 * 35.0 / 16.0 / h
+    for r in (0.3 * h, 0.7 * h, 0.99 * h):
+        u = r / h
+        f = (15.0 / 8.0 * u ** 4 - 21.0 / 4.0 * u ** 2 + 35.0 / 8.0) / h ** 3 * r
+        ph = -(-5.0 / 16.0 * u ** 6 + 21.0 / 16.0 * u ** 4 - 35.0 / 16.0 * u ** 2 + 35.0 / 16.0) / h
+        out = O.pp_pair(p, r, 0.0, 0.0, False)
+        assert abs(-out[0] - f) < 2e-6 * f and abs(out[3] - ph) < 2e-6 * abs(ph)
+    for r in (0.5 * rcut, 1.2 * rcut, 1.8 * rcut):
+        x = r / (2 * rs)
+        f = (erfc(x) + 2 * x / np.sqrt(np.pi) * np.exp(-x * x)) / r ** 2
+        out = O.pp_pair(p, 0.0, r, 0.0, False)
+        assert abs(-out[1] - f) < 5e-6 / r ** 2 + 1e-5 * f
+        assert abs(out[3] + erfc(x) / r) < 5e-6 / r
+        if r > rcut:
+            assert O.pp_pair(q, 0.0, r, 0.0, False)[1] == 0.0

@@ -28,6 +28,8 @@ MUTANTS = {
     "ewald_self_constant_dropped": ("double P = (double)n * M_PI / (a * a) + tsp * a;",
                                     "double P = (double)n * M_PI / (a * a);"),
     "rung_rounding": ("float nr = fmaxf(ceilf(log2f(t0f)", "float nr = fmaxf(floorf(log2f(t0f)"),
+    "pme_soft_variable": ("        const float q2 = q * q;\n        rinv3 = +15.0f / 8.0f;",
+                          "        const float q2 = q;\n        rinv3 = +15.0f / 8.0f;"),
     "drift_truncated": ("x[i] += (uint32_t)(uint64_t)llrint((double)vx[i] * f);",
                         "x[i] += (uint32_t)(uint64_t)(int64_t)((double)vx[i] * f);"),
 }

@@ -6,7 +6,7 @@ without the library, but creating a Context raises if the HIP library is
 missing or no GPU is visible.
 """
 from ._lib import (  # noqa: F401
-    TPM_CIC, TPM_TSC, TPM_CLOUD4, TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK,
+    TPM_CIC, TPM_TSC, TPM_CLOUD4, TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK, TPM_KICK_PME,
     Context, TpmError, TpmParams, TpmStats, TpmTuning, lib, lib_path, exported_symbols, DECLARED_SYMBOLS,
     TPM_BATCH_DRIFT, Batch, TpmBatchLimits, TpmBoxSpec, TpmBatchTuning, TpmBatchStats, box_spec, batch_arena_bytes,
 )

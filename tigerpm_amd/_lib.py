@@ -12,6 +12,7 @@ _HEADER = os.path.join(_ROOT, "include", "tigerpm_hip.h")
 
 TPM_CIC, TPM_TSC, TPM_CLOUD4 = 2, 3, 4
 TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK = 1, 2, 4
+TPM_KICK_PME = 16  # kick_pme's pair law (no cutoff, plain softening, no phi0)
 
 _STATUS = {0: "ok", -1: "HIP error", -2: "invalid argument", -3: "call out of order", -4: "rocFFT error",
            -5: "device memory budget exceeded"}

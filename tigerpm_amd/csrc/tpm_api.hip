@@ -291,6 +291,8 @@ extern "C" tpm_status tpm_gravity_long_get_green(tpm_ctx* ctx, float* table) {
 extern "C" tpm_status tpm_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                                tpm_stats* stats) {
     if (!ctx) return TPM_ERR_INVALID;
+    if (flags & ~(TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK | TPM_KICK_PME))
+        return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: unknown flags");
     (void)hipSetDevice(ctx->device);
     if (stats) std::memset(stats, 0, sizeof(*stats));
     TPM_TRY(tpm_pp_kick(ctx, min_rung, scale, t0, first_call, flags, stats));

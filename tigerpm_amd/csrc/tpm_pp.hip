@@ -81,6 +81,9 @@ __device__ __forceinline__ int wave_max(int v) {
 // (util.hpp:36-59, Abramowitz & Stegun 7.1.26), in 2^-32 units.  The sink's
 // own particle is one of its sources, as in the reference: at r = 0 the
 // softening branch gives no force and exactly -35/16 h^-1 to phi.
+// PME = true: kick_pme_kernel's law instead (kick_pme.cu:280-310): no cutoff,
+// and inside h the plain softened polynomials without the (erfc - 1) terms.
+template <bool PME>
 __device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, int oz, int4 s, float& gx, float& gy,
                                         float& gz, float& ph) {
     const float dx = (float)(int)((unsigned)ox - (unsigned)s.x);
@@ -96,7 +99,7 @@ __device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, in
     const float t1 = __builtin_amdgcn_rcpf(fmaf(a.p_inv2rs_s, r, 1.f));
     // exp(-x^2); the cutoff test (kick_treepm.cu:254) zeroes it, which
     // zeroes both erfc and the force factor below.
-    const float e = r2 < a.rcut2s ? __builtin_amdgcn_exp2f(r2 * a.e2s) : 0.f;
+    const float e = (PME || r2 < a.rcut2s) ? __builtin_amdgcn_exp2f(r2 * a.e2s) : 0.f;
     const float poly =
         t1 * fmaf(t1, fmaf(t1, fmaf(t1, fmaf(t1, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
                   0.254829592f);
@@ -108,22 +111,27 @@ __device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, in
         // Closer than h: Plummer polynomial plus the (erfc - 1) correction
         // (kick_treepm.cu:263-276).  At r = 0 (the sink itself) only d1
         // (times dx = 0) and d0 remain.
-        const bool pos = r2 > 0.f;
-        const float ri = pos ? rinv : 0.f;
-        const float em1 = pos ? fmaf(poly, e, -1.0f) : 0.f;
-        const float tre = pos ? a.tsp_inv2rs_s * r * e : 0.f;
         const float q2 = r2 * a.h2inv_s;
         float d1 = 15.0f / 8.0f;
         d1 = fmaf(d1, q2, -21.0f / 4.0f);
         d1 = fmaf(d1, q2, 35.0f / 8.0f);
         d1 *= a.h3inv_s;
-        rinv3 = fmaf((em1 + tre) * ri * ri, ri, d1);
         float d0 = -5.0f / 16.0f;
         d0 = fmaf(d0, q2, 21.0f / 16.0f);
         d0 = fmaf(d0, q2, -35.0f / 16.0f);
         d0 = fmaf(d0, q2, 35.0f / 16.0f);
         d0 *= a.hinv_s;
-        rinv1 = fmaf(em1, ri, d0);
+        if (PME) {
+            rinv3 = d1;
+            rinv1 = d0;
+        } else {
+            const bool pos = r2 > 0.f;
+            const float ri = pos ? rinv : 0.f;
+            const float em1 = pos ? fmaf(poly, e, -1.0f) : 0.f;
+            const float tre = pos ? a.tsp_inv2rs_s * r * e : 0.f;
+            rinv3 = fmaf((em1 + tre) * ri * ri, ri, d1);
+            rinv1 = fmaf(em1, ri, d0);
+        }
     }
     gx = fmaf(-dx, rinv3, gx);
     gy = fmaf(-dy, rinv3, gy);
@@ -158,7 +166,8 @@ __device__ __forceinline__ void interp6_weights(float x, float w[6], float dw[6]
 //     contiguous run per XCD; a workgroup drains its own XCD's queue first
 //     and then helps the others.  Every queue ends, so every workgroup exits.
 // BATCH = true: tile.w names the box whose constants apply (device table).
-template <int TS, bool BATCH, bool PERSIST>
+// PME = true: kick_pme's pair law (the caller turns culling and phi0 off).
+template <int TS, bool BATCH, bool PERSIST, bool PME = false>
 __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
     constexpr int SPLIT = 64 / TS;  // source streams per wave
     __shared__ int4 sbuf[SBUF];
@@ -243,7 +252,7 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
 #pragma unroll 2
                     for (int qq = 0; qq < padded; qq += SPLIT) {
                         const int4 sn = sbuf[qq + SPLIT + stream];
-                        pp_pair(B, ox, oy, oz, s, gx, gy, gz, ph);
+                        pp_pair<PME>(B, ox, oy, oz, s, gx, gy, gz, ph);
                         s = sn;
                     }
                     staged += (unsigned long long)nbuf;
@@ -434,8 +443,15 @@ __global__ void drift_kernel(int64_t n, int nbox, const tpm_box_dev* __restrict_
 }
 
 template <bool BATCH>
-static void pp_launch(int ts, bool persistent, unsigned grid, hipStream_t s, const PPArgs& a) {
-    if (persistent) {
+static void pp_launch(int ts, bool persistent, unsigned grid, hipStream_t s, const PPArgs& a, bool pme = false) {
+    if (pme) {
+        switch (ts) {
+            case 8: pp_kick_kernel<8, BATCH, false, true><<<grid, 64, 0, s>>>(a); break;
+            case 16: pp_kick_kernel<16, BATCH, false, true><<<grid, 64, 0, s>>>(a); break;
+            case 64: pp_kick_kernel<64, BATCH, false, true><<<grid, 64, 0, s>>>(a); break;
+            default: pp_kick_kernel<32, BATCH, false, true><<<grid, 64, 0, s>>>(a); break;
+        }
+    } else if (persistent) {
         switch (ts) {
             case 8: pp_kick_kernel<8, BATCH, true><<<grid, 64, 0, s>>>(a); break;
             case 16: pp_kick_kernel<16, BATCH, true><<<grid, 64, 0, s>>>(a); break;
@@ -527,6 +543,13 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     tpm_box_fill(&a.box0, ctx->p, min_rung, first_call, t0, scale);
     a.box0.cell0 = 0;
     a.box0.mesh0 = 0;
+    const bool pme = (flags & TPM_KICK_PME) != 0;
+    if (pme) {
+        // kick_pme: every pair of the 27 cells counts, and phi starts at 0
+        a.box0.cull = 0;
+        a.box0.phi0 = 0.f;
+    }
+    a.flags = flags & (TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK);
 
     const int64_t ncell = (int64_t)ctx->C * ctx->C * ctx->C;
     const int64_t tiles_max = ctx->n / ctx->tile + ncell + 1;
@@ -544,7 +567,7 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     }
     TPM_HIP(ctx, hipEventRecord(e0, s));
     if (ctx->n > 0) {
-        pp_launch<false>(ctx->tile, false, (unsigned)blocks, s, a);
+        pp_launch<false>(ctx->tile, false, (unsigned)blocks, s, a, pme);
         TPM_HIP(ctx, hipGetLastError());
     }
     TPM_HIP(ctx, hipEventRecord(e1, s));

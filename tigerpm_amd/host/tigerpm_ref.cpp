@@ -360,7 +360,7 @@ std::pair<float, std::array<float, NDIM>> gravity_long_force_at(const std::array
 }
 
 // ------------------------------------------------------------------- kicks
-static void device_kick(int min_rung, double scale, double t0, bool first_call) {
+static void device_kick(int min_rung, double scale, double t0, bool first_call, unsigned flags) {
     push_params(g_long_type);
     if (g_cells.empty()) chainmesh_create();
     if (!g_pm_computed) {
@@ -368,12 +368,12 @@ static void device_kick(int min_rung, double scale, double t0, bool first_call) 
         g_pm_computed = true;
     }
     tpm_stats st{};
-    tpm_status s = tpm_kick(ctx(), min_rung, scale, t0, first_call ? 1 : 0, 0u, &st);
+    tpm_status s = tpm_kick(ctx(), min_rung, scale, t0, first_call ? 1 : 0, flags, &st);
     if (s == TPM_ERR_STATE) {  // particles changed on the host since the sort
         chainmesh_create();
         check(tpm_gravity_long_compute(ctx()), "tpm_gravity_long_compute");
         g_pm_computed = true;
-        s = tpm_kick(ctx(), min_rung, scale, t0, first_call ? 1 : 0, 0u, &st);
+        s = tpm_kick(ctx(), min_rung, scale, t0, first_call ? 1 : 0, flags, &st);
     }
     check(s, "tpm_kick");
     g_report.ms_pp = st.ms_pp;
@@ -386,11 +386,13 @@ static void device_kick(int min_rung, double scale, double t0, bool first_call) 
 // reference builds per-cell trees and walks them; this build evaluates the
 // same erfc-split pair force over the 27 neighbour cells directly.
 void kick_treepm_begin(int min_rung, double scale, double t0, bool first_call) {
-    device_kick(min_rung, scale, t0, first_call);
+    device_kick(min_rung, scale, t0, first_call, 0u);
 }
 void kick_treepm_end() {}  // no ghost particles to drop on one device
+// kick_pme_begin (kick_pme.cpp, kick_pme.cu:128-357): the same kick with
+// kick_pme's pair law (no cutoff, plain softening inside h, no phi0).
 void kick_pme_begin(int min_rung, double scale, double t0, bool first_call) {
-    device_kick(min_rung, scale, t0, first_call);
+    device_kick(min_rung, scale, t0, first_call, TPM_KICK_PME);
 }
 void kick_pme_end() {}
 
@@ -520,12 +522,18 @@ void run_test(std::string test) {
             particles_random_init();
     };
     if (test == "treepm" || test == "kick_pme") {
-        timed("INIT", +[]() {
-            if (get_options().ic == "zeldovich")
-                initialize();
-            else
-                particles_random_init();
-        });
+        // kick_treepm_test / kick_pme_test (test.cpp:237-290, :182-235); the
+        // kick_pme scenario starts from random particles
+        const bool pme = test == "kick_pme";
+        if (pme)
+            timed("INIT", particles_random_init);
+        else
+            timed("INIT", +[]() {
+                if (get_options().ic == "zeldovich")
+                    initialize();
+                else
+                    particles_random_init();
+            });
         timed("DOMAIN SORT", particles_domain_sort);
         timed("FOURIER", +[]() { gravity_long_compute(GRAVITY_LONG_PME); });
         timed("SORT", chainmesh_create);
@@ -533,10 +541,16 @@ void run_test(std::string test) {
             chainmesh_exchange_begin();
             chainmesh_exchange_end();
         });
-        timed("KICK", +[]() {
-            kick_treepm_begin(0, 1.0, 1.0, true);
-            kick_treepm_end();
-        });
+        if (pme)
+            timed("KICK", +[]() {
+                kick_pme_begin(0, 1.0, 1.0, true);
+                kick_pme_end();
+            });
+        else
+            timed("KICK", +[]() {
+                kick_treepm_begin(0, 1.0, 1.0, true);
+                kick_treepm_end();
+            });
         const kick_report r = last_kick_report();
         std::printf("PP kernel %e ms, candidate pairs %e, evaluated pairs %e\n", r.ms_pp, r.candidate_pairs,
                     r.executed_pairs);
