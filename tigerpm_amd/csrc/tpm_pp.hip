@@ -167,8 +167,39 @@ __device__ __forceinline__ void interp6_weights(float x, float w[6], float dw[6]
 //     and then helps the others.  Every queue ends, so every workgroup exits.
 // BATCH = true: tile.w names the box whose constants apply (device table).
 // PME = true: kick_pme's pair law (the caller turns culling and phi0 off).
+// Weight and derivative k of the same 6-point set (k a compile-time index
+// when the caller's loop is unrolled).
+__device__ __forceinline__ void interp6_weight(float x, int k, float* w, float* dw) {
+    switch (k) {
+        case 0:
+            *w = x * ((1.f / 12.f) + x * (-(1.f / 24.f) + x * (-(3.f / 8.f) + x * ((13.f / 24.f) + x * -(5.f / 24.f)))));
+            *dw = (1.f / 12.f) + x * (-(2.f / 24.f) + x * (-(9.f / 8.f) + x * ((52.f / 24.f) + x * -(25.f / 24.f))));
+            break;
+        case 1:
+            *w = x * (-(2.f / 3.f) + x * ((2.f / 3.f) + x * ((13.f / 8.f) + x * (-(8.f / 3.f) + x * (25.f / 24.f)))));
+            *dw = -(2.f / 3.f) + x * ((4.f / 3.f) + x * ((39.f / 8.f) + x * (-(32.f / 3.f) + x * (125.f / 24.f))));
+            break;
+        case 2:
+            *w = 1.0f + x * x * (-(5.f / 4.f) + x * (-(35.f / 12.f) + x * ((21.f / 4.f) + x * -(25.f / 12.f))));
+            *dw = x * (-(10.f / 4.f) + x * (-(105.f / 12.f) + x * ((84.f / 4.f) + x * -(125.f / 12.f))));
+            break;
+        case 3:
+            *w = x * ((2.f / 3.f) + x * ((2.f / 3.f) + x * ((11.f / 4.f) + x * (-(31.f / 6.f) + x * (25.f / 12.f)))));
+            *dw = (2.f / 3.f) + x * ((4.f / 3.f) + x * ((33.f / 4.f) + x * (-(124.f / 6.f) + x * (125.f / 12.f))));
+            break;
+        case 4:
+            *w = x * (-(1.f / 12.f) + x * (-(1.f / 24.f) + x * (-(11.f / 8.f) + x * ((61.f / 24.f) + x * -(25.f / 24.f)))));
+            *dw = -(1.f / 12.f) + x * (-(2.f / 24.f) + x * (-(33.f / 8.f) + x * ((244.f / 24.f) + x * -(125.f / 24.f))));
+            break;
+        default:
+            *w = x * x * x * ((7.f / 24.f) + x * (-0.5f + x * (5.f / 24.f)));
+            *dw = x * x * ((21.f / 24.f) + x * (-2.0f + x * (25.f / 24.f)));
+            break;
+    }
+}
+
 template <int TS, bool BATCH, bool PERSIST, bool PME = false>
-__global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
+__global__ __launch_bounds__(64, PERSIST ? 1 : 7) void pp_kick_kernel(PPArgs a) {
     constexpr int SPLIT = 64 / TS;  // source streams per wave
     __shared__ int4 sbuf[SBUF];
     const int nt = *a.ntiles;
@@ -317,45 +348,66 @@ __global__ __launch_bounds__(64) void pp_kick_kernel(PPArgs a) {
                 // periodic mesh.
                 float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
                 if (a.phi && !(a.flags & TPM_SKIP_PM)) {
+                    // Separable: along z (w, dw), then y, then x, so only the
+                    // y and z weights are live; x weights are made where used.
                     const int M = B.M;
                     const float* phi = a.phi + B.mesh0;
                     const uint32_t pr[3] = {px, py, pz};
                     int I0[3];
-                    float w[3][6], dw[3][6];
+                    float X[3];
 #pragma unroll
                     for (int d = 0; d < 3; d++) {
                         const uint64_t P = (uint64_t)pr[d] * (uint64_t)M;
                         I0[d] = (int)(P >> 32) - 2 + M;
-                        const float X = (float)((uint32_t)P >> 8) * 0x1p-24f;
-                        interp6_weights(X, w[d], dw[d]);
+                        X[d] = (float)((uint32_t)P >> 8) * 0x1p-24f;
                     }
-                    const float Mf = (float)M;
+                    float wy[6], dwy[6], wz[6], dwz[6];
+                    interp6_weights(X[1], wy, dwy);
+                    interp6_weights(X[2], wz, dwz);
+                    int jz[6];
+#pragma unroll
+                    for (int cc = 0; cc < 6; cc++) {
+                        int j = I0[2] + cc;
+                        j -= (j >= M) ? M : 0;
+                        j -= (j >= M) ? M : 0;
+                        jz[cc] = j;
+                    }
+                    float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
+#pragma unroll 1
                     for (int aa = 0; aa < 6; aa++) {
                         int jx = I0[0] + aa;
                         jx -= (jx >= M) ? M : 0;
                         jx -= (jx >= M) ? M : 0;
+                        float u = 0.f, v = 0.f, t = 0.f;  // sum_y w s, sum_y dw s, sum_y w tz
+#pragma unroll 1
                         for (int bb = 0; bb < 6; bb++) {
                             int jy = I0[1] + bb;
                             jy -= (jy >= M) ? M : 0;
                             jy -= (jy >= M) ? M : 0;
                             const float* row = phi + ((int64_t)jx * M + jy) * M;
-                            const float wxy = w[0][aa] * w[1][bb];
-                            const float dxy = dw[0][aa] * w[1][bb];
-                            const float xdy = w[0][aa] * dw[1][bb];
+                            float s0 = 0.f, s1 = 0.f;  // sum_z w phi, sum_z dw phi
 #pragma unroll
                             for (int cc = 0; cc < 6; cc++) {
-                                int jz = I0[2] + cc;
-                                jz -= (jz >= M) ? M : 0;
-                                jz -= (jz >= M) ? M : 0;
-                                const float p0 = row[jz];
-                                const float p1 = p0 * Mf;
-                                pgx = fmaf(-dxy * w[2][cc], p1, pgx);
-                                pgy = fmaf(-xdy * w[2][cc], p1, pgy);
-                                pgz = fmaf(-wxy * dw[2][cc], p1, pgz);
-                                pph = fmaf(wxy * w[2][cc], p0, pph);
+                                const float p0 = row[jz[cc]];
+                                s0 = fmaf(wz[cc], p0, s0);
+                                s1 = fmaf(dwz[cc], p0, s1);
                             }
+                            u = fmaf(wy[bb], s0, u);
+                            v = fmaf(dwy[bb], s0, v);
+                            t = fmaf(wy[bb], s1, t);
                         }
+                        float wx, dwx;
+                        interp6_weight(X[0], aa, &wx, &dwx);
+                        sx = fmaf(dwx, u, sx);
+                        sy = fmaf(wx, v, sy);
+                        sz = fmaf(wx, t, sz);
+                        sp = fmaf(wx, u, sp);
                     }
+                    const float Mf = (float)M;
+                    pgx = -sx * Mf;
+                    pgy = -sy * Mf;
+                    pgz = -sz * Mf;
+                    pph = sp;
                 }
 
                 // Kick epilogue (kick_treepm.cu:716-754); rung_dt[r] = 2^-r.
