@@ -19,6 +19,7 @@ _LIB_PATH = os.environ.get("TPM_ORACLE_LIB") or os.path.join(_HERE, "liboracle_t
 u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 i8p = np.ctypeslib.ndpointer(np.int8, flags="C_CONTIGUOUS")
 f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -92,6 +93,12 @@ def lib():
             "orc_ewald_direct": (None, [c_i64, u32p, u32p, u32p, c_i64, i64p, c_dbl, f64p, f64p, f64p, f64p]),
             "orc_ewald_sf": (None, [c_i64, u32p, u32p, u32p, c_i64, i64p, c_dbl, c_dbl, f64p, f64p, f64p,
                                     f64p]),
+            "orc_tree_build": (c_int, [ctypes.POINTER(OrcParams), c_i64, u32p, u32p, u32p, i64p, i32p, u32p, u32p,
+                                       u32p, i64p, i64p, ctypes.c_void_p, c_i64, ctypes.c_void_p, c_i64]),
+            "orc_tree_sum": (c_int, [ctypes.POINTER(OrcParams), c_i64, u32p, u32p, u32p, i64p, i8p, c_int, c_int,
+                                     f32p, f32p, f32p, f32p, f64p]),
+            "orc_pc_pair": (None, [ctypes.POINTER(OrcParams), ctypes.c_float, ctypes.c_float, ctypes.c_float, f32p,
+                                   f32p]),
             "orc_num_threads": (c_int, []),
         }
         for name, (res, args) in sig.items():
@@ -254,8 +261,10 @@ def drift(x, y, z, vx, vy, vz, t0):
     return x, y, z
 
 
-def kick_full(params, x, y, z, vx, vy, vz, rung, min_rung=0, scale=1.0, t0=1.0, first_call=True, fp64_pp=False):
-    """Whole CPU kick. Returns dict of sorted arrays and stage times."""
+def kick_full(params, x, y, z, vx, vy, vz, rung, min_rung=0, scale=1.0, t0=1.0, first_call=True, fp64_pp=False,
+              tree=False):
+    """Whole CPU kick. Returns dict of sorted arrays and stage times.
+    fp64_pp: the direct sum in FP64; tree: kick_treepm's tree walk."""
     x, y, z = [np.array(a, np.uint32) for a in (x, y, z)]
     vx, vy, vz = [np.array(a, np.float32) for a in (vx, vy, vz)]
     rung = np.array(rung, np.int8)
@@ -265,7 +274,9 @@ def kick_full(params, x, y, z, vx, vy, vz, rung, min_rung=0, scale=1.0, t0=1.0, 
     cb = np.empty(C ** 3 + 1, np.int64)
     times = np.zeros(5, np.float64)
     lib().orc_kick_full(ctypes.byref(params), n, x, y, z, vx, vy, vz, rung, g[0], g[1], g[2], g[3], cb, min_rung,
-                        scale, t0, int(first_call), int(fp64_pp), times)
+                        scale, t0, int(first_call), 2 if tree else int(fp64_pp), times)
+    if times[3] < 0:
+        raise RuntimeError("tree depth limit exceeded")
     return dict(x=x, y=y, z=z, vx=vx, vy=vy, vz=vz, rung=rung, gx=g[0], gy=g[1], gz=g[2], pot=g[3],
                 cell_begin=cb, times=times)
 
@@ -289,6 +300,58 @@ def ewald_sf(x, y, z, samples, hsoft, alpha=10.0):
     samples = np.ascontiguousarray(samples, np.int64)
     out = [np.empty(len(samples), np.float64) for _ in range(4)]
     lib().orc_ewald_sf(len(x), x, y, z, len(samples), samples, hsoft, alpha, *out)
+    return out
+
+
+# Tree node and sink bucket of the per-cell trees (tree.hpp:17-36).
+TREE_NODE = np.dtype([("x", "<u4", 3), ("radius", "<f4"), ("child", "<i4", 2), ("begin", "<i4"), ("end", "<i4"),
+                      ("m", "<f4", 4)])
+TREE_BUCKET = np.dtype([("begin", "<i4"), ("end", "<i4"), ("x", "<u4", 3), ("radius", "<f4")])
+
+
+def tree_build(params, x, y, z, cell_begin):
+    """Per-cell trees of the chain-mesh-sorted particles.  Returns a dict:
+    order (tree position -> input index), tx/ty/tz, node_begin, bucket_begin,
+    nodes, buckets.  Raises on a tree deeper than the reference allows."""
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    n = len(x)
+    C3 = params.chain_dim ** 3
+    cb = np.ascontiguousarray(cell_begin, np.int64)
+    order = np.empty(n, np.int32)
+    t = [np.empty(n, np.uint32) for _ in range(3)]
+    nb, bb = np.empty(C3 + 1, np.int64), np.empty(C3 + 1, np.int64)
+    rc = lib().orc_tree_build(ctypes.byref(params), n, x, y, z, cb, order, *t, nb, bb, None, 0, None, 0)
+    if rc == 1:
+        raise RuntimeError("tree depth limit exceeded")
+    nodes = np.empty(int(nb[-1]), TREE_NODE)
+    buckets = np.empty(int(bb[-1]), TREE_BUCKET)
+    rc = lib().orc_tree_build(ctypes.byref(params), n, x, y, z, cb, order, *t, nb, bb, nodes.ctypes.data, len(nodes),
+                              buckets.ctypes.data, len(buckets))
+    assert rc == 0
+    return dict(order=order, tx=t[0], ty=t[1], tz=t[2], node_begin=nb, bucket_begin=bb, nodes=nodes,
+                buckets=buckets)
+
+
+def tree_sum(params, x, y, z, cell_begin, rung, min_rung=0, pc_exact=False):
+    """kick_treepm's short-range sum (tree walk, PC + PP) for the active
+    sinks of the chain-mesh-sorted particles; unit-mass sums in input order
+    and stats {pc, pp, multi, part}.  pc_exact: the multipole-list nodes
+    summed pair by pair instead (a check of the walk's coverage)."""
+    x, y, z = _u32(x), _u32(y), _u32(z)
+    n = len(x)
+    out = [np.empty(n, np.float32) for _ in range(4)]
+    st = np.zeros(4, np.float64)
+    rc = lib().orc_tree_sum(ctypes.byref(params), n, x, y, z, np.ascontiguousarray(cell_begin, np.int64),
+                            np.ascontiguousarray(rung, np.int8), int(min_rung), int(pc_exact), *out, st)
+    if rc:
+        raise RuntimeError("tree depth limit exceeded")
+    return out, dict(pc=st[0], pp=st[1], multi=st[2], part=st[3])
+
+
+def pc_pair(params, dx, dy, dz, M):
+    """One multipole (PC) interaction; M = (mass, -dipole)."""
+    out = np.empty(4, np.float32)
+    lib().orc_pc_pair(ctypes.byref(params), float(dx), float(dy), float(dz), np.ascontiguousarray(M, np.float32), out)
     return out
 
 

@@ -904,6 +904,590 @@ ORC_API double orc_candidate_pairs(int C, const int64_t* cell_begin) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Per-cell trees, tree walk and multipole (PC) interactions of kick_treepm  */
+/* (tree.cpp:7-233, kernels.hpp:13-152, kick_treepm.cu:226-240, 586-713).   */
+/* ------------------------------------------------------------------------ */
+
+#define ORC_TREE_LEAF 32    /* min(SOURCE_BUCKET_SIZE, SINK_BUCKET_SIZE), tigerpm.hpp:32-33 */
+#define ORC_TREE_BUCKET 128 /* SINK_BUCKET_SIZE, tigerpm.hpp:32 */
+#define ORC_TREE_DEPTH 50   /* MAX_DEPTH, tree.cpp:5 */
+#define ORC_TREE_THETA 0.4f /* params.theta, kick_treepm.cu:829 */
+
+/* tree_node (tree.hpp:17-27) with the children as indices inside the cell's
+ * own tree (preorder: a node precedes its subtrees, child 0 first). */
+typedef struct {
+    uint32_t x[3];      /* expansion centre, fixed32 */
+    float radius;       /* bounds the node's particles, plus hsoft */
+    int32_t child[2];   /* -1 for a leaf */
+    int32_t begin, end; /* particle range, tree order */
+    float m[4];         /* particle count and -(sum of offsets from the centre) */
+} orc_tree_node;
+
+/* sink_bucket (tree.hpp:29-36). */
+typedef struct {
+    int32_t begin, end;
+    uint32_t x[3];
+    float radius;
+} orc_tree_bucket;
+
+/* fixed32 from double (fixed.hpp:63-75): 2^32 x truncated; a centre that
+ * falls just outside [0, 1) wraps, which distance() treats periodically. */
+static inline uint32_t tree_fixed(double x) { return (uint32_t)(int64_t)(4294967296.0 * x); }
+
+/* sqr(a, b, c) (util.hpp:32-34) is fmaf(a, a, fmaf(b, b, c * c)); fmaf takes
+ * float, so the double instantiation that tree.cpp uses rounds to FP32. */
+static inline double sqr3_d(double a, double b, double c) {
+    return (double)fmaf((float)a, (float)a, fmaf((float)b, (float)b, (float)(c * c)));
+}
+static inline float sqr3_f(float a, float b, float c) { return fmaf(a, a, fmaf(b, b, c * c)); }
+static inline double dmax(double a, double b) { return (a < b) ? b : a; } /* std::max */
+static inline double dmin(double a, double b) { return (b < a) ? b : a; } /* std::min */
+
+typedef struct {
+    orc_tree_node* nodes;
+    int64_t nn, ncap;
+    orc_tree_bucket* buckets;
+    int64_t nb, bcap;
+    uint32_t* pos[3]; /* tree-order positions (whole arrays, permuted per cell) */
+    int32_t* idx;     /* tree order -> input order */
+    uint32_t* tpos[3];
+    int32_t* tidx; /* partition scratch, one cell long */
+    double hsoft;
+    int err;
+} tree_cell;
+
+/* particles_sort (particles.cpp:414-445) puts x < xmid on the left.  Here
+ * each side keeps its previous relative order (a stable partition); that
+ * fixes the order of the particles inside a leaf, which the reference leaves
+ * to its swap sequence. */
+static int tree_partition(tree_cell* t, int begin, int end, uint32_t thr, int dim) {
+    int nl = 0;
+    for (int i = begin; i < end; i++) nl += t->pos[dim][i] < thr;
+    int l = 0, r = nl;
+    for (int i = begin; i < end; i++) {
+        const int k = (t->pos[dim][i] < thr) ? l++ : r++;
+        for (int d = 0; d < 3; d++) t->tpos[d][k] = t->pos[d][i];
+        t->tidx[k] = t->idx[i];
+    }
+    for (int k = 0; k < end - begin; k++) {
+        for (int d = 0; d < 3; d++) t->pos[d][begin + k] = t->tpos[d][k];
+        t->idx[begin + k] = t->tidx[k];
+    }
+    return begin + nl;
+}
+
+/* Centre of the bounding box of [begin, end) (the box centre when empty)
+ * and the largest squared distance of a particle from it, in double
+ * (tree.cpp:25-59, 88-99 and 173-206). */
+static double tree_bbox(const tree_cell* t, const double bb[3], const double be[3], int begin, int end, double x[3]) {
+    double xmin[3] = {1.0, 1.0, 1.0}, xmax[3] = {0.0, 0.0, 0.0};
+    for (int i = begin; i < end; i++)
+        for (int d = 0; d < 3; d++) {
+            const double v = orc_fixed_to_double(t->pos[d][i]);
+            xmax[d] = dmax(xmax[d], v);
+            xmin[d] = dmin(xmin[d], v);
+        }
+    for (int d = 0; d < 3; d++) x[d] = (end > begin) ? (xmax[d] + xmin[d]) * 0.5 : (bb[d] + be[d]) * 0.5;
+    double r2max = 0.0;
+    for (int i = begin; i < end; i++) {
+        double r2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double e = x[d] - orc_fixed_to_double(t->pos[d][i]);
+            r2 += e * e;
+        }
+        r2max = dmax(r2max, r2);
+    }
+    return r2max;
+}
+
+/* sort() of tree.cpp:7-214: builds the node of [begin, end) in the box
+ * [bb, be), recursing into the two halves of its longest side. */
+static int tree_sort(tree_cell* t, const double bb[3], const double be[3], int begin, int end, int depth, int sunk) {
+    if (depth > ORC_TREE_DEPTH) {
+        t->err = 1; /* tree.cpp:16-19 aborts */
+        return -1;
+    }
+    if (t->nn == t->ncap) {
+        t->ncap = t->ncap ? 2 * t->ncap : 64;
+        t->nodes = realloc(t->nodes, sizeof(orc_tree_node) * (size_t)t->ncap);
+    }
+    const int index = (int)t->nn++;
+    orc_tree_node node;
+    node.begin = begin;
+    node.end = end;
+    if (end - begin <= ORC_TREE_LEAF) {
+        double x[3];
+        const double r2max = tree_bbox(t, bb, be, begin, end, x);
+        float m[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int i = begin; i < end; i++) {
+            /* P2M_kernel (kernels.hpp:121-131) of the FP32 offset */
+            float dx[3];
+            for (int d = 0; d < 3; d++) dx[d] = (float)(orc_fixed_to_double(t->pos[d][i]) - x[d]);
+            m[0] += 1.0f;
+            m[1] += -dx[0];
+            m[2] += -dx[1];
+            m[3] += -dx[2];
+        }
+        for (int j = 0; j < 4; j++) node.m[j] = m[j];
+        for (int d = 0; d < 3; d++) node.x[d] = tree_fixed(x[d]);
+        node.radius = (float)(sqrt(r2max) + t->hsoft);
+        node.child[0] = node.child[1] = -1;
+    } else {
+        /* range::split (range.hpp:118-133): the first strictly longest side */
+        int dim = 0;
+        double span = 0.0;
+        for (int d = 0; d < 3; d++)
+            if (be[d] - bb[d] > span) {
+                span = be[d] - bb[d];
+                dim = d;
+            }
+        const double mid = (be[dim] + bb[dim]) / 2.0;
+        const int m = tree_partition(t, begin, end, tree_fixed(mid), dim);
+        const int this_sunk = end - begin <= ORC_TREE_BUCKET;
+        double lb[3], le[3], rb[3], re[3];
+        for (int d = 0; d < 3; d++) {
+            lb[d] = rb[d] = bb[d];
+            le[d] = re[d] = be[d];
+        }
+        le[dim] = rb[dim] = mid;
+        const int i0 = tree_sort(t, lb, le, begin, m, depth + 1, this_sunk);
+        const int i1 = tree_sort(t, rb, re, m, end, depth + 1, this_sunk);
+        if (t->err) return -1;
+        const orc_tree_node* c0 = &t->nodes[i0];
+        const orc_tree_node* c1 = &t->nodes[i1];
+        const float mass0 = c0->m[0], mass1 = c1->m[0];
+        double n[3], x[3];
+        for (int d = 0; d < 3; d++) n[d] = orc_fixed_to_double(c1->x[d]) - orc_fixed_to_double(c0->x[d]);
+        const double norminv = 1.0 / sqrt(sqr3_d(n[0], n[1], n[2]));
+        for (int d = 0; d < 3; d++) n[d] *= norminv;
+        for (int d = 0; d < 3; d++) {
+            const double x0 = orc_fixed_to_double(c0->x[d]), x1 = orc_fixed_to_double(c1->x[d]);
+            if (mass1 == 0.0f)
+                x[d] = x0;
+            else if (mass0 == 0.0f)
+                x[d] = x1;
+            else
+                x[d] = (x0 + x1 + n[d] * (double)(c1->radius - c0->radius)) * 0.5;
+        }
+        /* M2M_kernel (kernels.hpp:134-152) of both children, summed */
+        float mm[2][4];
+        const orc_tree_node* cs[2] = {c0, c1};
+        for (int k = 0; k < 2; k++) {
+            float X[3];
+            for (int d = 0; d < 3; d++) X[d] = -(float)(orc_fixed_to_double(cs[k]->x[d]) - x[d]);
+            const float* Ma = cs[k]->m;
+            mm[k][0] = Ma[0];
+            mm[k][1] = fmaf(X[0], Ma[0], Ma[1]);
+            mm[k][3] = fmaf(X[2], Ma[0], Ma[3]);
+            mm[k][2] = fmaf(X[1], Ma[0], Ma[2]);
+        }
+        for (int j = 0; j < 4; j++) node.m[j] = mm[0][j] + mm[1][j];
+        for (int d = 0; d < 3; d++) node.x[d] = tree_fixed(x[d]);
+        double r20 = 0.0, r21 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            if (mass0 != 0.0f) {
+                const double e = orc_fixed_to_double(c0->x[d]) - x[d];
+                r20 += e * e;
+            }
+            if (mass1 != 0.0f) {
+                const double e = orc_fixed_to_double(c1->x[d]) - x[d];
+                r21 += e * e;
+            }
+        }
+        double r2_max = 0.0;
+        for (int corner = 0; corner < 8; corner++) {
+            const double cx = (corner & 4) ? be[0] : bb[0];
+            const double cy = (corner & 2) ? be[1] : bb[1];
+            const double cz = (corner & 1) ? be[2] : bb[2];
+            r2_max = dmax(r2_max, sqr3_d(cx - x[0], cy - x[1], cz - x[2]));
+        }
+        node.radius = (float)dmin(dmax(sqrt(r20) + (double)c0->radius, sqrt(r21) + (double)c1->radius),
+                                  sqrt(r2_max) + t->hsoft);
+        node.child[0] = i0;
+        node.child[1] = i1;
+    }
+    if (end - begin <= ORC_TREE_BUCKET && !sunk) {
+        if (t->nb == t->bcap) {
+            t->bcap = t->bcap ? 2 * t->bcap : 16;
+            t->buckets = realloc(t->buckets, sizeof(orc_tree_bucket) * (size_t)t->bcap);
+        }
+        orc_tree_bucket* bk = &t->buckets[t->nb++];
+        double x[3];
+        const double r2max = tree_bbox(t, bb, be, begin, end, x);
+        for (int d = 0; d < 3; d++) bk->x[d] = tree_fixed(x[d]);
+        bk->radius = (float)(sqrt(r2max) + t->hsoft);
+        bk->begin = begin;
+        bk->end = end;
+    }
+    t->nodes[index] = node;
+    return index;
+}
+
+typedef struct {
+    int64_t ncell;
+    int64_t* node_begin;   /* ncell + 1 */
+    int64_t* bucket_begin; /* ncell + 1 */
+    orc_tree_node* nodes;
+    orc_tree_bucket* buckets;
+    uint32_t* pos[3];
+    int32_t* idx;
+    int err;
+} orc_forest;
+
+static void forest_free(orc_forest* f) {
+    free(f->node_begin);
+    free(f->bucket_begin);
+    free(f->nodes);
+    free(f->buckets);
+    for (int d = 0; d < 3; d++) free(f->pos[d]);
+    free(f->idx);
+}
+
+/* tree_create (tree.cpp:216-245) for every chain cell, as the loop of
+ * kick_treepm_begin (kick_treepm.cpp:26-44) does: the root box of cell i is
+ * [i / C, (i + 1) / C) per axis in double. */
+static void forest_build(const orc_params* p, int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z,
+                         const int64_t* cell_begin, orc_forest* f) {
+    const int C = p->chain_dim;
+    const int64_t ncell = (int64_t)C * C * C;
+    const double ninv = 1.0 / (double)C;
+    f->ncell = ncell;
+    f->err = 0;
+    const uint32_t* src[3] = {x, y, z};
+    for (int d = 0; d < 3; d++) {
+        f->pos[d] = malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+        memcpy(f->pos[d], src[d], sizeof(uint32_t) * (size_t)n);
+    }
+    f->idx = malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; i++) f->idx[i] = (int32_t)i;
+    orc_tree_node** cn = calloc((size_t)ncell, sizeof(orc_tree_node*));
+    orc_tree_bucket** cbk = calloc((size_t)ncell, sizeof(orc_tree_bucket*));
+    f->node_begin = calloc((size_t)ncell + 1, sizeof(int64_t));
+    f->bucket_begin = calloc((size_t)ncell + 1, sizeof(int64_t));
+    int64_t maxcell = 1;
+    for (int64_t c = 0; c < ncell; c++)
+        if (cell_begin[c + 1] - cell_begin[c] > maxcell) maxcell = cell_begin[c + 1] - cell_begin[c];
+    int err = 0;
+#pragma omp parallel reduction(| : err)
+    {
+        tree_cell t;
+        memset(&t, 0, sizeof(t));
+        for (int d = 0; d < 3; d++) {
+            t.pos[d] = f->pos[d];
+            t.tpos[d] = malloc(sizeof(uint32_t) * (size_t)maxcell);
+        }
+        t.idx = f->idx;
+        t.tidx = malloc(sizeof(int32_t) * (size_t)maxcell);
+        t.hsoft = p->hsoft;
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t c = 0; c < ncell; c++) {
+            const int ix = (int)(c % C), iy = (int)((c / C) % C), iz = (int)(c / ((int64_t)C * C));
+            const int ic[3] = {ix, iy, iz};
+            double bb[3], be[3];
+            for (int d = 0; d < 3; d++) {
+                bb[d] = ic[d] * ninv;
+                be[d] = (ic[d] + 1) * ninv;
+            }
+            t.nn = t.nb = 0;
+            t.err = 0;
+            tree_sort(&t, bb, be, (int)cell_begin[c], (int)cell_begin[c + 1], 0, 0);
+            err |= t.err;
+            cn[c] = malloc(sizeof(orc_tree_node) * (size_t)(t.nn > 0 ? t.nn : 1));
+            memcpy(cn[c], t.nodes, sizeof(orc_tree_node) * (size_t)t.nn);
+            cbk[c] = malloc(sizeof(orc_tree_bucket) * (size_t)(t.nb > 0 ? t.nb : 1));
+            memcpy(cbk[c], t.buckets, sizeof(orc_tree_bucket) * (size_t)t.nb);
+            f->node_begin[c + 1] = t.nn;
+            f->bucket_begin[c + 1] = t.nb;
+        }
+        for (int d = 0; d < 3; d++) free(t.tpos[d]);
+        free(t.tidx);
+        free(t.nodes);
+        free(t.buckets);
+    }
+    f->err = err;
+    for (int64_t c = 0; c < ncell; c++) {
+        f->node_begin[c + 1] += f->node_begin[c];
+        f->bucket_begin[c + 1] += f->bucket_begin[c];
+    }
+    f->nodes = malloc(sizeof(orc_tree_node) * (size_t)(f->node_begin[ncell] + 1));
+    f->buckets = malloc(sizeof(orc_tree_bucket) * (size_t)(f->bucket_begin[ncell] + 1));
+    for (int64_t c = 0; c < ncell; c++) {
+        memcpy(f->nodes + f->node_begin[c], cn[c], sizeof(orc_tree_node) * (size_t)(f->node_begin[c + 1] - f->node_begin[c]));
+        memcpy(f->buckets + f->bucket_begin[c], cbk[c],
+               sizeof(orc_tree_bucket) * (size_t)(f->bucket_begin[c + 1] - f->bucket_begin[c]));
+        free(cn[c]);
+        free(cbk[c]);
+    }
+    free(cn);
+    free(cbk);
+}
+
+/* The trees of every chain cell.  order[i] = input index of tree position i;
+ * tx/ty/tz = positions in tree order; node_begin/bucket_begin get C^3 + 1
+ * offsets.  Node and bucket arrays are filled when their capacity suffices.
+ * Returns 0, 1 (depth limit exceeded) or 2 (capacity too small). */
+ORC_API int orc_tree_build(const orc_params* p, int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z,
+                           const int64_t* cell_begin, int32_t* order, uint32_t* tx, uint32_t* ty, uint32_t* tz,
+                           int64_t* node_begin, int64_t* bucket_begin, orc_tree_node* nodes, int64_t node_cap,
+                           orc_tree_bucket* buckets, int64_t bucket_cap) {
+    orc_forest f;
+    forest_build(p, n, x, y, z, cell_begin, &f);
+    memcpy(order, f.idx, sizeof(int32_t) * (size_t)n);
+    memcpy(tx, f.pos[0], sizeof(uint32_t) * (size_t)n);
+    memcpy(ty, f.pos[1], sizeof(uint32_t) * (size_t)n);
+    memcpy(tz, f.pos[2], sizeof(uint32_t) * (size_t)n);
+    memcpy(node_begin, f.node_begin, sizeof(int64_t) * (size_t)(f.ncell + 1));
+    memcpy(bucket_begin, f.bucket_begin, sizeof(int64_t) * (size_t)(f.ncell + 1));
+    int rc = f.err ? 1 : 0;
+    if (!rc) {
+        if (f.node_begin[f.ncell] <= node_cap && f.bucket_begin[f.ncell] <= bucket_cap) {
+            memcpy(nodes, f.nodes, sizeof(orc_tree_node) * (size_t)f.node_begin[f.ncell]);
+            memcpy(buckets, f.buckets, sizeof(orc_tree_bucket) * (size_t)f.bucket_begin[f.ncell]);
+        } else {
+            rc = 2;
+        }
+    }
+    forest_free(&f);
+    return rc;
+}
+
+/* compute_pc_interaction (kick_treepm.cu:226-240): greens_function
+ * (kernels.hpp:13-66) and the 4-component M2L_kernel (kernels.hpp:69-88),
+ * as written.  d = sink - expansion centre. */
+static inline void pc_fp32(const pp_consts* c, float dx, float dy, float dz, const float* M, float* g, float* phi) {
+    float D[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const float r2 = sqr3_f(dx, dy, dz);
+    const float rinv = 1.0f / sqrtf(r2);
+    const float r = r2 * rinv;
+    const float r0 = r * c->inv2rs;
+    float exp0;
+    const float erfc0 = erfcexp(r0, &exp0);
+    const float c0 = -2.f * r * c->inv2rs;
+    const float d0 = -erfc0 * rinv;
+    float e0 = c->twooversqrtpi * exp0 * rinv * c->inv2rs;
+    const float d1 = fmaf(-1.f * d0, rinv, e0);
+    e0 *= c0;
+    const float d2 = fmaf(-3.f * d1, rinv, e0);
+    const float D00 = d0 * 1.f, D10 = d1 * 1.f, D11 = d1 * rinv, D20 = d2 * 1.f;
+    const float x100 = dx * rinv, x010 = dy * rinv, x001 = dz * rinv;
+    const float x002 = x001 * x001, x011 = x010 * x001, x020 = x010 * x010;
+    const float x101 = x100 * x001, x110 = x100 * x010, x200 = x100 * x100;
+    float x21 = x002;
+    x21 += x020;
+    x21 += x200;
+    x21 *= D11;
+    D[0] = fmaf(1.f, D00, D[0]);
+    D[7] += x21;
+    D[3] = fmaf(x001, D10, D[3]);
+    D[1] = fmaf(x100, D10, D[1]);
+    D[9] = fmaf(x002, D20, D[9]);
+    D[6] = fmaf(x101, D20, D[6]);
+    D[9] += x21;
+    D[5] = fmaf(x110, D20, D[5]);
+    D[2] = fmaf(x010, D10, D[2]);
+    D[4] = fmaf(x200, D20, D[4]);
+    D[8] = fmaf(x011, D20, D[8]);
+    D[4] += x21;
+    D[7] = fmaf(x020, D20, D[7]);
+    float L[4] = {0.f, 0.f, 0.f, 0.f};
+    if (c->do_phi) {
+        L[0] = fmaf(M[0], D[0], L[0]);
+        L[0] = fmaf(M[3], D[3], L[0]);
+        L[0] = fmaf(M[2], D[2], L[0]);
+        L[0] = fmaf(M[1], D[1], L[0]);
+    }
+    L[1] = fmaf(M[0], D[1], L[1]);
+    L[2] = fmaf(M[2], D[7], L[2]);
+    L[1] = fmaf(M[3], D[6], L[1]);
+    L[2] = fmaf(M[1], D[5], L[2]);
+    L[1] = fmaf(M[2], D[5], L[1]);
+    L[3] = fmaf(M[0], D[3], L[3]);
+    L[1] = fmaf(M[1], D[4], L[1]);
+    L[3] = fmaf(M[3], D[9], L[3]);
+    L[2] = fmaf(M[0], D[2], L[2]);
+    L[3] = fmaf(M[2], D[8], L[3]);
+    L[2] = fmaf(M[3], D[8], L[2]);
+    L[3] = fmaf(M[1], D[6], L[3]);
+    g[0] -= L[1];
+    g[1] -= L[2];
+    g[2] -= L[3];
+    *phi += L[0];
+}
+
+/* One PC interaction (tests).  M = {mass, -dipole}; out = {gx, gy, gz, phi}. */
+ORC_API void orc_pc_pair(const orc_params* p, float dx, float dy, float dz, const float* M, float* out) {
+    pp_consts c = make_pp_consts(p);
+    float g[3] = {0.f, 0.f, 0.f}, ph = 0.f;
+    pc_fp32(&c, dx, dy, dz, M, g, &ph);
+    out[0] = g[0];
+    out[1] = g[1];
+    out[2] = g[2];
+    out[3] = ph;
+}
+
+typedef struct {
+    int32_t *check, *next, *tmp, *multi, *part;
+    int64_t cap;
+} walk_lists;
+
+static void lists_reserve(walk_lists* w, int64_t need) {
+    if (need <= w->cap) return;
+    w->cap = need;
+    int32_t** all[5] = {&w->check, &w->next, &w->tmp, &w->multi, &w->part};
+    for (int k = 0; k < 5; k++) *all[k] = realloc(*all[k], sizeof(int32_t) * (size_t)need);
+}
+
+/* Short-range sum of kick_treepm_kernel (kick_treepm.cu:586-713) for every
+ * active sink (rung >= min_rung): per sink bucket, for each of the 27
+ * neighbour trees (x outermost, kick_treepm.cu:930-940), a breadth-first walk
+ * that drops nodes beyond rcut of the bucket (cutoff), takes nodes far from
+ * the bucket (R > (r_src + r_snk) / theta) as multipoles, opens the others,
+ * and sends each near leaf to PC when every active sink is beyond
+ * r_src / theta of it and to PP otherwise; then PC (compute_pc_interaction)
+ * over the multipole list and PP (compute_pp_interaction) over the leaves'
+ * particles.  Unit-mass sums in input order (inactive sinks get 0).
+ * stats = {PC interactions, PP pairs, multipole-list nodes, PP-list leaves}.
+ * pc_exact != 0 (a check, not the reference): nodes of the multipole list
+ * are summed particle by particle with the pair law instead.  Returns 0, or
+ * 1 when a tree exceeds the depth limit. */
+ORC_API int orc_tree_sum(const orc_params* p, int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z,
+                         const int64_t* cell_begin, const int8_t* rung, int min_rung, int pc_exact, float* gx,
+                         float* gy, float* gz, float* pot, double* stats) {
+    orc_forest f;
+    forest_build(p, n, x, y, z, cell_begin, &f);
+    for (int64_t i = 0; i < n; i++) gx[i] = gy[i] = gz[i] = pot[i] = 0.f;
+    stats[0] = stats[1] = stats[2] = stats[3] = 0.0;
+    if (f.err) {
+        forest_free(&f);
+        return 1;
+    }
+    const int C = p->chain_dim;
+    const pp_consts c = make_pp_consts(p);
+    const float rcut = (float)p->rcut;
+    const float theta2inv = 1.0f / (ORC_TREE_THETA * ORC_TREE_THETA);
+    int64_t maxnodes = 1;
+    for (int64_t q = 0; q < f.ncell; q++)
+        if (f.node_begin[q + 1] - f.node_begin[q] > maxnodes) maxnodes = f.node_begin[q + 1] - f.node_begin[q];
+    double npc = 0, npp = 0, nmulti = 0, npart = 0;
+    const uint32_t* const* P = (const uint32_t* const*)f.pos;
+#pragma omp parallel reduction(+ : npc, npp, nmulti, npart)
+    {
+        walk_lists w;
+        memset(&w, 0, sizeof(w));
+        lists_reserve(&w, 2 * maxnodes + 2);
+        int32_t act[ORC_TREE_BUCKET];
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t cell = 0; cell < f.ncell; cell++) {
+            const int cx = (int)(cell % C), cy = (int)((cell / C) % C), cz = (int)(cell / ((int64_t)C * C));
+            for (int64_t bi = f.bucket_begin[cell]; bi < f.bucket_begin[cell + 1]; bi++) {
+                const orc_tree_bucket* bk = &f.buckets[bi];
+                int na = 0;
+                for (int i = bk->begin; i < bk->end; i++)
+                    if (rung[f.idx[i]] >= min_rung) act[na++] = i;
+                if (na == 0) continue;
+                float g[ORC_TREE_BUCKET][3], ph[ORC_TREE_BUCKET];
+                for (int s = 0; s < na; s++) g[s][0] = g[s][1] = g[s][2] = ph[s] = 0.f;
+                for (int ox = -1; ox <= 1; ox++)
+                    for (int oy = -1; oy <= 1; oy++)
+                        for (int oz = -1; oz <= 1; oz++) {
+                            const int jx = (cx + ox + C) % C, jy = (cy + oy + C) % C, jz = (cz + oz + C) % C;
+                            const int64_t tc = ((int64_t)jz * C + jy) * C + jx;
+                            const orc_tree_node* tr = f.nodes + f.node_begin[tc];
+                            int ncheck = 1, nmul = 0, npar = 0;
+                            w.check[0] = 0;
+                            while (ncheck) {
+                                int nnext = 0, ntmp = 0;
+                                for (int ci = 0; ci < ncheck; ci++) {
+                                    const orc_tree_node* nd = &tr[w.check[ci]];
+                                    const float dx = orc_distance(bk->x[0], nd->x[0]);
+                                    const float dy = orc_distance(bk->x[1], nd->x[1]);
+                                    const float dz = orc_distance(bk->x[2], nd->x[2]);
+                                    const float R2 = sqr3_f(dx, dy, dz);
+                                    const float R = sqrtf(R2);
+                                    const int cutoff = R - nd->radius - bk->radius > rcut;
+                                    const float rr = nd->radius + bk->radius;
+                                    const int far = R2 > rr * rr * theta2inv;
+                                    const int leaf = nd->end - nd->begin <= ORC_TREE_LEAF;
+                                    if (cutoff) continue;
+                                    if (far)
+                                        w.multi[nmul++] = w.check[ci];
+                                    else if (leaf)
+                                        w.tmp[ntmp++] = w.check[ci];
+                                    else {
+                                        w.next[nnext++] = nd->child[0];
+                                        w.next[nnext++] = nd->child[1];
+                                    }
+                                }
+                                int32_t* sw = w.check;
+                                w.check = w.next;
+                                w.next = sw;
+                                ncheck = nnext;
+                                for (int k = 0; k < ntmp; k++) {
+                                    const orc_tree_node* nd = &tr[w.tmp[k]];
+                                    const float lim = nd->radius * nd->radius * theta2inv;
+                                    int near = 0;
+                                    for (int s = 0; s < na && !near; s++) {
+                                        const int i = act[s];
+                                        const float dx = orc_distance(P[0][i], nd->x[0]);
+                                        const float dy = orc_distance(P[1][i], nd->x[1]);
+                                        const float dz = orc_distance(P[2][i], nd->x[2]);
+                                        near = !(sqr3_f(dx, dy, dz) > lim);
+                                    }
+                                    if (near)
+                                        w.part[npar++] = w.tmp[k];
+                                    else
+                                        w.multi[nmul++] = w.tmp[k];
+                                }
+                            }
+                            nmulti += nmul;
+                            npart += npar;
+                            for (int s = 0; s < na; s++) {
+                                const int i = act[s];
+                                for (int k = 0; k < nmul; k++) {
+                                    const orc_tree_node* nd = &tr[w.multi[k]];
+                                    if (pc_exact) {
+                                        for (int j = nd->begin; j < nd->end; j++)
+                                            pp_fp32(&c, orc_distance(P[0][i], P[0][j]), orc_distance(P[1][i], P[1][j]),
+                                                    orc_distance(P[2][i], P[2][j]), g[s], &ph[s]);
+                                    } else {
+                                        pc_fp32(&c, orc_distance(P[0][i], nd->x[0]), orc_distance(P[1][i], nd->x[1]),
+                                                orc_distance(P[2][i], nd->x[2]), nd->m, g[s], &ph[s]);
+                                    }
+                                }
+                                for (int k = 0; k < npar; k++) {
+                                    const orc_tree_node* nd = &tr[w.part[k]];
+                                    for (int j = nd->begin; j < nd->end; j++)
+                                        pp_fp32(&c, orc_distance(P[0][i], P[0][j]), orc_distance(P[1][i], P[1][j]),
+                                                orc_distance(P[2][i], P[2][j]), g[s], &ph[s]);
+                                    npp += nd->end - nd->begin;
+                                }
+                                npc += nmul;
+                            }
+                        }
+                for (int s = 0; s < na; s++) {
+                    const int32_t o = f.idx[act[s]];
+                    gx[o] = g[s][0];
+                    gy[o] = g[s][1];
+                    gz[o] = g[s][2];
+                    pot[o] = ph[s];
+                }
+            }
+        }
+        free(w.check);
+        free(w.next);
+        free(w.tmp);
+        free(w.multi);
+        free(w.part);
+    }
+    stats[0] = npc;
+    stats[1] = npp;
+    stats[2] = nmulti;
+    stats[3] = npart;
+    forest_free(&f);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Drift step of the batched serving loop (no reference counterpart: the      */
 /* reference only kicks).  x <- x + v t0 in fixed32: the displacement v t0 in */
 /* box units, formed in double, is rounded to the nearest raw unit (llrint,   */
@@ -975,7 +1559,9 @@ ORC_API void orc_kick_epilogue(const orc_params* p, int64_t n, float* gx, float*
 /* Sorts the particle arrays in place into chain-mesh order, runs the PM
  * solve, PP + interpolation and the kick epilogue.  Outputs are in sorted
  * order.  cell_begin receives C^3 + 1 entries.  times[0..4] = seconds in
- * chain mesh, deposit, FFT+Green, PP+interp, epilogue. */
+ * chain mesh, deposit, FFT+Green, PP+interp, epilogue.  fp64_pp: 0 the FP32
+ * direct 27-cell sum, 1 the same in FP64, 2 kick_treepm's tree walk with
+ * multipoles (times[3] = -1 when a tree exceeds the depth limit). */
 ORC_API void orc_kick_full(const orc_params* p, int64_t n, uint32_t* x, uint32_t* y, uint32_t* z, float* vx,
                            float* vy, float* vz, int8_t* rung, float* gx, float* gy, float* gz, float* pot,
                            int64_t* cell_begin, int min_rung, double scale, double t0, int first_call,
@@ -1033,10 +1619,16 @@ ORC_API void orc_kick_full(const orc_params* p, int64_t n, uint32_t* x, uint32_t
     free(rho1);
     times[2] = omp_get_wtime() - t;
     t = omp_get_wtime();
-    int64_t* sinks = malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
-    for (int64_t i = 0; i < n; i++) sinks[i] = i;
-    orc_pp_sum(p, n, x, y, z, cell_begin, n, sinks, fp64_pp, gx, gy, gz, pot);
-    free(sinks);
+    int tree_err = 0;
+    if (fp64_pp == 2) {
+        double st[4];
+        tree_err = orc_tree_sum(p, n, x, y, z, cell_begin, rung, min_rung, 0, gx, gy, gz, pot, st);
+    } else {
+        int64_t* sinks = malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+        for (int64_t i = 0; i < n; i++) sinks[i] = i;
+        orc_pp_sum(p, n, x, y, z, cell_begin, n, sinks, fp64_pp, gx, gy, gz, pot);
+        free(sinks);
+    }
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         float g[3], pp;
@@ -1047,7 +1639,7 @@ ORC_API void orc_kick_full(const orc_params* p, int64_t n, uint32_t* x, uint32_t
         pot[i] += pp;
     }
     free(phi);
-    times[3] = omp_get_wtime() - t;
+    times[3] = tree_err ? -1.0 : omp_get_wtime() - t;
     t = omp_get_wtime();
     orc_kick_epilogue(p, n, gx, gy, gz, pot, vx, vy, vz, rung, min_rung, scale, t0, first_call);
     times[4] = omp_get_wtime() - t;

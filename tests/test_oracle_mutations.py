@@ -1,5 +1,5 @@
 """Plants plausible mistakes in a copy of the CPU oracle and checks that the
-oracle tests catch every one (the pins are not retyped formulas).  CPU only:
+oracle tests (test_oracle.py, test_oracle_tree.py) catch every one (the pins are not retyped formulas).  CPU only:
 each mutant is compiled with gcc into a temporary directory and loaded
 through TPM_ORACLE_LIB in a child pytest process."""
 import os
@@ -32,6 +32,17 @@ MUTANTS = {
                           "        const float q2 = q;\n        rinv3 = +15.0f / 8.0f;"),
     "drift_truncated": ("x[i] += (uint32_t)(uint64_t)llrint((double)vx[i] * f);",
                         "x[i] += (uint32_t)(uint64_t)(int64_t)((double)vx[i] * f);"),
+    "tree_leaf_test_with_sink_radius": ("const float lim = nd->radius * nd->radius * theta2inv;",
+                                        "const float lim = (nd->radius + bk->radius) * (nd->radius + bk->radius) * "
+                                        "theta2inv;"),
+    "tree_dipole_sign": ("            m[1] += -dx[0];", "            m[1] += dx[0];"),
+    "tree_cutoff_radius_sign": ("const int cutoff = R - nd->radius - bk->radius > rcut;",
+                                "const int cutoff = R + nd->radius - bk->radius > rcut;"),
+    "pc_m2l_index": ("    L[2] = fmaf(M[2], D[7], L[2]);", "    L[2] = fmaf(M[2], D[8], L[2]);"),
+    "pc_d1_sign": ("const float d1 = fmaf(-1.f * d0, rinv, e0);", "const float d1 = fmaf(d0, rinv, e0);"),
+    "tree_leaf_radius_without_hsoft": ("        node.radius = (float)(sqrt(r2max) + t->hsoft);\n        node.child[0]",
+                                       "        node.radius = (float)(sqrt(r2max));\n        node.child[0]"),
+    "tree_split_last_longest_side": ("            if (be[d] - bb[d] > span) {", "            if (be[d] - bb[d] >= span) {"),
 }
 
 
@@ -46,7 +57,8 @@ def test_mutant_is_caught(name, tmp_path):
     subprocess.run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fPIC", "-fopenmp", "-shared", "-o", str(so),
                     str(c), "-lm"], check=True)
     env = dict(os.environ, TPM_ORACLE_LIB=str(so), OMP_NUM_THREADS="4")
-    r = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", "test_oracle.py"), "-q", "-x",
+    r = subprocess.run([sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", "test_oracle.py"),
+                        os.path.join(ROOT, "tests", "test_oracle_tree.py"), "-q", "-x",
                         "-m", "not gpu", "-p", "no:cacheprovider"], env=env, capture_output=True, text=True,
                        timeout=600, cwd=ROOT)
     assert r.returncode != 0, "mutant %s survived the oracle tests" % name
