@@ -55,6 +55,14 @@ typedef int tpm_status;
  * kick_treepm's -- every pair of the 27 cells with no cutoff test, plain
  * softened polynomials inside hsoft, phi starting at 0 (no phi0). */
 #define TPM_KICK_PME 16u
+/* tpm_kick flag: kick_treepm's tree walk (kick_treepm.cu:586-713) for the
+ * short-range term -- per chain cell a tree of <= 32-particle leaves
+ * (tree.cpp:7-245); per sink bucket of <= 128 particles, nodes beyond rcut
+ * are dropped, nodes with R > (r_src + r_snk) / 0.4 and leaves beyond
+ * r_src / 0.4 of every active sink act through their monopole + dipole
+ * (compute_pc_interaction, kick_treepm.cu:226-240), the other leaves pair by
+ * pair.  Builds the trees of the current chain mesh (tpm_tree_build). */
+#define TPM_KICK_TREE 32u
 
 /* Physical and numerical parameters.  Replaces the options struct
  * (options.hpp:5-24, derived in options.cpp:81-96) and the scalars that
@@ -91,6 +99,7 @@ typedef struct {
     int64_t n_tiles;
     int max_rung;
     int pad_;
+    double pc_interactions;  /* TPM_KICK_TREE: sink x multipole interactions */
 } tpm_stats;
 
 /* Launch-shape tuning (no reference counterpart: the reference fixes 32-thread
@@ -160,6 +169,42 @@ TPM_API tpm_status tpm_gravity_long_get_green(tpm_ctx* ctx, float* table);
  * stats may be NULL. */
 TPM_API tpm_status tpm_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                     tpm_stats* stats);
+
+/* Per-cell trees (tree_node / sink_bucket, tree.hpp:17-36).  Nodes of one
+ * cell are stored in preorder (a node before its subtrees, child 0 first);
+ * children are indices within the cell's own tree.  Particle ranges index
+ * the tree order, a permutation of the chain-mesh order inside each cell. */
+typedef struct {
+    uint32_t x[3];      /* expansion centre, fixed32 */
+    float radius;       /* bounds the node's particles, plus hsoft */
+    int32_t child[2];   /* -1 for a leaf */
+    int32_t begin, end; /* particle range, tree order */
+    float m[4];         /* particle count and -(sum of offsets from the centre) */
+} tpm_tree_node;
+
+typedef struct {
+    int32_t begin, end; /* sink range, tree order */
+    uint32_t x[3];      /* centre of the sinks' bounding box */
+    float radius;
+    int32_t cell;       /* chain cell, z-major */
+    int32_t pad_;
+} tpm_tree_bucket;
+
+/* tree_create (tree.cpp:216-245) for every chain cell, as the loop of
+ * kick_treepm_begin (kick_treepm.cpp:26-44) does, on the device: each cell's
+ * particles are split at the midpoint of the longest side of its box until
+ * <= 32 remain (x < the fixed32 midpoint goes left, each side keeps its
+ * order); sink buckets are the first nodes of <= 128 particles.  Needs
+ * tpm_chainmesh_create.  A tree deeper than 50 levels is TPM_ERR_INVALID
+ * (the reference aborts, tree.cpp:16-19). */
+TPM_API tpm_status tpm_tree_build(tpm_ctx* ctx);
+/* Sizes of the built trees (any pointer may be NULL). */
+TPM_API tpm_status tpm_tree_info(tpm_ctx* ctx, int64_t* n_nodes, int64_t* n_buckets);
+/* Copies the trees to the host: node_begin and bucket_begin get C^3 + 1
+ * offsets, order[i] = chain-mesh index of tree position i.  Any pointer may
+ * be NULL. */
+TPM_API tpm_status tpm_tree_download(tpm_ctx* ctx, int64_t* node_begin, int64_t* bucket_begin, tpm_tree_node* nodes,
+                                     tpm_tree_bucket* buckets, int32_t* order);
 
 /* One whole step: chain mesh + PM + kick (kick_treepm_test without IC and
  * comparison, test.cpp:237-290). */

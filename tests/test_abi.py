@@ -6,6 +6,7 @@ import sys
 
 import pytest
 
+import oracle as O
 import tigerpm_amd as T
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -94,6 +95,29 @@ def test_struct_layouts_match_header(tmp_path):
         assert int(got[c]) == ctypes.sizeof(py), c
         for f, _ in py._fields_:
             assert int(got["%s.%s" % (c, f)]) == getattr(py, f).offset, (c, f)
+
+
+def test_tree_record_layouts_match_header(tmp_path):
+    """The numpy records of the tree download equal tpm_tree_node /
+    tpm_tree_bucket as gcc lays them out, field by field."""
+    recs = {"tpm_tree_node": T.TREE_NODE, "tpm_tree_bucket": T.TREE_BUCKET}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tigerpm_hip.h"', "int main(void) {"]
+    for c, dt in recs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (c, c))
+        for f in dt.names:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (c, f, c, f))
+    lines.append("return 0; }")
+    src = tmp_path / "t.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "t"
+    subprocess.run(["gcc", "-std=c11", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(ln.split() for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                    check=True).stdout.splitlines())
+    for c, dt in recs.items():
+        assert int(got[c]) == dt.itemsize, c
+        for f in dt.names:
+            assert int(got["%s.%s" % (c, f)]) == dt.fields[f][1], (c, f)
+    assert T.TREE_NODE.itemsize == O.TREE_NODE.itemsize == 48
 
 
 def test_batch_without_gpu_fails_cleanly():

@@ -1,0 +1,150 @@
+"""GPU parity of the tree path (TPM_KICK_TREE, tpm_tree_*): the per-cell
+trees bit-identical to the oracle's; the walk's decisions identical (the
+same number of multipole interactions and pairs); forces and the whole kick
+within FP32 tolerance of the oracle's restatement of kick_treepm."""
+import numpy as np
+import pytest
+
+import oracle as O
+import tigerpm_amd as T
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.sqrt(((a - b) ** 2).sum() / max((b ** 2).sum(), 1e-300)))
+
+
+def positions(kind, n, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "uniform":
+        return [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    c = rng.random((12, 3))
+    k = rng.integers(0, 12, n)
+    pos = (c[k] + rng.standard_normal((n, 3)) * 0.03) % 1.0
+    return [np.minimum(np.floor(pos[:, d] * 2.0 ** 32), 2.0 ** 32 - 1).astype(np.uint32) for d in range(3)]
+
+
+def oparams(p):
+    return O.make_params(p.mesh_dim, p.chain_dim, p.order, p.rs, p.rcut, p.hsoft, p.GM, p.eta, p.n_total, p.do_phi,
+                         p.interlace)
+
+
+def setup(ctx, kind, n_side, seed, params=None):
+    n = n_side ** 3
+    x, y, z = positions(kind, n, seed)
+    p = ctx.set_params(params if params is not None else T.baseline_params(n_side, n_side, 4))
+    ctx.upload(x, y, z)
+    ctx.chainmesh_create()
+    operm, ocb = O.chainmesh_sort(x, y, z, p.chain_dim)
+    assert np.array_equal(ctx.chainmesh_perm(), operm)
+    return p, x[operm], y[operm], z[operm], ocb
+
+
+@pytest.mark.parametrize("kind,n_side", [("uniform", 24), ("clustered", 24), ("clustered", 40)])
+def test_trees_bit_identical(gpu_ctx, kind, n_side):
+    p, xs, ys, zs, cb = setup(gpu_ctx, kind, n_side, 3)
+    print("trees", kind, n_side, flush=True)
+    gpu_ctx.tree_build()
+    g = gpu_ctx.tree_download()
+    o = O.tree_build(oparams(p), xs, ys, zs, cb)
+    assert np.array_equal(g["order"], o["order"])
+    assert np.array_equal(g["node_begin"], o["node_begin"])
+    assert np.array_equal(g["bucket_begin"], o["bucket_begin"])
+    assert g["nodes"].tobytes() == o["nodes"].tobytes()
+    for f in ("begin", "end", "x", "radius"):
+        assert np.array_equal(g["buckets"][f], o["buckets"][f]), f
+    cells = np.repeat(np.arange(p.chain_dim ** 3), np.diff(o["bucket_begin"]))
+    assert np.array_equal(g["buckets"]["cell"], cells)
+
+
+@pytest.mark.parametrize("kind,n_side,min_rung", [("uniform", 32, 0), ("clustered", 32, 1), ("clustered", 24, 0)])
+def test_walk_matches_oracle(gpu_ctx, kind, n_side, min_rung):
+    """Short range alone (no mesh, no kick): the same decisions as the
+    oracle's walk (PC and PP counts equal) and the same sums to FP32
+    rounding (the GPU's rsqrt and summation order)."""
+    p, xs, ys, zs, cb = setup(gpu_ctx, kind, n_side, 5)
+    n = len(xs)
+    rung = np.random.default_rng(4).integers(0, 3, n).astype(np.int8)
+    gpu_ctx.upload(xs, ys, zs, rung=rung)  # already in chain-mesh order: the sort keeps it
+    gpu_ctx.chainmesh_create()
+    print("walk", kind, n_side, min_rung, flush=True)
+    st = gpu_ctx.kick(min_rung=min_rung, flags=T.TPM_KICK_TREE | T.TPM_SKIP_PM | T.TPM_SKIP_KICK)
+    d = gpu_ctx.download()
+    (ox, oy, oz, oph), ost = O.tree_sum(oparams(p), xs, ys, zs, cb, rung, min_rung)
+    assert st.pc_interactions == ost["pc"] and st.executed_pairs == ost["pp"]
+    act = rung >= min_rung
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1)[act] / p.GM
+    OG = np.stack([ox, oy, oz], 1)[act]
+    assert rel_rms(G, OG) < 1e-5
+    phi0 = p.n_total * 4.0 * np.pi * p.rs ** 2 + (35.0 / 16.0) / p.hsoft
+    assert rel_rms(d["pot"][act] / p.GM - phi0, oph[act]) < 1e-5
+    assert ost["pc"] > 0
+
+
+def test_tree_kick_matches_oracle(gpu_ctx):
+    """The whole kick (mesh + tree walk + kick epilogue, mixed rungs)."""
+    n_side = 32
+    p, xs, ys, zs, cb = setup(gpu_ctx, "clustered", n_side, 9)
+    n = len(xs)
+    rng = np.random.default_rng(2)
+    vx, vy, vz = [rng.standard_normal(n).astype(np.float32) * 1e-3 for _ in range(3)]
+    rung = rng.integers(0, 3, n).astype(np.int8)
+    gpu_ctx.upload(xs, ys, zs, vx, vy, vz, rung)
+    gpu_ctx.chainmesh_create()
+    gpu_ctx.gravity_long_compute()
+    print("tree kick", flush=True)
+    gpu_ctx.kick(min_rung=1, scale=1.0, t0=0.5, first_call=False, flags=T.TPM_KICK_TREE)
+    d = gpu_ctx.download()
+    r = O.kick_full(oparams(p), xs, ys, zs, vx, vy, vz, rung, min_rung=1, scale=1.0, t0=0.5, first_call=False,
+                    tree=True)
+    assert np.array_equal(d["x"], r["x"])
+    act = rung >= 1
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1)[act]
+    OG = np.stack([r["gx"], r["gy"], r["gz"]], 1)[act]
+    assert rel_rms(G, OG) < 1e-5
+    assert rel_rms(d["pot"][act], r["pot"][act]) < 1e-5
+    V = np.stack([d["vx"], d["vy"], d["vz"]], 1)
+    OV = np.stack([r["vx"], r["vy"], r["vz"]], 1)
+    assert rel_rms(V[act] - np.stack([vx, vy, vz], 1)[act], OV[act] - np.stack([vx, vy, vz], 1)[act]) < 1e-4
+    assert np.array_equal(V[~act], np.stack([vx, vy, vz], 1)[~act])
+    assert np.mean(d["rung"][act] == r["rung"][act]) > 0.999
+
+
+def test_tree_and_direct_kicks_differ_by_the_multipole_error(gpu_ctx):
+    """The tree kick and the direct kick on the same particles differ by the
+    multipole approximation only: by what the oracle's two paths differ."""
+    n_side = 32
+    p, xs, ys, zs, cb = setup(gpu_ctx, "uniform", n_side, 13)
+    gpu_ctx.gravity_long_compute()
+    print("tree vs direct", flush=True)
+    gpu_ctx.kick(flags=T.TPM_KICK_TREE | T.TPM_SKIP_KICK)
+    a = gpu_ctx.download()
+    gpu_ctx.kick(flags=T.TPM_SKIP_KICK)
+    b = gpu_ctx.download()
+    Ga = np.stack([a["gx"], a["gy"], a["gz"]], 1)
+    Gb = np.stack([b["gx"], b["gy"], b["gz"]], 1)
+    n = len(xs)
+    zf = np.zeros(n, np.float32)
+    z8 = np.zeros(n, np.int8)
+    ot = O.kick_full(oparams(p), xs, ys, zs, zf, zf, zf, z8, tree=True)
+    od = O.kick_full(oparams(p), xs, ys, zs, zf, zf, zf, z8)
+    Ot = np.stack([ot["gx"], ot["gy"], ot["gz"]], 1)
+    Od = np.stack([od["gx"], od["gy"], od["gz"]], 1)
+    gpu_diff, orc_diff = rel_rms(Ga, Gb), rel_rms(Ot, Od)
+    assert orc_diff > 1e-4
+    assert abs(gpu_diff - orc_diff) < 0.02 * orc_diff
+
+
+def test_depth_limit_is_an_error(gpu_ctx):
+    n_side = 24
+    n = n_side ** 3
+    x, y, z = positions("uniform", n, 1)
+    x[:40], y[:40], z[:40] = 123456789, 987654321, 555555555
+    gpu_ctx.set_params(T.baseline_params(n_side, n_side, 4))
+    gpu_ctx.upload(x, y, z)
+    gpu_ctx.chainmesh_create()
+    with pytest.raises(T.TpmError, match="depth"):
+        gpu_ctx.tree_build()

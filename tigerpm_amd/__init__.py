@@ -6,7 +6,8 @@ without the library, but creating a Context raises if the HIP library is
 missing or no GPU is visible.
 """
 from ._lib import (  # noqa: F401
-    TPM_CIC, TPM_TSC, TPM_CLOUD4, TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK, TPM_KICK_PME,
+    TPM_CIC, TPM_TSC, TPM_CLOUD4, TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK, TPM_KICK_PME, TPM_KICK_TREE,
+    TREE_NODE, TREE_BUCKET,
     Context, TpmError, TpmParams, TpmStats, TpmTuning, lib, lib_path, exported_symbols, DECLARED_SYMBOLS,
     TPM_BATCH_DRIFT, Batch, TpmBatchLimits, TpmBoxSpec, TpmBatchTuning, TpmBatchStats, box_spec, batch_arena_bytes,
 )

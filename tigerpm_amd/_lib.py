@@ -13,6 +13,13 @@ _HEADER = os.path.join(_ROOT, "include", "tigerpm_hip.h")
 TPM_CIC, TPM_TSC, TPM_CLOUD4 = 2, 3, 4
 TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK = 1, 2, 4
 TPM_KICK_PME = 16  # kick_pme's pair law (no cutoff, plain softening, no phi0)
+TPM_KICK_TREE = 32  # kick_treepm's tree walk: per-cell trees, theta = 0.4 multipoles + PP
+
+# tpm_tree_node / tpm_tree_bucket (include/tigerpm_hip.h)
+TREE_NODE = np.dtype([("x", "<u4", 3), ("radius", "<f4"), ("child", "<i4", 2), ("begin", "<i4"), ("end", "<i4"),
+                      ("m", "<f4", 4)])
+TREE_BUCKET = np.dtype([("begin", "<i4"), ("end", "<i4"), ("x", "<u4", 3), ("radius", "<f4"), ("cell", "<i4"),
+                        ("pad_", "<i4")])
 
 _STATUS = {0: "ok", -1: "HIP error", -2: "invalid argument", -3: "call out of order", -4: "rocFFT error",
            -5: "device memory budget exceeded"}
@@ -39,7 +46,7 @@ class TpmStats(ctypes.Structure):
     _fields_ = [("candidate_pairs", ctypes.c_double), ("executed_pairs", ctypes.c_double),
                 ("ms_chainmesh", ctypes.c_double), ("ms_pm", ctypes.c_double), ("ms_pp", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("n_active", ctypes.c_int64), ("n_tiles", ctypes.c_int64),
-                ("max_rung", ctypes.c_int), ("pad_", ctypes.c_int)]
+                ("max_rung", ctypes.c_int), ("pad_", ctypes.c_int), ("pc_interactions", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
@@ -146,6 +153,9 @@ def lib():
         "tpm_gravity_long_get_green": (_int, [vp, _f32p]),
         "tpm_kick": (_int, [vp, _int, _dbl, _dbl, _int, ctypes.c_uint, ctypes.POINTER(TpmStats)]),
         "tpm_kick_treepm": (_int, [vp, _int, _dbl, _dbl, _int, ctypes.POINTER(TpmStats)]),
+        "tpm_tree_build": (_int, [vp]),
+        "tpm_tree_info": (_int, [vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+        "tpm_tree_download": (_int, [vp, vp, vp, vp, vp, vp]),
         "tpm_ewald": (_int, [vp, _i64, _i64p, _dbl, _f64p, _f64p, _f64p, _f64p]),
         "tpm_ic_zeldovich": (_int, [vp, _int, ctypes.c_uint64, _dbl, _dbl]),
         "tpm_ic_uniform": (_int, [vp, _i64, ctypes.c_uint64]),
@@ -300,6 +310,22 @@ class Context:
         self._chk(self._L.tpm_kick(self._h, int(min_rung), float(scale), float(t0), int(bool(first_call)),
                                    int(flags), ctypes.byref(st)))
         return st
+
+    def tree_build(self):
+        self._chk(self._L.tpm_tree_build(self._h))
+
+    def tree_download(self):
+        """The per-cell trees: order (tree position -> chain-mesh index),
+        node_begin, bucket_begin, nodes, buckets (as the oracle's tree_build)."""
+        nn, nb = _i64(), _i64()
+        self._chk(self._L.tpm_tree_info(self._h, ctypes.byref(nn), ctypes.byref(nb)))
+        C3 = self.params.chain_dim ** 3
+        out = dict(node_begin=np.empty(C3 + 1, np.int64), bucket_begin=np.empty(C3 + 1, np.int64),
+                   nodes=np.empty(nn.value, TREE_NODE), buckets=np.empty(nb.value, TREE_BUCKET),
+                   order=np.empty(self.size(), np.int32))
+        self._chk(self._L.tpm_tree_download(self._h, *[_ptr(out[k]) for k in ("node_begin", "bucket_begin", "nodes",
+                                                                             "buckets", "order")]))
+        return out
 
     def kick_treepm(self, min_rung=0, scale=1.0, t0=1.0, first_call=True, stats=True):
         st = TpmStats()

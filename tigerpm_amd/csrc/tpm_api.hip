@@ -38,7 +38,9 @@ extern "C" tpm_status tpm_ctx_destroy(tpm_ctx* ctx) {
                     ctx->vx2,   ctx->vy2,     ctx->vz2,      ctx->rung2,     ctx->keys,     ctx->keys2, ctx->idx,
                     ctx->perm,  ctx->sort_tmp, ctx->bounds,  ctx->cell_begin, ctx->tile_cnt, ctx->tile_end,
                     ctx->tiles, ctx->ntiles,  ctx->scan_tmp, ctx->rho_q,     ctx->rho,      ctx->rho_hat, ctx->rho_hat1,
-                    ctx->phi,   ctx->green,   ctx->counters, ctx->dscratch};
+                    ctx->phi,   ctx->green,   ctx->counters, ctx->dscratch, ctx->tree_pos, ctx->tree_tmp,
+                    ctx->tree_cnt, ctx->node_begin, ctx->bucket_begin, ctx->tree_part, ctx->tree_flags, ctx->nodes,
+                    ctx->buckets};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ctx->ev)
@@ -291,11 +293,16 @@ extern "C" tpm_status tpm_gravity_long_get_green(tpm_ctx* ctx, float* table) {
 extern "C" tpm_status tpm_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                                tpm_stats* stats) {
     if (!ctx) return TPM_ERR_INVALID;
-    if (flags & ~(TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK | TPM_KICK_PME))
+    if (flags & ~(TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK | TPM_KICK_PME | TPM_KICK_TREE))
         return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: unknown flags");
+    if ((flags & TPM_KICK_PME) && (flags & TPM_KICK_TREE))
+        return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: TPM_KICK_PME and TPM_KICK_TREE exclude each other");
     (void)hipSetDevice(ctx->device);
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    TPM_TRY(tpm_pp_kick(ctx, min_rung, scale, t0, first_call, flags, stats));
+    if (flags & TPM_KICK_TREE)
+        TPM_TRY(tpm_tree_kick(ctx, min_rung, scale, t0, first_call, flags, stats));
+    else
+        TPM_TRY(tpm_pp_kick(ctx, min_rung, scale, t0, first_call, flags, stats));
     if (stats) stats->ms_total = stats->ms_pp;
     return TPM_OK;
 }

@@ -133,6 +133,7 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
         ctx->chain_valid = true;
         ctx->chain_C = C;
         ctx->chain_tile = ts;
+        ctx->chain_epoch++;
         return TPM_OK;
     }
 
@@ -189,6 +190,7 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
     ctx->chain_valid = true;
     ctx->chain_C = C;
     ctx->chain_tile = ts;
+    ctx->chain_epoch++;
     return TPM_OK;
 }
 
@@ -423,9 +425,10 @@ __global__ void btiles_write_kernel(int32_t ncell, int ts, int nbox, const tpm_b
 }  // namespace
 
 // Exclusive scan of in[0, n) into out[0, n] (out[n] = total) with the
-// three kernels above; part holds ceil(n / SCAN_CHUNK) ints.
-static tpm_status scan_i32(std::string& err, hipStream_t s, const int32_t* in, int64_t n, int32_t* part,
-                           int32_t* out) {
+// three kernels above; part holds ceil(n / 2048) ints.
+tpm_status tpm_scan_i32(std::string& err, hipStream_t s, const int32_t* in, int64_t n, int32_t* part,
+                        int32_t* out) {
+    static_assert(SCAN_CHUNK == 2048, "part size documented as n / 2048");
     const int64_t nchunk = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
     if (nchunk < 1 || nchunk > INT32_MAX) {
         err = "scan: bad size";
@@ -449,7 +452,7 @@ tpm_status tpm_batch_chainmesh(tpm_batch* bt, const tpm_batch_layout& L) {
         bkey_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, L.nbox, bt->d_boxes, ncell, bt->x, bt->y, bt->z, bt->key,
                                                      bt->slot, bt->cnt);
     TPM_HIP(bt, hipGetLastError());
-    TPM_TRY(scan_i32(bt->err, s, bt->cnt, ncell + 1, bt->scan_part, bt->cell_begin));
+    TPM_TRY(tpm_scan_i32(bt->err, s, bt->cnt, ncell + 1, bt->scan_part, bt->cell_begin));
     if (n > 0) {
         bscatter_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, bt->key, bt->slot, bt->cell_begin, bt->tmp);
         const int64_t waves = std::min<int64_t>(std::max<int32_t>(ncell, 1), 65536);
@@ -474,7 +477,7 @@ tpm_status tpm_batch_chainmesh(tpm_batch* bt, const tpm_batch_layout& L) {
     if (ncell > 0) {
         btiles_count_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ts, bt->cell_begin, bt->tile_cnt);
         TPM_HIP(bt, hipGetLastError());
-        TPM_TRY(scan_i32(bt->err, s, bt->tile_cnt, ncell, bt->scan_part, bt->tile_off));
+        TPM_TRY(tpm_scan_i32(bt->err, s, bt->tile_cnt, ncell, bt->scan_part, bt->tile_off));
         btiles_write_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ts, L.nbox, bt->d_boxes, bt->cell_begin,
                                                                  bt->tile_off, bt->tiles);
     } else {

@@ -110,6 +110,24 @@ struct tpm_ctx {
     int tile = 16;        // sinks per PP tile: 16, 32 or 64 (tuning)
     int chain_tile = 0;   // tile size the current tile list was built with
     int deposit_block = 0;  // chain cells per deposit block edge; 0 auto, -1 per-particle atomics
+    int64_t chain_epoch = 0;  // counts chain-mesh builds (trees remember the one they follow)
+
+    // Per-cell trees (tpm_tree.hip).
+    int4* tree_pos = nullptr;        // tree order: x, y, z, chain-mesh index
+    int4* tree_tmp = nullptr;        // partition ping-pong buffer
+    int64_t tree_pos_cap = 0;
+    int32_t* tree_cnt = nullptr;     // per cell: node counts [0, ncell), bucket counts [ncell, 2 ncell)
+    int32_t* node_begin = nullptr;   // ncell + 1 exclusive offsets
+    int32_t* bucket_begin = nullptr; // ncell + 1
+    int32_t* tree_part = nullptr;    // scan partials
+    int32_t* tree_flags = nullptr;   // [0] depth exceeded, [1] walk stack overflow
+    int64_t tree_cells_cap = 0;
+    tpm_tree_node* nodes = nullptr;
+    int64_t nodes_cap = 0;
+    tpm_tree_bucket* buckets = nullptr;
+    int64_t buckets_cap = 0;
+    int64_t n_nodes = 0, n_buckets = 0;
+    int64_t tree_epoch = -1;         // chain_epoch the trees were built from
 
     // PM mesh.
     int64_t mesh_cap = 0;  // M^3 capacity
@@ -390,6 +408,17 @@ struct tpm_batch_layout {
 // Internal entry points implemented in the .hip files.
 tpm_status tpm_check_params(std::string& err, tpm_params* p);
 tpm_status tpm_chainmesh_build(tpm_ctx* ctx);
+// Exclusive scan of in[0, n) into out[0, n] (out[n] = total); part holds
+// n / 2048 + 1 ints.
+tpm_status tpm_scan_i32(std::string& err, hipStream_t s, const int32_t* in, int64_t n, int32_t* part, int32_t* out);
+// Adds W_sr (candidate pairs of the 27-cell stencil) of the context's chain
+// mesh to *dst (device).
+tpm_status tpm_candidate_pairs(tpm_ctx* ctx, unsigned long long* dst);
+// Trees of the current chain mesh (no-op when they are current).
+tpm_status tpm_tree_ensure(tpm_ctx* ctx);
+// Tree-walk short-range sum + mesh interpolation + kick (TPM_KICK_TREE).
+tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
+                         tpm_stats* stats);
 tpm_status tpm_pm_solve(tpm_ctx* ctx);
 tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                        tpm_stats* stats);

@@ -1,0 +1,938 @@
+// tpm_tree.hip -- per-cell trees and kick_treepm's tree walk with multipole
+// (PC) interactions (tree.cpp:7-245, kernels.hpp:13-152,
+// kick_treepm.cu:226-240, 460-759), on the device.
+//
+// Build, one wave64 per chain cell:
+//   tree_partition_kernel: the recursive midpoint split of tree.cpp's sort()
+//     as a depth-first walk over an LDS stack.  Each split is a stable
+//     wave-wide partition (ballot + mbcnt) of int4 records (x, y, z,
+//     chain-mesh index) from one buffer into the other, so the leaves end in
+//     tree order; the kernel counts the cell's nodes and sink buckets.
+//   Exclusive scans of the counts give each cell's node and bucket base.
+//   tree_nodes_kernel: the same walk over the partitioned records (split
+//     points by binary search), nodes numbered in preorder and written after
+//     their children: a leaf's centre, radius and moments from its particles
+//     (bounding box and farthest distance by wave reductions, exact in FP64;
+//     moment sums in particle order), an internal node from its two children
+//     as tree.cpp:101-171 does, and the sink buckets (tree.cpp:173-211).
+//   The arithmetic is the reference's, operation for operation, so the trees
+//   are bit-identical to the CPU restatement's.
+//
+// Walk, one wave64 per sink bucket (<= 128 sinks, two per lane), for each of
+// the 27 neighbour trees: nodes are taken 64 at a time from an LDS stack,
+// each lane tests one (cutoff: R - r_src - r_snk > rcut; far: R^2 >
+// (r_src + r_snk)^2 / theta^2); far nodes go to the multipole list, near
+// internal nodes push their children, near leaves go to the particle list
+// unless every active sink is beyond r_src / theta of them
+// (kick_treepm.cu:630-680).  The multipole list is applied with
+// compute_pc_interaction (greens_function + M2L, kernels.hpp:13-88); the
+// leaves' particles are staged in LDS and applied with the pair law of the
+// direct kernel.  The decisions do not depend on the visiting order, so they
+// are the reference's breadth-first walk's.  Then the mesh force and the
+// kick epilogue, as in the direct kernel.
+#include "tpm_pp_device.hpp"
+
+#include <vector>
+
+#define TREE_LEAF 32     // min(SOURCE_BUCKET_SIZE, SINK_BUCKET_SIZE), tigerpm.hpp:32-33
+#define TREE_BUCKET 128  // SINK_BUCKET_SIZE
+#define TREE_DEPTH 50    // MAX_DEPTH, tree.cpp:5
+#define TREE_STACK 64    // build frames: a depth-first walk holds <= depth + 2
+#define WALK_STACK 2048  // walk: pending nodes of one neighbour tree
+#define WALK_LIST 128    // multipole / leaf list entries between flushes
+#define WALK_SBUF 256    // staged PP sources
+
+namespace {
+
+// fixed32 from double (fixed.hpp:63-75): 2^32 x truncated; a centre just
+// outside [0, 1) wraps, which distance() treats periodically.
+__device__ __forceinline__ uint32_t tree_fixed(double x) { return (uint32_t)(int64_t)(4294967296.0 * x); }
+__device__ __forceinline__ double raw_to_double(uint32_t r) { return (double)r * 0x1p-32; }
+// sqr(a, b, c) of util.hpp:32-34 is fmaf(a, a, fmaf(b, b, c * c)); the
+// double instantiation tree.cpp uses rounds through FP32.
+__device__ __forceinline__ double sqr3_d(double a, double b, double c) {
+    return (double)fmaf((float)a, (float)a, fmaf((float)b, (float)b, (float)(c * c)));
+}
+__device__ __forceinline__ float sqr3_f(float a, float b, float c) { return fmaf(a, a, fmaf(b, b, c * c)); }
+__device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ double dmin(double a, double b) { return (b < a) ? b : a; }  // std::min
+__device__ __forceinline__ uint32_t comp(int4 v, int d) { return (uint32_t)(d == 0 ? v.x : (d == 1 ? v.y : v.z)); }
+// distance() (gravity_short.hpp:20-22) in box units.
+__device__ __forceinline__ float dist(uint32_t a, uint32_t b) { return (float)(int)(a - b) * 0x1p-32f; }
+
+__device__ __forceinline__ double wave_dmin(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = dmin(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ double wave_dmax(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = dmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// range::split (range.hpp:118-133): the first strictly longest side.
+__device__ __forceinline__ int split_dim(const double* b, const double* e) {
+    int dim = 0;
+    double span = 0.0;
+    for (int d = 0; d < 3; d++)
+        if (e[d] - b[d] > span) {
+            span = e[d] - b[d];
+            dim = d;
+        }
+    return dim;
+}
+
+struct PartFrame {
+    double b[3], e[3];
+    int begin, end, depth, sunk;
+};
+
+__global__ __launch_bounds__(64) void tree_partition_kernel(int C, const int32_t* __restrict__ cell_begin,
+                                                            const uint32_t* __restrict__ x,
+                                                            const uint32_t* __restrict__ y,
+                                                            const uint32_t* __restrict__ z, int4* __restrict__ pos,
+                                                            int4* __restrict__ tmp, int32_t* __restrict__ cnt,
+                                                            int32_t* __restrict__ flags) {
+    __shared__ PartFrame st[TREE_STACK];
+    const int c = blockIdx.x;
+    const int ncell = C * C * C;
+    const int lane = (int)lane_id();
+    const int b0 = cell_begin[c], e0 = cell_begin[c + 1];
+    for (int i = b0 + lane; i < e0; i += 64) pos[i] = make_int4((int)x[i], (int)y[i], (int)z[i], i);
+    if (lane == 0) {
+        // tree_create (tree.cpp:216-245): the cell's box [i / C, (i + 1) / C)
+        const double ninv = 1.0 / (double)C;
+        const int ic[3] = {c % C, (c / C) % C, c / (C * C)};
+        PartFrame f;
+        for (int d = 0; d < 3; d++) {
+            f.b[d] = ic[d] * ninv;
+            f.e[d] = (ic[d] + 1) * ninv;
+        }
+        f.begin = b0;
+        f.end = e0;
+        f.depth = 0;
+        f.sunk = 0;
+        st[0] = f;
+    }
+    __syncthreads();
+    int sp = 1, nodes = 0, buckets = 0;
+    bool bad = false;
+    while (sp > 0) {
+        const PartFrame f = st[--sp];
+        __syncthreads();
+        if (f.depth > TREE_DEPTH) {  // tree.cpp:16-19 aborts
+            bad = true;
+            break;
+        }
+        nodes++;
+        const int n = f.end - f.begin;
+        if (n <= TREE_BUCKET && !f.sunk) buckets++;
+        if (n <= TREE_LEAF) {
+            // a leaf at odd depth was last written to tmp
+            if (f.depth & 1)
+                for (int i = f.begin + lane; i < f.end; i += 64) pos[i] = tmp[i];
+            continue;
+        }
+        const int dim = split_dim(f.b, f.e);
+        const double mid = (f.e[dim] + f.b[dim]) / 2.0;
+        const uint32_t thr = tree_fixed(mid);
+        const int4* src = (f.depth & 1) ? tmp : pos;
+        int4* dst = (f.depth & 1) ? pos : tmp;
+        // particles_sort (particles.cpp:414-445): x < xmid goes left; here
+        // each side keeps its previous order (a stable partition)
+        int nl = 0;
+        for (int i0 = f.begin; i0 < f.end; i0 += 64) {
+            const int i = i0 + lane;
+            const bool left = i < f.end && comp(src[i], dim) < thr;
+            nl += __popcll(__ballot(left));
+        }
+        int l = 0, r = 0;
+        for (int i0 = f.begin; i0 < f.end; i0 += 64) {
+            const int i = i0 + lane;
+            const bool valid = i < f.end;
+            const int4 v = valid ? src[i] : make_int4(0, 0, 0, 0);
+            const bool left = valid && comp(v, dim) < thr;
+            const unsigned long long ml = __ballot(left), mr = __ballot(valid && !left);
+            if (valid) dst[left ? f.begin + l + (int)count_below(ml) : f.begin + nl + r + (int)count_below(mr)] = v;
+            l += __popcll(ml);
+            r += __popcll(mr);
+        }
+        if (sp + 2 > TREE_STACK) {  // unreachable below the depth limit
+            bad = true;
+            break;
+        }
+        if (lane == 0) {
+            PartFrame L = f, R = f;
+            L.e[dim] = mid;
+            R.b[dim] = mid;
+            L.end = R.begin = f.begin + nl;
+            L.depth = R.depth = f.depth + 1;
+            L.sunk = R.sunk = n <= TREE_BUCKET;
+            st[sp] = R;
+            st[sp + 1] = L;
+        }
+        sp += 2;
+        __syncthreads();
+    }
+    if (lane == 0) {
+        cnt[c] = nodes;
+        cnt[ncell + c] = buckets;
+        if (bad) atomicOr(&flags[0], 1);
+    }
+}
+
+// Centre of the bounding box of [begin, end) (the box centre when empty) and
+// the largest squared distance of a particle from it (tree.cpp:25-59,
+// 88-99); every lane returns the same values.
+__device__ double tree_bbox(const int4* __restrict__ pos, int begin, int end, const double* bb, const double* be,
+                            double* xc) {
+    const int lane = (int)lane_id();
+    double mn[3] = {1.0, 1.0, 1.0}, mx[3] = {0.0, 0.0, 0.0};
+    for (int i = begin + lane; i < end; i += 64) {
+        const int4 v = pos[i];
+        for (int d = 0; d < 3; d++) {
+            const double w = raw_to_double(comp(v, d));
+            mx[d] = dmax(mx[d], w);
+            mn[d] = dmin(mn[d], w);
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        mn[d] = wave_dmin(mn[d]);
+        mx[d] = wave_dmax(mx[d]);
+        xc[d] = (end > begin) ? (mx[d] + mn[d]) * 0.5 : (bb[d] + be[d]) * 0.5;
+    }
+    double r2max = 0.0;
+    for (int i = begin + lane; i < end; i += 64) {
+        const int4 v = pos[i];
+        double r2 = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double e = xc[d] - raw_to_double(comp(v, d));
+            r2 += e * e;
+        }
+        r2max = dmax(r2max, r2);
+    }
+    return wave_dmax(r2max);
+}
+
+// A leaf of <= 32 particles (tree.cpp:24-100): lane k holds particle k; the
+// moments {N, -sum dx} are summed in particle order.
+__device__ tpm_tree_node tree_leaf(const int4* __restrict__ pos, int begin, int end, const double* bb,
+                                   const double* be, double hsoft) {
+    const int lane = (int)lane_id();
+    const int n = end - begin;
+    double xc[3];
+    const double r2max = tree_bbox(pos, begin, end, bb, be, xc);
+    float dx0 = 0.f, dx1 = 0.f, dx2 = 0.f;
+    if (lane < n) {
+        const int4 v = pos[begin + lane];
+        dx0 = (float)(raw_to_double(comp(v, 0)) - xc[0]);
+        dx1 = (float)(raw_to_double(comp(v, 1)) - xc[1]);
+        dx2 = (float)(raw_to_double(comp(v, 2)) - xc[2]);
+    }
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+    for (int k = 0; k < n; k++) {
+        m0 += 1.0f;
+        m1 += -__shfl(dx0, k);
+        m2 += -__shfl(dx1, k);
+        m3 += -__shfl(dx2, k);
+    }
+    tpm_tree_node nd;
+    for (int d = 0; d < 3; d++) nd.x[d] = tree_fixed(xc[d]);
+    nd.radius = (float)(sqrt(r2max) + hsoft);
+    nd.child[0] = nd.child[1] = -1;
+    nd.begin = begin;
+    nd.end = end;
+    nd.m[0] = m0;
+    nd.m[1] = m1;
+    nd.m[2] = m2;
+    nd.m[3] = m3;
+    return nd;
+}
+
+// An internal node from its children (tree.cpp:101-171).
+__device__ tpm_tree_node tree_internal(const tpm_tree_node& c0, const tpm_tree_node& c1, const double* bb,
+                                       const double* be, double hsoft) {
+    const float mass0 = c0.m[0], mass1 = c1.m[0];
+    double n[3], xc[3];
+    for (int d = 0; d < 3; d++) n[d] = raw_to_double(c1.x[d]) - raw_to_double(c0.x[d]);
+    const double norminv = 1.0 / sqrt(sqr3_d(n[0], n[1], n[2]));
+    for (int d = 0; d < 3; d++) n[d] *= norminv;
+    for (int d = 0; d < 3; d++) {
+        const double x0 = raw_to_double(c0.x[d]), x1 = raw_to_double(c1.x[d]);
+        if (mass1 == 0.0f)
+            xc[d] = x0;
+        else if (mass0 == 0.0f)
+            xc[d] = x1;
+        else
+            xc[d] = (x0 + x1 + n[d] * (double)(c1.radius - c0.radius)) * 0.5;
+    }
+    // M2M_kernel (kernels.hpp:134-152) of each child about the new centre
+    tpm_tree_node nd;
+    float mm[2][4];
+    const tpm_tree_node* cs[2] = {&c0, &c1};
+    for (int k = 0; k < 2; k++) {
+        float X[3];
+        for (int d = 0; d < 3; d++) X[d] = -(float)(raw_to_double(cs[k]->x[d]) - xc[d]);
+        const float* Ma = cs[k]->m;
+        mm[k][0] = Ma[0];
+        mm[k][1] = fmaf(X[0], Ma[0], Ma[1]);
+        mm[k][3] = fmaf(X[2], Ma[0], Ma[3]);
+        mm[k][2] = fmaf(X[1], Ma[0], Ma[2]);
+    }
+    for (int j = 0; j < 4; j++) nd.m[j] = mm[0][j] + mm[1][j];
+    for (int d = 0; d < 3; d++) nd.x[d] = tree_fixed(xc[d]);
+    double r20 = 0.0, r21 = 0.0;
+    for (int d = 0; d < 3; d++) {
+        if (mass0 != 0.0f) {
+            const double e = raw_to_double(c0.x[d]) - xc[d];
+            r20 += e * e;
+        }
+        if (mass1 != 0.0f) {
+            const double e = raw_to_double(c1.x[d]) - xc[d];
+            r21 += e * e;
+        }
+    }
+    double r2_max = 0.0;
+    for (int corner = 0; corner < 8; corner++) {
+        const double cx = (corner & 4) ? be[0] : bb[0];
+        const double cy = (corner & 2) ? be[1] : bb[1];
+        const double cz = (corner & 1) ? be[2] : bb[2];
+        r2_max = dmax(r2_max, sqr3_d(cx - xc[0], cy - xc[1], cz - xc[2]));
+    }
+    nd.radius = (float)dmin(dmax(sqrt(r20) + (double)c0.radius, sqrt(r21) + (double)c1.radius), sqrt(r2_max) + hsoft);
+    nd.begin = c0.begin;
+    nd.end = c1.end;
+    return nd;
+}
+
+struct NodeFrame {
+    double b[3], e[3];
+    int begin, end, depth, sunk;
+    int state;  // 0: not entered; 1: children pending
+    int which;  // 0: child 0 of its parent, 1: child 1
+    int index;  // preorder index in the cell's tree
+    int c1;     // child 1's index
+    tpm_tree_node child[2];
+};
+
+__global__ __launch_bounds__(64) void tree_nodes_kernel(int C, const int32_t* __restrict__ cell_begin,
+                                                        const int4* __restrict__ pos,
+                                                        const int32_t* __restrict__ node_begin,
+                                                        const int32_t* __restrict__ bucket_begin,
+                                                        tpm_tree_node* __restrict__ nodes,
+                                                        tpm_tree_bucket* __restrict__ buckets, double hsoft,
+                                                        int32_t* __restrict__ flags) {
+    __shared__ NodeFrame st[TREE_STACK];
+    const int c = blockIdx.x;
+    const int lane = (int)lane_id();
+    tpm_tree_node* out = nodes + node_begin[c];
+    tpm_tree_bucket* bout = buckets + bucket_begin[c];
+    // the partition kernel counted these; a walk that disagrees stops
+    const int nmax = node_begin[c + 1] - node_begin[c], bmax = bucket_begin[c + 1] - bucket_begin[c];
+    if (lane == 0) {
+        const double ninv = 1.0 / (double)C;
+        const int ic[3] = {c % C, (c / C) % C, c / (C * C)};
+        NodeFrame f;
+        for (int d = 0; d < 3; d++) {
+            f.b[d] = ic[d] * ninv;
+            f.e[d] = (ic[d] + 1) * ninv;
+        }
+        f.begin = cell_begin[c];
+        f.end = cell_begin[c + 1];
+        f.depth = 0;
+        f.sunk = 0;
+        f.state = 0;
+        f.which = 0;
+        f.index = 0;
+        f.c1 = -1;
+        st[0] = f;
+    }
+    __syncthreads();
+    int sp = 1, next = 0, nb = 0;
+    while (sp > 0) {
+        NodeFrame& f = st[sp - 1];
+        const int begin = f.begin, end = f.end, n = end - begin;
+        double bb[3], be[3];
+        for (int d = 0; d < 3; d++) {
+            bb[d] = f.b[d];
+            be[d] = f.e[d];
+        }
+        tpm_tree_node done;
+        int index;
+        if (f.state == 0) {
+            index = next++;
+            if (index >= nmax || sp + 2 > TREE_STACK || (n <= TREE_BUCKET && !f.sunk && nb >= bmax)) {
+                if (lane == 0) atomicOr(&flags[2], 1);
+                break;
+            }
+            if (n <= TREE_BUCKET && !f.sunk) {
+                double xc[3];
+                const double r2max = tree_bbox(pos, begin, end, bb, be, xc);
+                if (lane == 0) {
+                    tpm_tree_bucket bk;
+                    bk.begin = begin;
+                    bk.end = end;
+                    for (int d = 0; d < 3; d++) bk.x[d] = tree_fixed(xc[d]);
+                    bk.radius = (float)(sqrt(r2max) + hsoft);
+                    bk.cell = c;
+                    bk.pad_ = 0;
+                    bout[nb] = bk;
+                }
+                nb++;
+            }
+            if (n > TREE_LEAF) {
+                const int dim = split_dim(bb, be);
+                const double mid = (be[dim] + bb[dim]) / 2.0;
+                const uint32_t thr = tree_fixed(mid);
+                int lo = begin, hi = end;  // first particle of the right half
+                while (lo < hi) {
+                    const int q = (lo + hi) >> 1;
+                    if (comp(pos[q], dim) < thr)
+                        lo = q + 1;
+                    else
+                        hi = q;
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    f.state = 1;
+                    f.index = index;
+                    NodeFrame L, R;
+                    for (int d = 0; d < 3; d++) {
+                        L.b[d] = R.b[d] = bb[d];
+                        L.e[d] = R.e[d] = be[d];
+                    }
+                    L.e[dim] = mid;
+                    R.b[dim] = mid;
+                    L.begin = begin;
+                    L.end = R.begin = lo;
+                    R.end = end;
+                    L.depth = R.depth = f.depth + 1;
+                    L.sunk = R.sunk = n <= TREE_BUCKET;
+                    L.state = R.state = 0;
+                    L.which = 0;
+                    R.which = 1;
+                    L.index = R.index = 0;
+                    L.c1 = R.c1 = -1;
+                    st[sp] = R;
+                    st[sp + 1] = L;
+                }
+                sp += 2;
+                __syncthreads();
+                continue;
+            }
+            done = tree_leaf(pos, begin, end, bb, be, hsoft);
+        } else {
+            index = f.index;
+            done = tree_internal(f.child[0], f.child[1], bb, be, hsoft);
+            done.child[0] = index + 1;
+            done.child[1] = f.c1;
+        }
+        // finished: store it, hand it to the parent, pop
+        const int which = f.which;
+        __syncthreads();
+        if (lane == 0) {
+            out[index] = done;
+            if (sp > 1) {
+                NodeFrame& parent = st[which == 0 ? sp - 3 : sp - 2];
+                parent.child[which] = done;
+                if (which == 1) parent.c1 = index;
+            }
+        }
+        sp--;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The walk.
+// ---------------------------------------------------------------------------
+
+struct WalkArgs {
+    const int4* __restrict__ pos;
+    const tpm_tree_node* __restrict__ nodes;
+    const int32_t* __restrict__ node_begin;
+    const tpm_tree_bucket* __restrict__ buckets;
+    const float* __restrict__ phi;
+    int32_t* __restrict__ flags_dev;  // [1]: walk stack overflow
+    tpm_kick_out o;
+    tpm_box_dev B;
+    int nbuckets, per_xcd;
+    int n;                               // particles
+    float rcut, theta2inv, inv2rs, tsp;  // box units
+    unsigned flags;
+};
+
+// compute_pc_interaction (kick_treepm.cu:226-240): greens_function
+// (kernels.hpp:13-66) and the 4-component M2L_kernel (kernels.hpp:69-88),
+// in box units; d = sink - expansion centre.
+__device__ __forceinline__ void pc_interaction(const WalkArgs& a, float dx, float dy, float dz, const float* M,
+                                               float& gx, float& gy, float& gz, float& ph) {
+    const float r2 = sqr3_f(dx, dy, dz);
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float r0 = r * a.inv2rs;
+    // erfcexp (util.hpp:36-59)
+    const float t1 = 1.f / fmaf(0.3275911f, r0, 1.f);
+    const float exp0 = expf(-r0 * r0);
+    const float t2 = t1 * t1, t3 = t2 * t1, t4 = t2 * t2, t5 = t2 * t3;
+    const float erfc0 = fmaf(0.254829592f, t1,
+                             fmaf(-0.284496736f, t2, fmaf(1.421413741f, t3, fmaf(-1.453152027f, t4, 1.061405429f * t5)))) *
+                        exp0;
+    const float c0 = -2.f * r * a.inv2rs;
+    const float d0 = -erfc0 * rinv;
+    float e0 = a.tsp * exp0 * rinv * a.inv2rs;
+    const float d1 = fmaf(-1.f * d0, rinv, e0);
+    e0 *= c0;
+    const float d2 = fmaf(-3.f * d1, rinv, e0);
+    const float D11 = d1 * rinv;
+    const float x100 = dx * rinv, x010 = dy * rinv, x001 = dz * rinv;
+    const float x002 = x001 * x001, x011 = x010 * x001, x020 = x010 * x010;
+    const float x101 = x100 * x001, x110 = x100 * x010, x200 = x100 * x100;
+    float x21 = x002;
+    x21 += x020;
+    x21 += x200;
+    x21 *= D11;
+    const float D0 = d0, D1 = x100 * d1, D2 = x010 * d1, D3 = x001 * d1;
+    const float D4 = fmaf(x200, d2, 0.f) + x21;
+    const float D5 = x110 * d2;
+    const float D6 = x101 * d2;
+    const float D7 = fmaf(x020, d2, x21);
+    const float D8 = x011 * d2;
+    const float D9 = fmaf(x002, d2, 0.f) + x21;
+    float L0 = 0.f, L1 = 0.f, L2 = 0.f, L3 = 0.f;
+    if (a.B.do_phi) {
+        L0 = fmaf(M[0], D0, L0);
+        L0 = fmaf(M[3], D3, L0);
+        L0 = fmaf(M[2], D2, L0);
+        L0 = fmaf(M[1], D1, L0);
+    }
+    L1 = fmaf(M[0], D1, L1);
+    L2 = fmaf(M[2], D7, L2);
+    L1 = fmaf(M[3], D6, L1);
+    L2 = fmaf(M[1], D5, L2);
+    L1 = fmaf(M[2], D5, L1);
+    L3 = fmaf(M[0], D3, L3);
+    L1 = fmaf(M[1], D4, L1);
+    L3 = fmaf(M[3], D9, L3);
+    L2 = fmaf(M[0], D2, L2);
+    L3 = fmaf(M[2], D8, L3);
+    L2 = fmaf(M[3], D8, L2);
+    L3 = fmaf(M[1], D6, L3);
+    gx -= L1;
+    gy -= L2;
+    gz -= L3;
+    ph += L0;
+}
+
+__global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
+    __shared__ int stack[WALK_STACK];
+    __shared__ int mlist[WALK_LIST];
+    __shared__ int plist[WALK_LIST];
+    __shared__ int4 sbuf[WALK_SBUF];
+    // one contiguous run of buckets per XCD (workgroups are dealt round-robin)
+    const int wg = blockIdx.x;
+    const int k = wg >> 3;
+    if (k >= a.per_xcd) return;
+    const int bi = (wg & 7) * a.per_xcd + k;
+    if (bi >= a.nbuckets) return;
+    const tpm_tree_bucket bk = a.buckets[bi];
+    const tpm_box_dev& B = a.B;
+    const int lane = (int)lane_id();
+    const int ns = bk.end - bk.begin;
+    const bool two = ns > 64;
+    int4 s0 = make_int4((int)bk.x[0], (int)bk.x[1], (int)bk.x[2], -1), s1 = s0;
+    int r0 = 0, r1 = 0;
+    if (lane < ns) {
+        s0 = a.pos[bk.begin + lane];
+        r0 = a.o.rung[s0.w];
+    }
+    if (lane + 64 < ns) {
+        s1 = a.pos[bk.begin + 64 + lane];
+        r1 = a.o.rung[s1.w];
+    }
+    const bool act0 = lane < ns && r0 >= B.min_rung;
+    const bool act1 = lane + 64 < ns && r1 >= B.min_rung;
+    const int nact = __popcll(__ballot(act0)) + __popcll(__ballot(act1));
+    if (nact == 0) return;
+    // offsets from the bucket centre (exact int32, as the direct kernel's)
+    const int ox0 = (int)((uint32_t)s0.x - bk.x[0]), oy0 = (int)((uint32_t)s0.y - bk.x[1]),
+              oz0 = (int)((uint32_t)s0.z - bk.x[2]);
+    const int ox1 = (int)((uint32_t)s1.x - bk.x[0]), oy1 = (int)((uint32_t)s1.y - bk.x[1]),
+              oz1 = (int)((uint32_t)s1.z - bk.x[2]);
+    float pg0[4] = {0.f, 0.f, 0.f, 0.f}, pg1[4] = {0.f, 0.f, 0.f, 0.f};  // PP, 2^-64 / 2^-32 units
+    float cg0[4] = {0.f, 0.f, 0.f, 0.f}, cg1[4] = {0.f, 0.f, 0.f, 0.f};  // PC, box units
+    unsigned long long npairs = 0, npc = 0;
+    bool overflow = false;
+    if (!(a.flags & TPM_SKIP_PP)) {
+        const int C = B.C;
+        const int cx = bk.cell % C, cy = (bk.cell / C) % C, cz = bk.cell / (C * C);
+        const float rcut = a.rcut, t2 = a.theta2inv;
+        const float sr = bk.radius;
+        for (int nbr = 0; nbr < 27 && !overflow; nbr++) {
+            // x outermost, z innermost (kick_treepm.cu:930-940)
+            int jx = cx + nbr / 9 - 1, jy = cy + (nbr / 3) % 3 - 1, jz = cz + nbr % 3 - 1;
+            jx += (jx < 0) ? C : 0;
+            jx -= (jx >= C) ? C : 0;
+            jy += (jy < 0) ? C : 0;
+            jy -= (jy >= C) ? C : 0;
+            jz += (jz < 0) ? C : 0;
+            jz -= (jz >= C) ? C : 0;
+            const int tcell = (jz * C + jy) * C + jx;
+            const tpm_tree_node* tr = a.nodes + a.node_begin[tcell];
+            const int tn = a.node_begin[tcell + 1] - a.node_begin[tcell];
+            int nm = 0, np = 0;
+            // applies and empties the multipole list
+            auto flush_pc = [&]() {
+                for (int q = 0; q < nm; q++) {
+                    const tpm_tree_node& nd = tr[__builtin_amdgcn_readfirstlane(mlist[q])];
+                    pc_interaction(a, dist(s0.x, nd.x[0]), dist(s0.y, nd.x[1]), dist(s0.z, nd.x[2]), nd.m, cg0[0],
+                                   cg0[1], cg0[2], cg0[3]);
+                    if (two)
+                        pc_interaction(a, dist(s1.x, nd.x[0]), dist(s1.y, nd.x[1]), dist(s1.z, nd.x[2]), nd.m,
+                                       cg1[0], cg1[1], cg1[2], cg1[3]);
+                }
+                npc += (unsigned long long)nm * nact;
+                nm = 0;
+            };
+            // stages the leaf list's particles and applies them pair by pair
+            auto flush_pp = [&]() {
+                int q = 0;
+                while (q < np) {
+                    int nbuf = 0;
+                    while (q < np) {
+                        const tpm_tree_node& nd = tr[__builtin_amdgcn_readfirstlane(plist[q])];
+                        const int cnt = min(max(nd.end - nd.begin, 0), TREE_LEAF);
+                        if (nbuf + cnt > WALK_SBUF) break;
+                        if (lane < cnt && nd.begin + lane < a.n) {
+                            const int4 v = a.pos[nd.begin + lane];
+                            sbuf[nbuf + lane] = make_int4((int)((uint32_t)v.x - bk.x[0]), (int)((uint32_t)v.y - bk.x[1]),
+                                                          (int)((uint32_t)v.z - bk.x[2]), 0);
+                        }
+                        nbuf += cnt;
+                        q++;
+                    }
+                    __syncthreads();
+                    if (two) {
+                        for (int j = 0; j < nbuf; j++) {
+                            const int4 s = sbuf[j];
+                            pp_pair<false>(B, ox0, oy0, oz0, s, pg0[0], pg0[1], pg0[2], pg0[3]);
+                            pp_pair<false>(B, ox1, oy1, oz1, s, pg1[0], pg1[1], pg1[2], pg1[3]);
+                        }
+                    } else {
+                        for (int j = 0; j < nbuf; j++) {
+                            const int4 s = sbuf[j];
+                            pp_pair<false>(B, ox0, oy0, oz0, s, pg0[0], pg0[1], pg0[2], pg0[3]);
+                        }
+                    }
+                    npairs += (unsigned long long)nbuf * nact;
+                    __syncthreads();
+                }
+                np = 0;
+            };
+            int sp = 1, visited = 0;
+            if (lane == 0) stack[0] = 0;
+            __syncthreads();
+            while (sp > 0) {
+                const int take = sp < 64 ? sp : 64;
+                visited += take;
+                if (visited > tn) {  // each node is met at most once
+                    overflow = true;
+                    break;
+                }
+                const int idx = lane < take ? stack[sp - 1 - lane] : 0;
+                const bool valid = lane < take && idx >= 0 && idx < tn;
+                __syncthreads();
+                sp -= take;
+                bool is_multi = false, is_open = false, is_leaf = false;
+                tpm_tree_node nd;
+                if (valid) {
+                    nd = tr[idx];
+                    const float dx = dist(bk.x[0], nd.x[0]), dy = dist(bk.x[1], nd.x[1]), dz = dist(bk.x[2], nd.x[2]);
+                    const float R2 = sqr3_f(dx, dy, dz);
+                    const float R = sqrtf(R2);
+                    const bool cutoff = R - nd.radius - sr > rcut;
+                    const float rr = nd.radius + sr;
+                    const bool far = R2 > rr * rr * t2;
+                    const bool leaf = nd.end - nd.begin <= TREE_LEAF;
+                    is_multi = !cutoff && far;
+                    is_open = !cutoff && !far && !leaf;
+                    is_leaf = !cutoff && !far && leaf;
+                }
+                // near internal nodes: children onto the stack
+                const unsigned long long mo = __ballot(is_open);
+                const int nopen = __popcll(mo);
+                if (sp + 2 * nopen > WALK_STACK) {
+                    overflow = true;
+                    break;
+                }
+                if (is_open) {
+                    const int at = sp + 2 * (int)count_below(mo);
+                    stack[at] = nd.child[1];
+                    stack[at + 1] = nd.child[0];
+                }
+                sp += 2 * nopen;
+                // far nodes: the multipole list
+                const unsigned long long mm = __ballot(is_multi);
+                if (nm + __popcll(mm) > WALK_LIST) {
+                    __syncthreads();
+                    flush_pc();
+                }
+                if (is_multi) mlist[nm + (int)count_below(mm)] = idx;
+                nm += __popcll(mm);
+                // near leaves: PC when every active sink is beyond r_src / theta
+                unsigned long long ml = __ballot(is_leaf);
+                while (ml) {
+                    const int j = __ffsll((long long)ml) - 1;
+                    ml &= ml - 1;
+                    const int lidx = __shfl(idx, j);
+                    const uint32_t lx = (uint32_t)__shfl((int)nd.x[0], j), ly = (uint32_t)__shfl((int)nd.x[1], j),
+                                   lz = (uint32_t)__shfl((int)nd.x[2], j);
+                    const float lr = __shfl(nd.radius, j);
+                    const float lim = lr * lr * t2;
+                    const bool n0 = act0 && !(sqr3_f(dist(s0.x, lx), dist(s0.y, ly), dist(s0.z, lz)) > lim);
+                    const bool n1 = act1 && !(sqr3_f(dist(s1.x, lx), dist(s1.y, ly), dist(s1.z, lz)) > lim);
+                    if (__ballot(n0 || n1) != 0ull) {
+                        if (np == WALK_LIST) {
+                            __syncthreads();
+                            flush_pp();
+                        }
+                        if (lane == 0) plist[np] = lidx;
+                        np++;
+                    } else {
+                        if (nm == WALK_LIST) {
+                            __syncthreads();
+                            flush_pc();
+                        }
+                        if (lane == 0) mlist[nm] = lidx;
+                        nm++;
+                    }
+                }
+                __syncthreads();
+            }
+            flush_pc();
+            flush_pp();
+        }
+    }
+    if (overflow) {
+        if (lane == 0) atomicOr(&a.flags_dev[1], 1);
+        return;
+    }
+    if (lane == 0 && a.o.counters) {
+        atomicAdd(&a.o.counters[0], npairs);
+        atomicAdd(&a.o.counters[3], npc);
+    }
+    // short range in box units (PP sums are in 2^-64 / 2^-32), mesh, kick
+    auto finish = [&](const int4& s, int r, const float* pg, const float* cg) {
+        const float gx = pg[0] * 0x1p64f + cg[0], gy = pg[1] * 0x1p64f + cg[1], gz = pg[2] * 0x1p64f + cg[2];
+        const float ph = pg[3] * 0x1p32f + cg[3];
+        float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
+        if (a.phi && !(a.flags & TPM_SKIP_PM))
+            mesh_interp6(a.phi, B.M, (uint32_t)s.x, (uint32_t)s.y, (uint32_t)s.z, pgx, pgy, pgz, pph);
+        kick_epilogue(B, a.flags, a.o, s.w, r, pgx, pgy, pgz, pph, gx, gy, gz, ph);
+    };
+    if (act0) finish(s0, r0, pg0, cg0);
+    if (act1) finish(s1, r1, pg1, cg1);
+}
+
+__global__ void tree_order_kernel(int64_t n, const int4* __restrict__ pos, int32_t* __restrict__ order) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        order[i] = pos[i].w;
+}
+
+__global__ void tree_fill_i32_kernel(int32_t* p, int n, int32_t v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <class T>
+tpm_status grow(tpm_ctx* ctx, T** p, int64_t* cap, int64_t need) {
+    if (need <= *cap && *p) return TPM_OK;
+    if (*p) TPM_HIP(ctx, hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    TPM_HIP(ctx, hipMalloc((void**)p, sizeof(T) * (size_t)(need > 0 ? need : 1)));
+    *cap = need;
+    return TPM_OK;
+}
+
+}  // namespace
+
+tpm_status tpm_tree_ensure(tpm_ctx* ctx) {
+    if (!ctx->chain_valid || ctx->chain_C != ctx->C)
+        return tpm_fail(ctx, TPM_ERR_STATE, "tree: chain mesh not built (call tpm_chainmesh_create)");
+    if (ctx->tree_epoch == ctx->chain_epoch) return TPM_OK;
+    hipStream_t s = ctx->stream;
+    const int C = ctx->C;
+    const int64_t ncell = (int64_t)C * C * C;
+    const int64_t n = ctx->n;
+    int64_t cap = ctx->tree_pos_cap;
+    TPM_TRY(grow(ctx, &ctx->tree_pos, &cap, n));
+    cap = ctx->tree_pos_cap;
+    TPM_TRY(grow(ctx, &ctx->tree_tmp, &cap, n));
+    ctx->tree_pos_cap = cap;
+    if (ncell > ctx->tree_cells_cap || !ctx->tree_cnt) {
+        int64_t c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+        if (ctx->tree_cnt) TPM_HIP(ctx, hipFree(ctx->tree_cnt));
+        if (ctx->node_begin) TPM_HIP(ctx, hipFree(ctx->node_begin));
+        if (ctx->bucket_begin) TPM_HIP(ctx, hipFree(ctx->bucket_begin));
+        if (ctx->tree_part) TPM_HIP(ctx, hipFree(ctx->tree_part));
+        if (ctx->tree_flags) TPM_HIP(ctx, hipFree(ctx->tree_flags));
+        ctx->tree_cnt = ctx->node_begin = ctx->bucket_begin = ctx->tree_part = ctx->tree_flags = nullptr;
+        TPM_TRY(grow(ctx, &ctx->tree_cnt, &c1, 2 * ncell));
+        TPM_TRY(grow(ctx, &ctx->node_begin, &c2, ncell + 1));
+        TPM_TRY(grow(ctx, &ctx->bucket_begin, &c3, ncell + 1));
+        TPM_TRY(grow(ctx, &ctx->tree_part, &c4, ncell / 2048 + 2));
+        TPM_TRY(grow(ctx, &ctx->tree_flags, &c5, 4));
+        ctx->tree_cells_cap = ncell;
+    }
+    tree_fill_i32_kernel<<<1, 64, 0, s>>>(ctx->tree_flags, 4, 0);
+    tree_partition_kernel<<<(unsigned)ncell, 64, 0, s>>>(C, ctx->cell_begin, ctx->x, ctx->y, ctx->z, ctx->tree_pos,
+                                                          ctx->tree_tmp, ctx->tree_cnt, ctx->tree_flags);
+    TPM_HIP(ctx, hipGetLastError());
+    TPM_TRY(tpm_scan_i32(ctx->err, s, ctx->tree_cnt, ncell, ctx->tree_part, ctx->node_begin));
+    TPM_TRY(tpm_scan_i32(ctx->err, s, ctx->tree_cnt + ncell, ncell, ctx->tree_part, ctx->bucket_begin));
+    int32_t tot[3] = {0, 0, 0};
+    TPM_HIP(ctx, hipMemcpyAsync(&tot[0], ctx->node_begin + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TPM_HIP(ctx, hipMemcpyAsync(&tot[1], ctx->bucket_begin + ncell, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TPM_HIP(ctx, hipMemcpyAsync(&tot[2], ctx->tree_flags, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TPM_HIP(ctx, hipStreamSynchronize(s));
+    if (tot[2])
+        return tpm_fail(ctx, TPM_ERR_INVALID,
+                        "tree: depth limit (50) exceeded -- more than 32 particles too close to be split");
+    TPM_TRY(grow(ctx, &ctx->nodes, &ctx->nodes_cap, tot[0]));
+    TPM_TRY(grow(ctx, &ctx->buckets, &ctx->buckets_cap, tot[1]));
+    tree_nodes_kernel<<<(unsigned)ncell, 64, 0, s>>>(C, ctx->cell_begin, ctx->tree_pos, ctx->node_begin,
+                                                      ctx->bucket_begin, ctx->nodes, ctx->buckets, ctx->p.hsoft,
+                                                      ctx->tree_flags);
+    TPM_HIP(ctx, hipGetLastError());
+    TPM_HIP(ctx, hipMemcpyAsync(&tot[2], ctx->tree_flags + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    TPM_HIP(ctx, hipStreamSynchronize(s));
+    if (tot[2]) return tpm_fail(ctx, TPM_ERR_HIP, "tree: node numbering disagrees with the partition counts");
+    ctx->n_nodes = tot[0];
+    ctx->n_buckets = tot[1];
+    ctx->tree_epoch = ctx->chain_epoch;
+    return TPM_OK;
+}
+
+tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
+                         tpm_stats* stats) {
+    const bool use_pm = !(flags & TPM_SKIP_PM);
+    if (!ctx->chain_valid || ctx->chain_C != ctx->C)
+        return tpm_fail(ctx, TPM_ERR_STATE, "tpm_kick: chain mesh not built (call tpm_chainmesh_create)");
+    if (use_pm && (!ctx->pm_valid || ctx->green_M != ctx->M))
+        return tpm_fail(ctx, TPM_ERR_STATE, "tpm_kick: PM mesh not computed (call tpm_gravity_long_compute)");
+    if (t0 <= 0.0) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: t0 must be positive");
+    TPM_TRY(tpm_tree_ensure(ctx));
+    hipStream_t s = ctx->stream;
+    if (!ctx->counters) TPM_HIP(ctx, hipMalloc(&ctx->counters, sizeof(unsigned long long) * 4));
+    TPM_TRY(tpm_fill_u64(ctx, ctx->counters, 4));
+    tree_fill_i32_kernel<<<1, 64, 0, s>>>(ctx->tree_flags, 4, 0);
+    WalkArgs a{};
+    a.pos = ctx->tree_pos;
+    a.nodes = ctx->nodes;
+    a.node_begin = ctx->node_begin;
+    a.buckets = ctx->buckets;
+    a.phi = use_pm ? ctx->phi : nullptr;
+    a.flags_dev = ctx->tree_flags;
+    a.o = tpm_kick_out{ctx->gx, ctx->gy, ctx->gz, ctx->pot, ctx->vx, ctx->vy, ctx->vz, ctx->rung, ctx->counters};
+    tpm_box_fill(&a.B, ctx->p, min_rung, first_call, t0, scale);
+    a.B.cell0 = 0;
+    a.B.mesh0 = 0;
+    a.nbuckets = (int)ctx->n_buckets;
+    a.n = (int)ctx->n;
+    a.per_xcd = (int)((ctx->n_buckets + 7) / 8);
+    // treepm_params (kick_treepm.cu:826-853): rcut, theta = 0.4, 1 / (2 rs)
+    a.rcut = (float)ctx->p.rcut;
+    const float theta = 0.4f;
+    a.theta2inv = 1.0f / (theta * theta);
+    a.inv2rs = 1.0f / (float)ctx->p.rs / 2.0f;
+    a.tsp = 2.0f / sqrtf((float)M_PI);
+    a.flags = flags & (TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK);
+    if (!ctx->ev[0]) {
+        for (int e = 0; e < 8; e++) TPM_HIP(ctx, hipEventCreate(&ctx->ev[e]));
+    }
+    hipEvent_t e0 = ctx->ev[2], e1 = ctx->ev[3];
+    if (ctx->prof_on) {
+        e0 = tpm_prof_event(ctx);
+        e1 = tpm_prof_event(ctx);
+        if (!e0 || !e1) return tpm_fail(ctx, TPM_ERR_HIP, "hipEventCreate failed");
+        ctx->prof_pp.emplace_back(e0, e1);
+    }
+    TPM_HIP(ctx, hipEventRecord(e0, s));
+    if (ctx->n > 0 && a.per_xcd > 0) {
+        tree_walk_kernel<<<(unsigned)(8 * (int64_t)a.per_xcd), 64, 0, s>>>(a);
+        TPM_HIP(ctx, hipGetLastError());
+    }
+    TPM_HIP(ctx, hipEventRecord(e1, s));
+    int32_t fl[2] = {0, 0};
+    TPM_HIP(ctx, hipMemcpyAsync(fl, ctx->tree_flags, sizeof(fl), hipMemcpyDeviceToHost, s));
+    unsigned long long hc[4] = {0, 0, 0, 0};
+    if (stats) {
+        TPM_TRY(tpm_candidate_pairs(ctx, ctx->counters + 2));
+        TPM_HIP(ctx, hipMemcpyAsync(hc, ctx->counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+    }
+    TPM_HIP(ctx, hipStreamSynchronize(s));
+    if (fl[1])
+        return tpm_fail(ctx, TPM_ERR_INVALID,
+                        "tpm_kick: tree walk stopped (more than 2048 pending nodes, or a node met twice)");
+    if (stats) {
+        float ms = 0.f;
+        TPM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+        stats->executed_pairs = (double)hc[0];
+        stats->max_rung = (int)hc[1];
+        stats->candidate_pairs = (double)hc[2];
+        stats->pc_interactions = (double)hc[3];
+        stats->ms_pp = ms;
+        stats->n_tiles = ctx->n_buckets;
+    }
+    return TPM_OK;
+}
+
+extern "C" tpm_status tpm_tree_build(tpm_ctx* ctx) {
+    if (!ctx) return TPM_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    return tpm_tree_ensure(ctx);
+}
+
+extern "C" tpm_status tpm_tree_info(tpm_ctx* ctx, int64_t* n_nodes, int64_t* n_buckets) {
+    if (!ctx) return TPM_ERR_INVALID;
+    if (ctx->tree_epoch != ctx->chain_epoch || !ctx->chain_valid)
+        return tpm_fail(ctx, TPM_ERR_STATE, "tpm_tree_info: trees not built for the current chain mesh");
+    if (n_nodes) *n_nodes = ctx->n_nodes;
+    if (n_buckets) *n_buckets = ctx->n_buckets;
+    return TPM_OK;
+}
+
+extern "C" tpm_status tpm_tree_download(tpm_ctx* ctx, int64_t* node_begin, int64_t* bucket_begin,
+                                        tpm_tree_node* nodes, tpm_tree_bucket* buckets, int32_t* order) {
+    if (!ctx) return TPM_ERR_INVALID;
+    if (ctx->tree_epoch != ctx->chain_epoch || !ctx->chain_valid)
+        return tpm_fail(ctx, TPM_ERR_STATE, "tpm_tree_download: trees not built for the current chain mesh");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const int64_t ncell = (int64_t)ctx->C * ctx->C * ctx->C;
+    std::vector<int32_t> nb((size_t)ncell + 1), bb((size_t)ncell + 1);
+    TPM_HIP(ctx, hipMemcpyAsync(nb.data(), ctx->node_begin, sizeof(int32_t) * (ncell + 1), hipMemcpyDeviceToHost, s));
+    TPM_HIP(ctx,
+            hipMemcpyAsync(bb.data(), ctx->bucket_begin, sizeof(int32_t) * (ncell + 1), hipMemcpyDeviceToHost, s));
+    if (nodes && ctx->n_nodes)
+        TPM_HIP(ctx, hipMemcpyAsync(nodes, ctx->nodes, sizeof(tpm_tree_node) * ctx->n_nodes, hipMemcpyDeviceToHost, s));
+    if (buckets && ctx->n_buckets)
+        TPM_HIP(ctx, hipMemcpyAsync(buckets, ctx->buckets, sizeof(tpm_tree_bucket) * ctx->n_buckets,
+                                    hipMemcpyDeviceToHost, s));
+    int32_t* dorder = nullptr;
+    if (order && ctx->n) {
+        TPM_HIP(ctx, hipMallocAsync((void**)&dorder, sizeof(int32_t) * ctx->n, s));
+        tree_order_kernel<<<grid_for(ctx->n, 256), 256, 0, s>>>(ctx->n, ctx->tree_pos, dorder);
+        TPM_HIP(ctx, hipGetLastError());
+        TPM_HIP(ctx, hipMemcpyAsync(order, dorder, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost, s));
+        TPM_HIP(ctx, hipFreeAsync(dorder, s));
+    }
+    TPM_HIP(ctx, hipStreamSynchronize(s));
+    for (int64_t c = 0; c <= ncell; c++) {
+        if (node_begin) node_begin[c] = nb[c];
+        if (bucket_begin) bucket_begin[c] = bb[c];
+    }
+    return TPM_OK;
+}
