@@ -134,6 +134,38 @@ def accuracy(ctx, p, nsamp, seed):
             "potential_rms": rel_rms(P, ph * p.GM), "ewald_seconds": t_ew}
 
 
+def tree_leg(T, ctx, p, steps, nsamp, seed, ewald=True):
+    """The same particles through kick_treepm's own algorithm (TPM_KICK_TREE:
+    per-cell trees, theta = 0.4 monopole + dipole, PP for near leaves): one
+    step = chain mesh + PM + tree build + walk/kick, each stage waited for;
+    forces against the FP64 Ewald sum.  Not the benchmark value."""
+    rows = []
+    st = None
+    for k in range(steps + 1):
+        ctx.sync()
+        t = time.perf_counter()
+        ctx.chainmesh_create()
+        ctx.gravity_long_compute()
+        ctx.sync()
+        t1 = time.perf_counter()
+        ctx.tree_build()
+        ctx.sync()
+        t2 = time.perf_counter()
+        st = ctx.kick(0, 1.0, 1.0, True, flags=T.TPM_KICK_TREE)
+        t3 = time.perf_counter()
+        if k:  # the first step is a warm-up
+            rows.append((t1 - t, t2 - t1, t3 - t2, st.ms_pp, t3 - t))
+    r = np.mean(np.array(rows), 0)
+    out = {"steps": steps, "kicks_per_s": ctx.size() / r[4], "ms_per_step": r[4] * 1e3,
+           "stages_ms": {"chainmesh_pm": r[0] * 1e3, "tree_build": r[1] * 1e3, "walk_kick": r[2] * 1e3,
+                         "walk_kernel_events": r[3]},
+           "pp_pairs": st.executed_pairs, "pc_interactions": st.pc_interactions,
+           "candidate_pairs": st.candidate_pairs, "sink_buckets": int(st.n_tiles)}
+    if ewald:
+        out["accuracy"] = accuracy(ctx, p, nsamp, seed)
+    return out
+
+
 def ewald_error(ctx, d, nsamp, seed):
     """Vector rms force error of downloaded state d against the FP64 Ewald
     sum of the same particles (resident in ctx)."""
@@ -311,9 +343,10 @@ def main():
     ap.add_argument("--no-pcie", action="store_true", help="skip the host-buffer (PCIe-inclusive) measurement")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-box batched-ensemble leg")
     ap.add_argument("--no-serving", action="store_true", help="skip the continuous-batching serving leg")
+    ap.add_argument("--no-tree", action="store_true", help="skip the tree-walk (kick_treepm algorithm) leg")
     ap.add_argument("--ensemble-graph", type=int, default=1, choices=[0, 1],
                     help="replay the ensemble step as a captured graph (0: plain launches, e.g. for tracing)")
-    ap.add_argument("--only", choices=["ensemble", "serving"],
+    ap.add_argument("--only", choices=["ensemble", "serving", "tree"],
                     help="run only this leg and print its result (for profiling; not the benchmark line)")
     ap.add_argument("--serving-rates", default="4,8,12,16", help="Poisson arrival rates (requests/s)")
     ap.add_argument("--serving-requests", type=int, default=40)
@@ -337,6 +370,16 @@ def main():
     w = WORKLOADS[args.workload]
     ctx = T.Context(0)
     p = ctx.set_params(T.baseline_params(w["n_side"], w["mesh"], w["order"], interlace=w["interlace"]))
+    if args.only == "tree":
+        if w["ic"] == "zeldovich":
+            ctx.ic_zeldovich(w["n_side"], w["seed"], w["rms"], -2.0)
+        else:
+            ctx.ic_uniform(w["n_side"] ** 3, w["seed"])
+        ctx.kick_treepm(0, 1.0, 1.0, True, stats=True)
+        print(json.dumps({"tree": tree_leg(T, ctx, p, max(args.steps, 1), w["samples"], w["seed"],
+                                           ewald=not args.no_ewald)}), flush=True)
+        ctx.close()
+        return
     log("workload", args.workload, "params", p.as_dict())
     t = time.perf_counter()
     if w["ic"] == "zeldovich":
@@ -420,6 +463,9 @@ def main():
         log("FP64 Ewald on %d samples..." % w["samples"])
         out["accuracy"] = accuracy(ctx, p, w["samples"], w["seed"])
         log("accuracy", out["accuracy"])
+    if not args.no_tree:
+        out["tree"] = tree_leg(T, ctx, p, min(args.steps, 3), w["samples"], w["seed"], ewald=not args.no_ewald)
+        log("tree", out["tree"])
     if not args.no_cpu:
         log("CPU baseline (budget %.0f s)..." % args.cpu_budget)
         out["cpu_baseline"] = cpu_baseline(ctx, p, args.cpu_budget, w["seed"])

@@ -104,10 +104,12 @@ struct options {
     uint64_t seed = 42;
     int tile_sinks = 16;     // PP work-item size (tpm_tuning)
     int interlace = 0;       // 1: interlaced (two-grid) deposit
+    std::string short_range = "direct";  // kick_treepm's short range: direct 27-cell sum | tree (theta = 0.4 walk)
 };
 // Parses --parts_dim --parts_o_four --four_o_chain --test plus the extensions
 // (--four_dim --rs_cells --rcut_rs --order --mass=unit|1/N --ic --rms_disp
-// --seed --tile_sinks); derives the dependent fields (options.cpp:81-96).
+// --seed --tile_sinks --interlace --short=direct|tree); derives the dependent
+// fields (options.cpp:81-96).
 bool process_options(int argc, char* argv[]);
 const options& get_options();
 void set_options(const options& opts);  // derives and installs
@@ -186,5 +188,6 @@ void run_test(std::string test);
 struct kick_report {
     double ms_chainmesh, ms_pm, ms_pp;
     double candidate_pairs, executed_pairs;
+    double pc_interactions;  // --short=tree: sink x multipole interactions
 };
 kick_report last_kick_report();

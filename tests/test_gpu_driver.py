@@ -32,6 +32,17 @@ def test_treepm_scenario_reference_defaults():
     assert e["Force RMS Error"] < 1e-2
 
 
+def test_treepm_scenario_tree_walk():
+    # --short=tree: the reference's own short-range algorithm (per-cell trees,
+    # theta = 0.4 monopole + dipole, PP for near leaves) at its defaults
+    out, e = run("--test=treepm", "--parts_dim=90", "--short=tree")
+    m = re.search(r"multipole interactions (\S+)", out)
+    assert m and float(m.group(1)) > 0
+    _, d = run("--test=treepm", "--parts_dim=90")
+    # the multipoles add their approximation error to the direct sum's
+    assert d["Force RMS Error"] < e["Force RMS Error"] < 5e-2
+
+
 def test_kick_pm_scenario():
     out, e = run("--test=kick_pm", "--parts_dim=60")
     assert "Force RMS Error" in out

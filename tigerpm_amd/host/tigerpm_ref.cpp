@@ -130,7 +130,8 @@ bool process_options(int argc, char* argv[]) {
         std::string s = argv[a];
         if (s == "--help") {
             std::printf("options: --parts_dim --parts_o_four --four_o_chain --test --four_dim --rs_cells --rcut_rs "
-                        "--order --mass=unit|1/N --ic=uniform|zeldovich --rms_disp --seed --tile_sinks\n");
+                        "--order --mass=unit|1/N --ic=uniform|zeldovich --rms_disp --seed --tile_sinks --interlace "
+                        "--short=direct|tree\n");
             return false;
         }
         const size_t eq = s.find('=');
@@ -154,6 +155,7 @@ bool process_options(int argc, char* argv[]) {
         else if (k == "seed") o.seed = std::stoull(v);
         else if (k == "tile_sinks") o.tile_sinks = std::stoi(v);
         else if (k == "interlace") o.interlace = std::stoi(v);
+        else if (k == "short" && (v == "direct" || v == "tree")) o.short_range = v;
         else {
             std::fprintf(stderr, "unrecognised option %s\n", s.c_str());
             return false;
@@ -162,8 +164,9 @@ bool process_options(int argc, char* argv[]) {
     set_options(o);
     const options& d = get_options();
     std::printf("Simulation Options\nparts_dim = %d\nfour_dim = %d\nchain_dim = %d\nrs = %e\nrcut = %e\nhsoft = %e\n"
-                "order = %d\ntest = %s\n",
-                d.parts_dim, d.four_dim, d.chain_dim, d.rs, d.rcut, d.hsoft, d.order, d.test.c_str());
+                "order = %d\nshort = %s\ntest = %s\n",
+                d.parts_dim, d.four_dim, d.chain_dim, d.rs, d.rcut, d.hsoft, d.order, d.short_range.c_str(),
+                d.test.c_str());
     return true;
 }
 
@@ -379,14 +382,16 @@ static void device_kick(int min_rung, double scale, double t0, bool first_call, 
     g_report.ms_pp = st.ms_pp;
     g_report.candidate_pairs = st.candidate_pairs;
     g_report.executed_pairs = st.executed_pairs;
+    g_report.pc_interactions = st.pc_interactions;
     download_state(true);
 }
 
-// kick_treepm_begin (kick_treepm.cpp:10-54 + kick_treepm.cu:761-1089): the
-// reference builds per-cell trees and walks them; this build evaluates the
-// same erfc-split pair force over the 27 neighbour cells directly.
+// kick_treepm_begin (kick_treepm.cpp:10-54 + kick_treepm.cu:761-1089).
+// --short=tree: the reference's algorithm -- per-cell trees built and walked
+// on the device (multipoles for far nodes).  --short=direct (default): the
+// erfc-split pair force over every pair of the 27 neighbour cells.
 void kick_treepm_begin(int min_rung, double scale, double t0, bool first_call) {
-    device_kick(min_rung, scale, t0, first_call, 0u);
+    device_kick(min_rung, scale, t0, first_call, get_options().short_range == "tree" ? TPM_KICK_TREE : 0u);
 }
 void kick_treepm_end() {}  // no ghost particles to drop on one device
 // kick_pme_begin (kick_pme.cpp, kick_pme.cu:128-357): the same kick with
@@ -552,8 +557,8 @@ void run_test(std::string test) {
                 kick_treepm_end();
             });
         const kick_report r = last_kick_report();
-        std::printf("PP kernel %e ms, candidate pairs %e, evaluated pairs %e\n", r.ms_pp, r.candidate_pairs,
-                    r.executed_pairs);
+        std::printf("PP kernel %e ms, candidate pairs %e, evaluated pairs %e, multipole interactions %e\n", r.ms_pp,
+                    r.candidate_pairs, r.executed_pairs, r.pc_interactions);
         std::printf("COMPARE\n");
         gravity_short_ewald_compare(100);
     } else if (test == "kick_pm") {
