@@ -210,16 +210,30 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 7) void pp_kick_kernel(PPArgs a) 
                 gz *= 0x1p64f;
                 ph *= 0x1p32f;
             }
+            // Mesh force and potential: the source streams split the six x
+            // planes of the 6^3 stencil, then sum.
+            float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
+            const bool use_mesh = a.phi && !(a.flags & TPM_SKIP_PM);
+            if (use_mesh) {
+                mesh_interp6_part(a.phi + B.mesh0, B.M, px, py, pz, stream, SPLIT, sx, sy, sz, sp);
+#pragma unroll
+                for (int o = TS; o < 64; o <<= 1) {
+                    sx += __shfl_xor(sx, o);
+                    sy += __shfl_xor(sy, o);
+                    sz += __shfl_xor(sz, o);
+                    sp += __shfl_xor(sp, o);
+                }
+            }
             if constexpr (!PERSIST) {
                 if (!active || stream != 0) return;
             } else {
                 if (!active || stream != 0) goto next_tile;
             }
             {
-                // Mesh force and potential, then the kick.
-                float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
-                if (a.phi && !(a.flags & TPM_SKIP_PM))
-                    mesh_interp6(a.phi + B.mesh0, B.M, px, py, pz, pgx, pgy, pgz, pph);
+                // Then the kick.
+                const float Mf = (float)B.M;
+                const float pgx = use_mesh ? -sx * Mf : 0.f, pgy = use_mesh ? -sy * Mf : 0.f;
+                const float pgz = use_mesh ? -sz * Mf : 0.f, pph = use_mesh ? sp : 0.f;
                 const tpm_kick_out o{a.gx, a.gy, a.gz, a.pot, a.vx, a.vy, a.vz, a.rung, a.counters};
                 kick_epilogue(B, a.flags, o, i, r, pgx, pgy, pgz, pph, gx, gy, gz, ph);
             }

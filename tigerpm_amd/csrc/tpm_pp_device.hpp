@@ -143,9 +143,12 @@ __device__ __forceinline__ void interp6_weight(float x, int k, float* w, float* 
 // Mesh force and potential of one sink (kick_treepm.cu:505-573) on the
 // periodic mesh phi (M^3), unit-mass: g = -M sum dw w w phi, p = sum w w w phi.
 // Separable: along z (w, dw), then y, then x, so only the y and z weights
-// are live; the x weights are made where used.
-__device__ __forceinline__ void mesh_interp6(const float* __restrict__ phi, int M, uint32_t px, uint32_t py,
-                                             uint32_t pz, float& pgx, float& pgy, float& pgz, float& pph) {
+// are live; the x weights are made where used.  This part adds the x planes
+// aa0, aa0 + astep, ... (< 6) to the raw sums sx, sy, sz (gradient, before
+// the factor -M) and sp, so several lanes can share one sink's stencil.
+__device__ __forceinline__ void mesh_interp6_part(const float* __restrict__ phi, int M, uint32_t px, uint32_t py,
+                                                  uint32_t pz, int aa0, int astep, float& sx, float& sy, float& sz,
+                                                  float& sp) {
     const uint32_t pr[3] = {px, py, pz};
     int I0[3];
     float X[3];
@@ -166,9 +169,8 @@ __device__ __forceinline__ void mesh_interp6(const float* __restrict__ phi, int 
         j -= (j >= M) ? M : 0;
         jz[cc] = j;
     }
-    float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
 #pragma unroll 1
-    for (int aa = 0; aa < 6; aa++) {
+    for (int aa = aa0; aa < 6; aa += astep) {
         int jx = I0[0] + aa;
         jx -= (jx >= M) ? M : 0;
         jx -= (jx >= M) ? M : 0;
@@ -197,6 +199,13 @@ __device__ __forceinline__ void mesh_interp6(const float* __restrict__ phi, int 
         sz = fmaf(wx, t, sz);
         sp = fmaf(wx, u, sp);
     }
+}
+
+// The whole stencil in one lane.
+__device__ __forceinline__ void mesh_interp6(const float* __restrict__ phi, int M, uint32_t px, uint32_t py,
+                                             uint32_t pz, float& pgx, float& pgy, float& pgz, float& pph) {
+    float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
+    mesh_interp6_part(phi, M, px, py, pz, 0, 1, sx, sy, sz, sp);
     const float Mf = (float)M;
     pgx = -sx * Mf;
     pgy = -sy * Mf;
