@@ -138,6 +138,31 @@ def test_tree_and_direct_kicks_differ_by_the_multipole_error(gpu_ctx):
     assert abs(gpu_diff - orc_diff) < 0.02 * orc_diff
 
 
+@pytest.mark.parametrize("n,params", [(60, "baseline32"), (45 ** 3, "reference45")])
+def test_sparse_and_reference_boxes(gpu_ctx, n, params):
+    """A box with mostly empty cells (empty trees and buckets) and the
+    reference's own parameters (rcut = 1/chain_dim, C = 3, unit masses)."""
+    p = T.baseline_params(32, 32, 4) if params == "baseline32" else T.reference_params(45)
+    rng = np.random.default_rng(17)
+    x, y, z = [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    p = gpu_ctx.set_params(p)
+    gpu_ctx.upload(x, y, z)
+    gpu_ctx.chainmesh_create()
+    operm, ocb = O.chainmesh_sort(x, y, z, p.chain_dim)
+    xs, ys, zs = x[operm], y[operm], z[operm]
+    print("sparse/reference", n, params, flush=True)
+    gpu_ctx.tree_build()
+    g = gpu_ctx.tree_download()
+    o = O.tree_build(oparams(p), xs, ys, zs, ocb)
+    assert g["nodes"].tobytes() == o["nodes"].tobytes()
+    st = gpu_ctx.kick(flags=T.TPM_KICK_TREE | T.TPM_SKIP_PM | T.TPM_SKIP_KICK)
+    d = gpu_ctx.download()
+    (ox, oy, oz, oph), ost = O.tree_sum(oparams(p), xs, ys, zs, ocb, np.zeros(n, np.int8))
+    assert st.pc_interactions == ost["pc"] and st.executed_pairs == ost["pp"]
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1) / p.GM
+    assert rel_rms(G, np.stack([ox, oy, oz], 1)) < 1e-5
+
+
 def test_depth_limit_is_an_error(gpu_ctx):
     n_side = 24
     n = n_side ** 3
