@@ -18,7 +18,10 @@
 //   The arithmetic is the reference's, operation for operation, so the trees
 //   are bit-identical to the CPU restatement's.
 //
-// Walk, one wave64 per sink bucket (<= 128 sinks, two per lane), for each of
+// Walk, one wave64 per sink bucket of <= 128 sinks: lane l holds sinks
+// (l & 31) + 32 k for k < K = ceil(sinks / 32) and takes every other
+// multipole node and staged particle (source stream l >> 5), so a bucket of
+// 89 sinks keeps 93 % of the lanes' sink slots busy.  For each of
 // the 27 neighbour trees: nodes are taken 64 at a time from an LDS stack,
 // each lane tests one (cutoff: R - r_src - r_snk > rcut; far: R^2 >
 // (r_src + r_snk)^2 / theta^2); far nodes go to the multipole list, near
@@ -38,7 +41,7 @@
 #define TREE_BUCKET 128  // SINK_BUCKET_SIZE
 #define TREE_DEPTH 50    // MAX_DEPTH, tree.cpp:5
 #define TREE_STACK 64    // build frames: a depth-first walk holds <= depth + 2
-#define WALK_STACK 2048  // walk: pending nodes of one neighbour tree
+#define WALK_STACK 1024  // walk: pending nodes of one neighbour tree
 #define WALK_LIST 128    // multipole / leaf list entries between flushes
 #define WALK_SBUF 256    // staged PP sources
 
@@ -525,43 +528,46 @@ __device__ __forceinline__ void pc_interaction(const WalkArgs& a, float dx, floa
     ph += L0;
 }
 
-__global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
-    __shared__ int stack[WALK_STACK];
-    __shared__ int mlist[WALK_LIST];
-    __shared__ int plist[WALK_LIST];
-    __shared__ int4 sbuf[WALK_SBUF];
-    // one contiguous run of buckets per XCD (workgroups are dealt round-robin)
-    const int wg = blockIdx.x;
-    const int k = wg >> 3;
-    if (k >= a.per_xcd) return;
-    const int bi = (wg & 7) * a.per_xcd + k;
-    if (bi >= a.nbuckets) return;
-    const tpm_tree_bucket bk = a.buckets[bi];
+struct WalkShared {
+    int stack[WALK_STACK];
+    int mlist[WALK_LIST];
+    int plist[WALK_LIST];
+    int4 sbuf[WALK_SBUF];
+};
+
+// One sink bucket of <= 32 K sinks.  Lane l holds sinks (l & 31) + 32 k,
+// k < K, and works on source stream l >> 5 (every other multipole node and
+// staged particle); the two streams' sums meet at the end.
+template <int K>
+__device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bucket& bk, WalkShared& sh) {
     const tpm_box_dev& B = a.B;
     const int lane = (int)lane_id();
+    const int slot = lane & 31, stream = lane >> 5;
     const int ns = bk.end - bk.begin;
-    const bool two = ns > 64;
-    int4 s0 = make_int4((int)bk.x[0], (int)bk.x[1], (int)bk.x[2], -1), s1 = s0;
-    int r0 = 0, r1 = 0;
-    if (lane < ns) {
-        s0 = a.pos[bk.begin + lane];
-        r0 = a.o.rung[s0.w];
+    int4 s[K];
+    int r[K], ox[K], oy[K], oz[K];
+    bool act[K];
+    float pg[K][4], cg[K][4];  // PP sums (2^-64 / 2^-32 units), PC sums (box units)
+    int nact = 0;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int j = slot + 32 * k;
+        s[k] = make_int4((int)bk.x[0], (int)bk.x[1], (int)bk.x[2], -1);
+        r[k] = 0;
+        if (j < ns) {
+            s[k] = a.pos[bk.begin + j];
+            r[k] = a.o.rung[s[k].w];
+        }
+        act[k] = j < ns && r[k] >= B.min_rung;
+        nact += __popcll(__ballot(act[k] && stream == 0));
+        // offsets from the bucket centre (exact int32, as the direct kernel's)
+        ox[k] = (int)((uint32_t)s[k].x - bk.x[0]);
+        oy[k] = (int)((uint32_t)s[k].y - bk.x[1]);
+        oz[k] = (int)((uint32_t)s[k].z - bk.x[2]);
+#pragma unroll
+        for (int c = 0; c < 4; c++) pg[k][c] = cg[k][c] = 0.f;
     }
-    if (lane + 64 < ns) {
-        s1 = a.pos[bk.begin + 64 + lane];
-        r1 = a.o.rung[s1.w];
-    }
-    const bool act0 = lane < ns && r0 >= B.min_rung;
-    const bool act1 = lane + 64 < ns && r1 >= B.min_rung;
-    const int nact = __popcll(__ballot(act0)) + __popcll(__ballot(act1));
     if (nact == 0) return;
-    // offsets from the bucket centre (exact int32, as the direct kernel's)
-    const int ox0 = (int)((uint32_t)s0.x - bk.x[0]), oy0 = (int)((uint32_t)s0.y - bk.x[1]),
-              oz0 = (int)((uint32_t)s0.z - bk.x[2]);
-    const int ox1 = (int)((uint32_t)s1.x - bk.x[0]), oy1 = (int)((uint32_t)s1.y - bk.x[1]),
-              oz1 = (int)((uint32_t)s1.z - bk.x[2]);
-    float pg0[4] = {0.f, 0.f, 0.f, 0.f}, pg1[4] = {0.f, 0.f, 0.f, 0.f};  // PP, 2^-64 / 2^-32 units
-    float cg0[4] = {0.f, 0.f, 0.f, 0.f}, cg1[4] = {0.f, 0.f, 0.f, 0.f};  // PC, box units
     unsigned long long npairs = 0, npc = 0;
     bool overflow = false;
     if (!(a.flags & TPM_SKIP_PP)) {
@@ -569,6 +575,8 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
         const int cx = bk.cell % C, cy = (bk.cell / C) % C, cz = bk.cell / (C * C);
         const float rcut = a.rcut, t2 = a.theta2inv;
         const float sr = bk.radius;
+        // Far dummy that pads a staged list to an even length: r >= rcut.
+        const int4 far = make_int4(0x40000000, 0x40000000, 0x40000000, 0);
         for (int nbr = 0; nbr < 27 && !overflow; nbr++) {
             // x outermost, z innermost (kick_treepm.cu:930-940)
             int jx = cx + nbr / 9 - 1, jy = cy + (nbr / 3) % 3 - 1, jz = cz + nbr % 3 - 1;
@@ -582,15 +590,14 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
             const tpm_tree_node* tr = a.nodes + a.node_begin[tcell];
             const int tn = a.node_begin[tcell + 1] - a.node_begin[tcell];
             int nm = 0, np = 0;
-            // applies and empties the multipole list
+            // applies and empties the multipole list (each stream every other node)
             auto flush_pc = [&]() {
-                for (int q = 0; q < nm; q++) {
-                    const tpm_tree_node& nd = tr[__builtin_amdgcn_readfirstlane(mlist[q])];
-                    pc_interaction(a, dist(s0.x, nd.x[0]), dist(s0.y, nd.x[1]), dist(s0.z, nd.x[2]), nd.m, cg0[0],
-                                   cg0[1], cg0[2], cg0[3]);
-                    if (two)
-                        pc_interaction(a, dist(s1.x, nd.x[0]), dist(s1.y, nd.x[1]), dist(s1.z, nd.x[2]), nd.m,
-                                       cg1[0], cg1[1], cg1[2], cg1[3]);
+                for (int q = stream; q < nm; q += 2) {
+                    const tpm_tree_node& nd = tr[sh.mlist[q]];
+#pragma unroll
+                    for (int k = 0; k < K; k++)
+                        pc_interaction(a, dist(s[k].x, nd.x[0]), dist(s[k].y, nd.x[1]), dist(s[k].z, nd.x[2]), nd.m,
+                                       cg[k][0], cg[k][1], cg[k][2], cg[k][3]);
                 }
                 npc += (unsigned long long)nm * nact;
                 nm = 0;
@@ -601,29 +608,26 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
                 while (q < np) {
                     int nbuf = 0;
                     while (q < np) {
-                        const tpm_tree_node& nd = tr[__builtin_amdgcn_readfirstlane(plist[q])];
+                        const tpm_tree_node& nd = tr[__builtin_amdgcn_readfirstlane(sh.plist[q])];
                         const int cnt = min(max(nd.end - nd.begin, 0), TREE_LEAF);
-                        if (nbuf + cnt > WALK_SBUF) break;
+                        if (nbuf + cnt > WALK_SBUF - 1) break;
                         if (lane < cnt && nd.begin + lane < a.n) {
                             const int4 v = a.pos[nd.begin + lane];
-                            sbuf[nbuf + lane] = make_int4((int)((uint32_t)v.x - bk.x[0]), (int)((uint32_t)v.y - bk.x[1]),
-                                                          (int)((uint32_t)v.z - bk.x[2]), 0);
+                            sh.sbuf[nbuf + lane] = make_int4((int)((uint32_t)v.x - bk.x[0]),
+                                                             (int)((uint32_t)v.y - bk.x[1]),
+                                                             (int)((uint32_t)v.z - bk.x[2]), 0);
                         }
                         nbuf += cnt;
                         q++;
                     }
+                    if (lane == 0 && (nbuf & 1)) sh.sbuf[nbuf] = far;
                     __syncthreads();
-                    if (two) {
-                        for (int j = 0; j < nbuf; j++) {
-                            const int4 s = sbuf[j];
-                            pp_pair<false>(B, ox0, oy0, oz0, s, pg0[0], pg0[1], pg0[2], pg0[3]);
-                            pp_pair<false>(B, ox1, oy1, oz1, s, pg1[0], pg1[1], pg1[2], pg1[3]);
-                        }
-                    } else {
-                        for (int j = 0; j < nbuf; j++) {
-                            const int4 s = sbuf[j];
-                            pp_pair<false>(B, ox0, oy0, oz0, s, pg0[0], pg0[1], pg0[2], pg0[3]);
-                        }
+                    const int padded = (nbuf + 1) & ~1;
+                    for (int j = stream; j < padded; j += 2) {
+                        const int4 sv = sh.sbuf[j];
+#pragma unroll
+                        for (int k = 0; k < K; k++)
+                            pp_pair<false>(B, ox[k], oy[k], oz[k], sv, pg[k][0], pg[k][1], pg[k][2], pg[k][3]);
                     }
                     npairs += (unsigned long long)nbuf * nact;
                     __syncthreads();
@@ -631,16 +635,19 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
                 np = 0;
             };
             int sp = 1, visited = 0;
-            if (lane == 0) stack[0] = 0;
+            if (lane == 0) sh.stack[0] = 0;
             __syncthreads();
             while (sp > 0) {
-                const int take = sp < 64 ? sp : 64;
+                // 64 nodes a round while the stack has room for their
+                // children; near the top one at a time, which grows the stack
+                // by at most the remaining tree depth (<= 50)
+                const int take = (sp + 128 <= WALK_STACK) ? (sp < 64 ? sp : 64) : 1;
                 visited += take;
                 if (visited > tn) {  // each node is met at most once
                     overflow = true;
                     break;
                 }
-                const int idx = lane < take ? stack[sp - 1 - lane] : 0;
+                const int idx = lane < take ? sh.stack[sp - 1 - lane] : 0;
                 const bool valid = lane < take && idx >= 0 && idx < tn;
                 __syncthreads();
                 sp -= take;
@@ -653,23 +660,23 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
                     const float R = sqrtf(R2);
                     const bool cutoff = R - nd.radius - sr > rcut;
                     const float rr = nd.radius + sr;
-                    const bool far = R2 > rr * rr * t2;
+                    const bool far_node = R2 > rr * rr * t2;
                     const bool leaf = nd.end - nd.begin <= TREE_LEAF;
-                    is_multi = !cutoff && far;
-                    is_open = !cutoff && !far && !leaf;
-                    is_leaf = !cutoff && !far && leaf;
+                    is_multi = !cutoff && far_node;
+                    is_open = !cutoff && !far_node && !leaf;
+                    is_leaf = !cutoff && !far_node && leaf;
                 }
                 // near internal nodes: children onto the stack
                 const unsigned long long mo = __ballot(is_open);
                 const int nopen = __popcll(mo);
-                if (sp + 2 * nopen > WALK_STACK) {
+                if (sp + 2 * nopen > WALK_STACK) {  // unreachable below the depth limit
                     overflow = true;
                     break;
                 }
                 if (is_open) {
                     const int at = sp + 2 * (int)count_below(mo);
-                    stack[at] = nd.child[1];
-                    stack[at + 1] = nd.child[0];
+                    sh.stack[at] = nd.child[1];
+                    sh.stack[at + 1] = nd.child[0];
                 }
                 sp += 2 * nopen;
                 // far nodes: the multipole list
@@ -678,7 +685,7 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
                     __syncthreads();
                     flush_pc();
                 }
-                if (is_multi) mlist[nm + (int)count_below(mm)] = idx;
+                if (is_multi) sh.mlist[nm + (int)count_below(mm)] = idx;
                 nm += __popcll(mm);
                 // near leaves: PC when every active sink is beyond r_src / theta
                 unsigned long long ml = __ballot(is_leaf);
@@ -690,26 +697,29 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
                                    lz = (uint32_t)__shfl((int)nd.x[2], j);
                     const float lr = __shfl(nd.radius, j);
                     const float lim = lr * lr * t2;
-                    const bool n0 = act0 && !(sqr3_f(dist(s0.x, lx), dist(s0.y, ly), dist(s0.z, lz)) > lim);
-                    const bool n1 = act1 && !(sqr3_f(dist(s1.x, lx), dist(s1.y, ly), dist(s1.z, lz)) > lim);
-                    if (__ballot(n0 || n1) != 0ull) {
+                    bool nearby = false;
+#pragma unroll
+                    for (int k = 0; k < K; k++)
+                        nearby |= act[k] && !(sqr3_f(dist(s[k].x, lx), dist(s[k].y, ly), dist(s[k].z, lz)) > lim);
+                    if (__ballot(nearby) != 0ull) {
                         if (np == WALK_LIST) {
                             __syncthreads();
                             flush_pp();
                         }
-                        if (lane == 0) plist[np] = lidx;
+                        if (lane == 0) sh.plist[np] = lidx;
                         np++;
                     } else {
                         if (nm == WALK_LIST) {
                             __syncthreads();
                             flush_pc();
                         }
-                        if (lane == 0) mlist[nm] = lidx;
+                        if (lane == 0) sh.mlist[nm] = lidx;
                         nm++;
                     }
                 }
                 __syncthreads();
             }
+            __syncthreads();
             flush_pc();
             flush_pp();
         }
@@ -722,17 +732,52 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
         atomicAdd(&a.o.counters[0], npairs);
         atomicAdd(&a.o.counters[3], npc);
     }
-    // short range in box units (PP sums are in 2^-64 / 2^-32), mesh, kick
-    auto finish = [&](const int4& s, int r, const float* pg, const float* cg) {
-        const float gx = pg[0] * 0x1p64f + cg[0], gy = pg[1] * 0x1p64f + cg[1], gz = pg[2] * 0x1p64f + cg[2];
-        const float ph = pg[3] * 0x1p32f + cg[3];
+    // the two streams' sums, then (stream 0) short range in box units, mesh, kick
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            pg[k][c] += __shfl_xor(pg[k][c], 32);
+            cg[k][c] += __shfl_xor(cg[k][c], 32);
+        }
+    if (stream != 0) return;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (!act[k]) continue;
+        const float gx = pg[k][0] * 0x1p64f + cg[k][0], gy = pg[k][1] * 0x1p64f + cg[k][1];
+        const float gz = pg[k][2] * 0x1p64f + cg[k][2], ph = pg[k][3] * 0x1p32f + cg[k][3];
         float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
         if (a.phi && !(a.flags & TPM_SKIP_PM))
-            mesh_interp6(a.phi, B.M, (uint32_t)s.x, (uint32_t)s.y, (uint32_t)s.z, pgx, pgy, pgz, pph);
-        kick_epilogue(B, a.flags, a.o, s.w, r, pgx, pgy, pgz, pph, gx, gy, gz, ph);
-    };
-    if (act0) finish(s0, r0, pg0, cg0);
-    if (act1) finish(s1, r1, pg1, cg1);
+            mesh_interp6(a.phi, B.M, (uint32_t)s[k].x, (uint32_t)s[k].y, (uint32_t)s[k].z, pgx, pgy, pgz, pph);
+        kick_epilogue(B, a.flags, a.o, s[k].w, r[k], pgx, pgy, pgz, pph, gx, gy, gz, ph);
+    }
+}
+
+__global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
+    __shared__ WalkShared sh;
+    // one contiguous run of buckets per XCD (workgroups are dealt round-robin)
+    const int wg = blockIdx.x;
+    const int k = wg >> 3;
+    if (k >= a.per_xcd) return;
+    const int bi = (wg & 7) * a.per_xcd + k;
+    if (bi >= a.nbuckets) return;
+    const tpm_tree_bucket bk = a.buckets[bi];
+    switch ((bk.end - bk.begin + 31) >> 5) {  // sinks per lane
+        case 1:
+            walk_bucket<1>(a, bk, sh);
+            break;
+        case 2:
+            walk_bucket<2>(a, bk, sh);
+            break;
+        case 3:
+            walk_bucket<3>(a, bk, sh);
+            break;
+        case 4:
+            walk_bucket<4>(a, bk, sh);
+            break;
+        default:  // an empty bucket (buckets hold <= 128 sinks)
+            break;
+    }
 }
 
 __global__ void tree_order_kernel(int64_t n, const int4* __restrict__ pos, int32_t* __restrict__ order) {
@@ -874,8 +919,7 @@ tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, in
     }
     TPM_HIP(ctx, hipStreamSynchronize(s));
     if (fl[1])
-        return tpm_fail(ctx, TPM_ERR_INVALID,
-                        "tpm_kick: tree walk stopped (more than 2048 pending nodes, or a node met twice)");
+        return tpm_fail(ctx, TPM_ERR_HIP, "tpm_kick: tree walk stopped (stack exhausted or a node met twice)");
     if (stats) {
         float ms = 0.f;
         TPM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
