@@ -163,6 +163,21 @@ def test_sparse_and_reference_boxes(gpu_ctx, n, params):
     assert rel_rms(G, np.stack([ox, oy, oz], 1)) < 1e-5
 
 
+def test_trees_follow_the_chain_mesh(gpu_ctx):
+    """Trees are reused while the chain mesh stands and rebuilt (buffers
+    grown) when a larger, different box replaces it in the same context."""
+    for kind, n_side, seed in (("uniform", 24, 31), ("clustered", 32, 32)):
+        p, xs, ys, zs, cb = setup(gpu_ctx, kind, n_side, seed)
+        print("rebuild", kind, n_side, flush=True)
+        a = gpu_ctx.kick(flags=T.TPM_KICK_TREE | T.TPM_SKIP_PM | T.TPM_SKIP_KICK)
+        b = gpu_ctx.kick(flags=T.TPM_KICK_TREE | T.TPM_SKIP_PM | T.TPM_SKIP_KICK)
+        assert (a.executed_pairs, a.pc_interactions) == (b.executed_pairs, b.pc_interactions)
+        g = gpu_ctx.tree_download()
+        o = O.tree_build(oparams(p), xs, ys, zs, cb)
+        assert g["nodes"].tobytes() == o["nodes"].tobytes()
+        assert np.array_equal(g["order"], o["order"])
+
+
 def test_depth_limit_is_an_error(gpu_ctx):
     n_side = 24
     n = n_side ** 3

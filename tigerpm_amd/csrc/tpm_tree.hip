@@ -592,12 +592,26 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
             int nm = 0, np = 0;
             // applies and empties the multipole list (each stream every other node)
             auto flush_pc = [&]() {
+                // the next node's centre and moments load while this one is applied
+                uint32_t nx = 0, ny = 0, nz = 0;
+                float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+                if (stream < nm) {
+                    const tpm_tree_node& nd = tr[sh.mlist[stream]];
+                    nx = nd.x[0], ny = nd.x[1], nz = nd.x[2];
+                    m0 = nd.m[0], m1 = nd.m[1], m2 = nd.m[2], m3 = nd.m[3];
+                }
                 for (int q = stream; q < nm; q += 2) {
-                    const tpm_tree_node& nd = tr[sh.mlist[q]];
+                    const uint32_t cx0 = nx, cy0 = ny, cz0 = nz;
+                    const float M[4] = {m0, m1, m2, m3};
+                    if (q + 2 < nm) {
+                        const tpm_tree_node& nn = tr[sh.mlist[q + 2]];
+                        nx = nn.x[0], ny = nn.x[1], nz = nn.x[2];
+                        m0 = nn.m[0], m1 = nn.m[1], m2 = nn.m[2], m3 = nn.m[3];
+                    }
 #pragma unroll
                     for (int k = 0; k < K; k++)
-                        pc_interaction(a, dist(s[k].x, nd.x[0]), dist(s[k].y, nd.x[1]), dist(s[k].z, nd.x[2]), nd.m,
-                                       cg[k][0], cg[k][1], cg[k][2], cg[k][3]);
+                        pc_interaction(a, dist(s[k].x, cx0), dist(s[k].y, cy0), dist(s[k].z, cz0), M, cg[k][0],
+                                       cg[k][1], cg[k][2], cg[k][3]);
                 }
                 npc += (unsigned long long)nm * nact;
                 nm = 0;
@@ -610,7 +624,7 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                     while (q < np) {
                         const tpm_tree_node& nd = tr[__builtin_amdgcn_readfirstlane(sh.plist[q])];
                         const int cnt = min(max(nd.end - nd.begin, 0), TREE_LEAF);
-                        if (nbuf + cnt > WALK_SBUF - 1) break;
+                        if (nbuf + cnt > WALK_SBUF - 3) break;
                         if (lane < cnt && nd.begin + lane < a.n) {
                             const int4 v = a.pos[nd.begin + lane];
                             sh.sbuf[nbuf + lane] = make_int4((int)((uint32_t)v.x - bk.x[0]),
@@ -623,11 +637,15 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                     if (lane == 0 && (nbuf & 1)) sh.sbuf[nbuf] = far;
                     __syncthreads();
                     const int padded = (nbuf + 1) & ~1;
+                    // one staged source ahead (the read past the end stays below
+                    // padded + 2 <= WALK_SBUF and is not used)
+                    int4 sv = sh.sbuf[stream];
                     for (int j = stream; j < padded; j += 2) {
-                        const int4 sv = sh.sbuf[j];
+                        const int4 sn = sh.sbuf[j + 2];
 #pragma unroll
                         for (int k = 0; k < K; k++)
                             pp_pair<false>(B, ox[k], oy[k], oz[k], sv, pg[k][0], pg[k][1], pg[k][2], pg[k][3]);
+                        sv = sn;
                     }
                     npairs += (unsigned long long)nbuf * nact;
                     __syncthreads();
@@ -811,11 +829,16 @@ tpm_status tpm_tree_ensure(tpm_ctx* ctx) {
     const int C = ctx->C;
     const int64_t ncell = (int64_t)C * C * C;
     const int64_t n = ctx->n;
-    int64_t cap = ctx->tree_pos_cap;
-    TPM_TRY(grow(ctx, &ctx->tree_pos, &cap, n));
-    cap = ctx->tree_pos_cap;
-    TPM_TRY(grow(ctx, &ctx->tree_tmp, &cap, n));
-    ctx->tree_pos_cap = cap;
+    if (n > ctx->tree_pos_cap || !ctx->tree_pos || !ctx->tree_tmp) {
+        int64_t c1 = 0, c2 = 0;
+        if (ctx->tree_pos) TPM_HIP(ctx, hipFree(ctx->tree_pos));
+        if (ctx->tree_tmp) TPM_HIP(ctx, hipFree(ctx->tree_tmp));
+        ctx->tree_pos = ctx->tree_tmp = nullptr;
+        ctx->tree_pos_cap = 0;
+        TPM_TRY(grow(ctx, &ctx->tree_pos, &c1, n));
+        TPM_TRY(grow(ctx, &ctx->tree_tmp, &c2, n));
+        ctx->tree_pos_cap = n;
+    }
     if (ncell > ctx->tree_cells_cap || !ctx->tree_cnt) {
         int64_t c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
         if (ctx->tree_cnt) TPM_HIP(ctx, hipFree(ctx->tree_cnt));
