@@ -178,6 +178,46 @@ def test_trees_follow_the_chain_mesh(gpu_ctx):
         assert np.array_equal(g["order"], o["order"])
 
 
+def test_walk_on_a_displaced_lattice(gpu_ctx):
+    """Near-lattice particles (Zel'dovich, 0.5 cell rms), where the net
+    forces are small and the multipole errors weigh most: the GPU walk still
+    equals the oracle's to FP32 rounding."""
+    n_side = 40
+    p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, 3))
+    gpu_ctx.ic_zeldovich(n_side, 7, 0.5, -2.0)
+    gpu_ctx.chainmesh_create()
+    d0 = gpu_ctx.download()
+    xs, ys, zs = d0["x"], d0["y"], d0["z"]
+    cb = gpu_ctx.chainmesh_cells()
+    print("lattice walk", flush=True)
+    st = gpu_ctx.kick(flags=T.TPM_KICK_TREE | T.TPM_SKIP_PM | T.TPM_SKIP_KICK)
+    d = gpu_ctx.download()
+    n = len(xs)
+    (ox, oy, oz, oph), ost = O.tree_sum(oparams(p), xs, ys, zs, cb, np.zeros(n, np.int8))
+    assert st.pc_interactions == ost["pc"] and st.executed_pairs == ost["pp"]
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1) / p.GM
+    assert rel_rms(G, np.stack([ox, oy, oz], 1)) < 1e-5
+
+
+def test_tree_api_errors(gpu_ctx):
+    """Out-of-order calls and conflicting flags are errors, not results."""
+    n_side = 24
+    x, y, z = positions("uniform", n_side ** 3, 2)
+    gpu_ctx.set_params(T.baseline_params(n_side, n_side, 4))
+    gpu_ctx.upload(x, y, z)
+    with pytest.raises(T.TpmError, match="chain mesh"):
+        gpu_ctx.tree_build()
+    with pytest.raises(T.TpmError, match="not built"):
+        gpu_ctx.tree_download()
+    gpu_ctx.chainmesh_create()
+    with pytest.raises(T.TpmError, match="exclude"):
+        gpu_ctx.kick(flags=T.TPM_KICK_TREE | T.TPM_KICK_PME | T.TPM_SKIP_PM)
+    gpu_ctx.tree_build()
+    gpu_ctx.upload(x, y, z)  # new particles: the trees no longer apply
+    with pytest.raises(T.TpmError, match="not built"):
+        gpu_ctx.tree_download()
+
+
 def test_depth_limit_is_an_error(gpu_ctx):
     n_side = 24
     n = n_side ** 3

@@ -544,29 +544,34 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
     const int lane = (int)lane_id();
     const int slot = lane & 31, stream = lane >> 5;
     const int ns = bk.end - bk.begin;
-    int4 s[K];
-    int r[K], ox[K], oy[K], oz[K];
-    bool act[K];
+    // per sink slot: offsets from the bucket centre (exact int32, as the
+    // direct kernel's; the position is the centre plus the offset) and the
+    // chain-mesh index; active slots as a bit mask
+    int ox[K], oy[K], oz[K], w[K];
+    unsigned act = 0u;
     float pg[K][4], cg[K][4];  // PP sums (2^-64 / 2^-32 units), PC sums (box units)
     int nact = 0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         const int j = slot + 32 * k;
-        s[k] = make_int4((int)bk.x[0], (int)bk.x[1], (int)bk.x[2], -1);
-        r[k] = 0;
+        ox[k] = oy[k] = oz[k] = 0;
+        w[k] = -1;
+        bool on = false;
         if (j < ns) {
-            s[k] = a.pos[bk.begin + j];
-            r[k] = a.o.rung[s[k].w];
+            const int4 v = a.pos[bk.begin + j];
+            ox[k] = (int)((uint32_t)v.x - bk.x[0]);
+            oy[k] = (int)((uint32_t)v.y - bk.x[1]);
+            oz[k] = (int)((uint32_t)v.z - bk.x[2]);
+            w[k] = v.w;
+            on = a.o.rung[v.w] >= B.min_rung;
         }
-        act[k] = j < ns && r[k] >= B.min_rung;
-        nact += __popcll(__ballot(act[k] && stream == 0));
-        // offsets from the bucket centre (exact int32, as the direct kernel's)
-        ox[k] = (int)((uint32_t)s[k].x - bk.x[0]);
-        oy[k] = (int)((uint32_t)s[k].y - bk.x[1]);
-        oz[k] = (int)((uint32_t)s[k].z - bk.x[2]);
+        act |= on ? 1u << k : 0u;
+        nact += __popcll(__ballot(on && stream == 0));
 #pragma unroll
         for (int c = 0; c < 4; c++) pg[k][c] = cg[k][c] = 0.f;
     }
+    // distance() from slot k's sink to a point: the offset plus the centre's
+    auto sdist = [&](int o, uint32_t centre, uint32_t to) { return dist((uint32_t)o + centre, to); };
     if (nact == 0) return;
     unsigned long long npairs = 0, npc = 0;
     bool overflow = false;
@@ -610,7 +615,8 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                     }
 #pragma unroll
                     for (int k = 0; k < K; k++)
-                        pc_interaction(a, dist(s[k].x, cx0), dist(s[k].y, cy0), dist(s[k].z, cz0), M, cg[k][0],
+                        pc_interaction(a, sdist(ox[k], bk.x[0], cx0), sdist(oy[k], bk.x[1], cy0),
+                                       sdist(oz[k], bk.x[2], cz0), M, cg[k][0],
                                        cg[k][1], cg[k][2], cg[k][3]);
                 }
                 npc += (unsigned long long)nm * nact;
@@ -718,7 +724,9 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                     bool nearby = false;
 #pragma unroll
                     for (int k = 0; k < K; k++)
-                        nearby |= act[k] && !(sqr3_f(dist(s[k].x, lx), dist(s[k].y, ly), dist(s[k].z, lz)) > lim);
+                        nearby |= ((act >> k) & 1u) &&
+                                  !(sqr3_f(sdist(ox[k], bk.x[0], lx), sdist(oy[k], bk.x[1], ly),
+                                           sdist(oz[k], bk.x[2], lz)) > lim);
                     if (__ballot(nearby) != 0ull) {
                         if (np == WALK_LIST) {
                             __syncthreads();
@@ -761,13 +769,14 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
     if (stream != 0) return;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        if (!act[k]) continue;
+        if (!((act >> k) & 1u)) continue;
         const float gx = pg[k][0] * 0x1p64f + cg[k][0], gy = pg[k][1] * 0x1p64f + cg[k][1];
         const float gz = pg[k][2] * 0x1p64f + cg[k][2], ph = pg[k][3] * 0x1p32f + cg[k][3];
         float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
         if (a.phi && !(a.flags & TPM_SKIP_PM))
-            mesh_interp6(a.phi, B.M, (uint32_t)s[k].x, (uint32_t)s[k].y, (uint32_t)s[k].z, pgx, pgy, pgz, pph);
-        kick_epilogue(B, a.flags, a.o, s[k].w, r[k], pgx, pgy, pgz, pph, gx, gy, gz, ph);
+            mesh_interp6(a.phi, B.M, (uint32_t)ox[k] + bk.x[0], (uint32_t)oy[k] + bk.x[1],
+                         (uint32_t)oz[k] + bk.x[2], pgx, pgy, pgz, pph);
+        kick_epilogue(B, a.flags, a.o, w[k], a.o.rung[w[k]], pgx, pgy, pgz, pph, gx, gy, gz, ph);
     }
 }
 
