@@ -319,3 +319,19 @@ def test_kick_pme_law_vs_oracle(gpu_ctx, flags_extra):
         assert rel_rms(d["pot"][act], r["pot"][act]) < 1e-5
         V = np.stack([d["vx"], d["vy"], d["vz"]], 1)
         assert rel_rms(V[act], np.stack([r["vx"], r["vy"], r["vz"]], 1)[act]) < 1e-5
+
+
+def test_kick_is_deterministic(gpu_ctx):
+    """Two whole steps from the same particles give the same bits: the
+    deposit is integer, the PP sums have a fixed order, nothing is summed by
+    float atomics."""
+    n_side = 32
+    x, y, z = clustered(n_side ** 3, 41)
+    out = []
+    for _ in range(2):
+        gpu_ctx.set_params(T.baseline_params(n_side, n_side, 3))
+        gpu_ctx.upload(x, y, z)
+        gpu_ctx.kick_treepm(0, 1.0, 1.0, True)
+        out.append(gpu_ctx.download())
+    for k in ("x", "gx", "gy", "gz", "pot", "vx", "rung"):
+        assert np.array_equal(out[0][k], out[1][k]), k
