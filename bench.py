@@ -355,8 +355,8 @@ def main():
                     help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported (multi-GPU sharding is "
-              "described in DESIGN.md, not implemented)" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU only; --gpus %d is not supported (sharding a box over several GPUs "
+              "is designed but not implemented)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     if args.only == "ensemble":

@@ -221,7 +221,7 @@ TPM_API tpm_status tpm_kick_treepm(tpm_ctx* ctx, int min_rung, double scale, dou
 TPM_API tpm_status tpm_profile(tpm_ctx* ctx, int enable);
 TPM_API tpm_status tpm_profile_read(tpm_ctx* ctx, double* ms, int64_t* pp_launches, int64_t* steps);
 
-/* FP64 Ewald judge (gravity_short_ewald_call_kernel, gravity_short.cu:10-91,
+/* FP64 Ewald check (gravity_short_ewald_call_kernel, gravity_short.cu:10-91,
  * reformulated): for each sample (an index into the current particle
  * order) the periodic potential and force of all resident unit masses,
  * softened inside hsoft, excluding the sample itself, plus its own image /
@@ -233,7 +233,7 @@ TPM_API tpm_status tpm_ewald(tpm_ctx* ctx, int64_t nsamples, const int64_t* samp
  * lattice (i+1/2)/n_side displaced by a Gaussian field with power
  * P(k) ~ k^spectral_index exp(-(k/k_Nyq)^2), scaled so that the rms 3-D
  * displacement is rms_disp lattice cells.  Velocities and rungs zero.
- * Replaces initialize() (initialize.cpp:91-221) for the build's workloads. */
+ * Replaces initialize() (initialize.cpp:91-221) for the benchmark workloads. */
 TPM_API tpm_status tpm_ic_zeldovich(tpm_ctx* ctx, int n_side, uint64_t seed, double rms_disp, double spectral_index);
 /* n uniform random positions from a counter-based RNG (seeded). */
 TPM_API tpm_status tpm_ic_uniform(tpm_ctx* ctx, int64_t n, uint64_t seed);

@@ -92,7 +92,7 @@ struct options {
     double rs = 0.0;
     std::string config_file;
     std::string test;
-    // Workload extensions of this build (defaults reproduce the reference):
+    // Workload extensions (defaults reproduce the reference):
     double rs_cells = 0.0;   // split scale in mesh cells; 0 = reference (1 / four_dim)
     double rcut_rs = 0.0;    // cutoff in units of rs; 0 = reference (1 / chain_dim)
     double rcut = 0.0;       // derived

@@ -25,7 +25,7 @@ typedef struct {
     int chain_dim;  /* C: chain-mesh cells per side (reference chain_dim)        */
     int order;      /* mass assignment: 2 CIC, 3 TSC, 4 cloud4 (cubic B-spline)  */
     int do_phi;     /* accumulate the potential (reference do_phi)               */
-    int interlace;  /* second deposit on the half-cell-shifted grid (this build) */
+    int interlace;  /* second deposit on the half-cell-shifted grid (not in the reference) */
     int pme;        /* 1: kick_pme's pair law (no cutoff, plain softening, no phi0) */
     double rs;      /* split scale, box units                                    */
     double rcut;    /* short-range cutoff, box units                             */
@@ -77,7 +77,7 @@ ORC_API void orc_chain_bounds(int C, uint64_t* b) {
 /* Chain cell coordinate along one axis: the k with b[k] <= raw < b[k+1],
  * which is where the reference's partition (particles.cpp:414-445, left iff
  * x < xmid) leaves the particle.  Also returns the 2-bit sub-cell used to
- * order particles inside a cell (this build's deterministic in-cell order). */
+ * order particles inside a cell (the device sort's deterministic in-cell order). */
 static inline int chain_coord(uint32_t raw, int C, const uint64_t* b, int* sub) {
     const uint64_t P = (uint64_t)raw * (uint64_t)C;
     int k = (int)(P >> 32);
@@ -452,7 +452,7 @@ ORC_API void orc_fft3d_r2c(int M, const double* in, double* out) {
 }
 
 /* Unnormalized complex-to-real inverse (fft.cpp:69-84 without the 1/N,
- * which this build folds into the Green's table). */
+ * which the device code folds into the Green's table). */
 ORC_API void orc_fft3d_c2r(int M, const double* in, double* out) {
     const int64_t M3 = (int64_t)M * M * M;
     const int Mh = M / 2 + 1;
@@ -1646,7 +1646,7 @@ ORC_API void orc_kick_full(const orc_params* p, int64_t n, uint32_t* x, uint32_t
 }
 
 /* ------------------------------------------------------------------------ */
-/* a14: FP64 Ewald sum (the accuracy judge)                                  */
+/* a14: FP64 Ewald sum (the accuracy reference)                              */
 /* ------------------------------------------------------------------------ */
 
 /* Periodic potential of a unit mass in the unit box with a neutralizing
@@ -1723,7 +1723,7 @@ static inline void soft_poly(double r2, double h, double* d0, double* d1) {
     *d0 = (((-5.0 / 16.0 * q2 + 21.0 / 16.0) * q2 - 35.0 / 16.0) * q2 + 35.0 / 16.0) / h;
 }
 
-/* Brute-force Ewald judge for small N (pair by pair with ewald_psi, alpha 2,
+/* Brute-force Ewald sum for small N (pair by pair with ewald_psi, alpha 2,
  * images |n| <= 4, |h| <= 4): for each sample particle s, phi = sum_{j != s}
  * psi_soft(x_s - x_j) + xi, g = -grad phi, unit masses.  psi_soft is psi with
  * the Newtonian -1/r replaced by the softened polynomial for r < hsoft;
@@ -1768,7 +1768,7 @@ ORC_API void orc_ewald_direct(int64_t n, const uint32_t* x, const uint32_t* y, c
     }
 }
 
-/* Structure-factor Ewald judge (the algorithm the GPU judge runs): real
+/* Structure-factor Ewald sum (the algorithm the GPU's tpm_ewald runs): real
  * space on the minimum image only (rc = 4.8/alpha < 1/2, erfc(4.8) ~ 6e-12),
  * reciprocal space through S(h) = sum_j exp(2 pi i h.x_j) over the half
  * space 0 < |h| <= 5.2 alpha/pi.  Same outputs and conventions as

@@ -1,4 +1,4 @@
-// tpm_ewald.hip -- FP64 Ewald sum at sampled particles (the accuracy judge).
+// tpm_ewald.hip -- FP64 Ewald sum at sampled particles (the accuracy reference).
 //
 // Reference: gravity_ewald_kernel (gravity_short.cu:93-201), one block per
 // sample over all sources, FP32 per-pair Ewald sums with alpha = 2 (real

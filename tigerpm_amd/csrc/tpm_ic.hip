@@ -2,8 +2,8 @@
 //
 // Reference: initialize() (initialize.cpp:91-221) draws a Gaussian random
 // field from a tabulated P(k), builds the Zel'dovich displacement per axis
-// through inverse FFTs and displaces a particle lattice.  The workloads of
-// this build use a power law instead:
+// through inverse FFTs and displaces a particle lattice.  The benchmark
+// workloads use a power law instead:
 //   P(k) ~ k^n exp(-(k / k_Nyq)^2),
 // white noise from a counter-based hash (splitmix64 + Box-Muller, so a seed
 // fixes the field on any device), displacement psi_hat = i k / k^2 delta_hat,

@@ -434,7 +434,7 @@ static std::vector<int64_t> sample_indices(int nsamples) {
 
 std::pair<std::vector<double>, std::array<std::vector<double>, NDIM>> gravity_short_ewald_call_kernel(
     const std::vector<fixed32>& sinkx, const std::vector<fixed32>& sinky, const std::vector<fixed32>& sinkz) {
-    // The judge evaluates at particles (excluding the particle itself):
+    // The Ewald check evaluates at particles (excluding the particle itself):
     // find each sink among the resident particles.
     std::unordered_multimap<uint64_t, int64_t> where;
     for (int i = 0; i < particles_size(); i++)
