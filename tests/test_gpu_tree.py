@@ -163,6 +163,46 @@ def test_sparse_and_reference_boxes(gpu_ctx, n, params):
     assert rel_rms(G, np.stack([ox, oy, oz], 1)) < 1e-5
 
 
+def test_crowded_cell_and_deep_tree(gpu_ctx):
+    """A cell of 920 particles (built from device memory instead of LDS) and
+    40 particles within 2^16 raw units of each other (a tree 40 levels
+    deep): trees bit-identical, walk equal to the oracle's.  The softening
+    length is cut to 1e-7: with the default one the close pairs sit where
+    the softened (erfc - 1) term of kick_treepm.cu:263-276 cancels to FP32
+    rounding noise (r < ~4e-3 h), which no two implementations share."""
+    n_side = 24
+    rng = np.random.default_rng(5)
+    x, y, z = [rng.integers(0, 2 ** 32, n_side ** 3, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    cl = (np.array([0.3, 0.6, 0.1]) + rng.standard_normal((700, 3)) * 0.01) % 1.0
+    cl = np.minimum(np.floor(cl * 2.0 ** 32), 2.0 ** 32 - 1).astype(np.uint32)
+    tight = (np.array([0.71, 0.42, 0.83]) * 2.0 ** 32).astype(np.uint64)
+    tight = (tight + rng.integers(0, 2 ** 16, (40, 3), dtype=np.uint64)).astype(np.uint32)
+    x, y, z = [np.concatenate([a, cl[:, d], tight[:, d]]) for d, a in enumerate((x, y, z))]
+    n = len(x)
+    p = T.baseline_params(n_side, n_side, 4, n_total=n)
+    p.hsoft = 1e-7
+    p = gpu_ctx.set_params(p)
+    gpu_ctx.upload(x, y, z)
+    gpu_ctx.chainmesh_create()
+    operm, cb = O.chainmesh_sort(x, y, z, p.chain_dim)
+    assert np.diff(cb).max() > 512
+    xs, ys, zs = x[operm], y[operm], z[operm]
+    print("crowded/deep", flush=True)
+    gpu_ctx.tree_build()
+    g = gpu_ctx.tree_download()
+    o = O.tree_build(oparams(p), xs, ys, zs, cb)
+    assert np.array_equal(g["order"], o["order"])
+    assert g["nodes"].tobytes() == o["nodes"].tobytes()
+    for f in ("begin", "end", "x", "radius"):
+        assert np.array_equal(g["buckets"][f], o["buckets"][f]), f
+    st = gpu_ctx.kick(flags=T.TPM_KICK_TREE | T.TPM_SKIP_PM | T.TPM_SKIP_KICK)
+    d = gpu_ctx.download()
+    (ox, oy, oz, oph), ost = O.tree_sum(oparams(p), xs, ys, zs, cb, np.zeros(n, np.int8))
+    assert st.pc_interactions == ost["pc"] and st.executed_pairs == ost["pp"]
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1) / p.GM
+    assert rel_rms(G, np.stack([ox, oy, oz], 1)) < 1e-5
+
+
 def test_trees_follow_the_chain_mesh(gpu_ctx):
     """Trees are reused while the chain mesh stands and rebuilt (buffers
     grown) when a larger, different box replaces it in the same context."""

@@ -40,7 +40,9 @@
 #define TREE_LEAF 32     // min(SOURCE_BUCKET_SIZE, SINK_BUCKET_SIZE), tigerpm.hpp:32-33
 #define TREE_BUCKET 128  // SINK_BUCKET_SIZE
 #define TREE_DEPTH 50    // MAX_DEPTH, tree.cpp:5
-#define TREE_STACK 64    // build frames: a depth-first walk holds <= depth + 2
+#define TREE_STACK 64    // partition frames: a depth-first walk holds <= depth + 2
+#define NODE_STACK (2 * TREE_DEPTH + 4)  // numbering frames: the path and its pending right siblings
+#define TREE_LDS_CAP 512  // cells up to this many particles are built in LDS
 #define WALK_STACK 1024  // walk: pending nodes of one neighbour tree
 #define WALK_LIST 128    // multipole / leaf list entries between flushes
 #define WALK_SBUF 256    // staged PP sources
@@ -63,9 +65,14 @@ __device__ __forceinline__ uint32_t comp(int4 v, int d) { return (uint32_t)(d ==
 // distance() (gravity_short.hpp:20-22) in box units.
 __device__ __forceinline__ float dist(uint32_t a, uint32_t b) { return (float)(int)(a - b) * 0x1p-32f; }
 
-__device__ __forceinline__ double wave_dmin(double v) {
+__device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = dmin(v, __shfl_xor(v, o));
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
     return v;
 }
 __device__ __forceinline__ double wave_dmax(double v) {
@@ -78,6 +85,7 @@ __device__ __forceinline__ double wave_dmax(double v) {
 __device__ __forceinline__ int split_dim(const double* b, const double* e) {
     int dim = 0;
     double span = 0.0;
+#pragma unroll
     for (int d = 0; d < 3; d++)
         if (e[d] - b[d] > span) {
             span = e[d] - b[d];
@@ -91,18 +99,53 @@ struct PartFrame {
     int begin, end, depth, sunk;
 };
 
-__global__ __launch_bounds__(64) void tree_partition_kernel(int C, const int32_t* __restrict__ cell_begin,
-                                                            const uint32_t* __restrict__ x,
-                                                            const uint32_t* __restrict__ y,
-                                                            const uint32_t* __restrict__ z, int4* __restrict__ pos,
-                                                            int4* __restrict__ tmp, int32_t* __restrict__ cnt,
-                                                            int32_t* __restrict__ flags) {
-    __shared__ PartFrame st[TREE_STACK];
-    const int c = blockIdx.x;
-    const int ncell = C * C * C;
+// Where a cell's records live while it is split.  A split at depth d reads
+// buffer d & 1 and writes buffer (d + 1) & 1; a leaf leaves its particles in
+// `pos` in tree order.
+//   PartLds: cells of <= TREE_LDS_CAP particles.  Positions are read once
+//     into LDS and the splits move 16-bit local indices; each leaf writes its
+//     records (x, y, z, chain-mesh index) to `pos` once.
+//   PartGlobal: larger cells; int4 records move between pos and tmp.
+struct PartLds {
+    const uint32_t *xs, *ys, *zs;
+    uint16_t *ia, *ib;
+    int b0;
+    int4* pos;
+    using rec = int;
+    __device__ rec load(int d, int i) const { return ((d & 1) ? ib : ia)[i - b0]; }
+    __device__ uint32_t coord(rec k, int dim) const { return dim == 0 ? xs[k] : (dim == 1 ? ys[k] : zs[k]); }
+    __device__ void store(int d, int i, rec k) const { ((d & 1) ? ib : ia)[i - b0] = (uint16_t)k; }
+    __device__ void leaf(int d, int begin, int end) const {
+        const uint16_t* buf = (d & 1) ? ib : ia;
+        for (int i = begin + (int)lane_id(); i < end; i += 64) {
+            const int k = buf[i - b0];
+            pos[i] = make_int4((int)xs[k], (int)ys[k], (int)zs[k], b0 + k);
+        }
+    }
+};
+struct PartGlobal {
+    int4 *pos, *tmp;
+    using rec = int4;
+    // (a pointer chosen by d & 1 would be looked up in scratch memory)
+    __device__ rec load(int d, int i) const { return (d & 1) ? tmp[i] : pos[i]; }
+    __device__ uint32_t coord(rec v, int dim) const { return comp(v, dim); }
+    __device__ void store(int d, int i, rec v) const {
+        if (d & 1)
+            tmp[i] = v;
+        else
+            pos[i] = v;
+    }
+    __device__ void leaf(int d, int begin, int end) const {
+        if (d & 1)
+            for (int i = begin + (int)lane_id(); i < end; i += 64) pos[i] = tmp[i];
+    }
+};
+
+// tree.cpp's sort() for one cell as a depth-first walk; returns the node
+// and sink-bucket counts, or false past the depth limit.
+template <class S>
+__device__ __forceinline__ bool partition_cell(const S s, PartFrame* st, int c, int C, int b0, int e0, int& nodes, int& buckets) {
     const int lane = (int)lane_id();
-    const int b0 = cell_begin[c], e0 = cell_begin[c + 1];
-    for (int i = b0 + lane; i < e0; i += 64) pos[i] = make_int4((int)x[i], (int)y[i], (int)z[i], i);
     if (lane == 0) {
         // tree_create (tree.cpp:216-245): the cell's box [i / C, (i + 1) / C)
         const double ninv = 1.0 / (double)C;
@@ -119,91 +162,147 @@ __global__ __launch_bounds__(64) void tree_partition_kernel(int C, const int32_t
         st[0] = f;
     }
     __syncthreads();
-    int sp = 1, nodes = 0, buckets = 0;
-    bool bad = false;
+    int sp = 1;
+    nodes = buckets = 0;
     while (sp > 0) {
-        const PartFrame f = st[--sp];
-        __syncthreads();
-        if (f.depth > TREE_DEPTH) {  // tree.cpp:16-19 aborts
-            bad = true;
-            break;
+        --sp;
+        PartFrame f;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            f.b[d] = st[sp].b[d];
+            f.e[d] = st[sp].e[d];
         }
+        f.begin = st[sp].begin;
+        f.end = st[sp].end;
+        f.depth = st[sp].depth;
+        f.sunk = st[sp].sunk;
+        const int dim = split_dim(f.b, f.e);
+        const double mid = (st[sp].e[dim] + st[sp].b[dim]) / 2.0;  // from LDS: f.b[dim] would go to scratch
+        __syncthreads();
+        if (f.depth > TREE_DEPTH) return false;  // tree.cpp:16-19 aborts
         nodes++;
         const int n = f.end - f.begin;
         if (n <= TREE_BUCKET && !f.sunk) buckets++;
         if (n <= TREE_LEAF) {
-            // a leaf at odd depth was last written to tmp
-            if (f.depth & 1)
-                for (int i = f.begin + lane; i < f.end; i += 64) pos[i] = tmp[i];
+            s.leaf(f.depth, f.begin, f.end);
             continue;
         }
-        const int dim = split_dim(f.b, f.e);
-        const double mid = (f.e[dim] + f.b[dim]) / 2.0;
         const uint32_t thr = tree_fixed(mid);
-        const int4* src = (f.depth & 1) ? tmp : pos;
-        int4* dst = (f.depth & 1) ? pos : tmp;
         // particles_sort (particles.cpp:414-445): x < xmid goes left; here
         // each side keeps its previous order (a stable partition)
         int nl = 0;
         for (int i0 = f.begin; i0 < f.end; i0 += 64) {
             const int i = i0 + lane;
-            const bool left = i < f.end && comp(src[i], dim) < thr;
+            const bool left = i < f.end && s.coord(s.load(f.depth, i), dim) < thr;
             nl += __popcll(__ballot(left));
         }
         int l = 0, r = 0;
         for (int i0 = f.begin; i0 < f.end; i0 += 64) {
             const int i = i0 + lane;
             const bool valid = i < f.end;
-            const int4 v = valid ? src[i] : make_int4(0, 0, 0, 0);
-            const bool left = valid && comp(v, dim) < thr;
+            typename S::rec v{};
+            if (valid) v = s.load(f.depth, i);
+            const bool left = valid && s.coord(v, dim) < thr;
             const unsigned long long ml = __ballot(left), mr = __ballot(valid && !left);
-            if (valid) dst[left ? f.begin + l + (int)count_below(ml) : f.begin + nl + r + (int)count_below(mr)] = v;
+            if (valid)
+                s.store(f.depth + 1,
+                        left ? f.begin + l + (int)count_below(ml) : f.begin + nl + r + (int)count_below(mr), v);
             l += __popcll(ml);
             r += __popcll(mr);
         }
-        if (sp + 2 > TREE_STACK) {  // unreachable below the depth limit
-            bad = true;
-            break;
-        }
+        if (sp + 2 > TREE_STACK) return false;  // unreachable below the depth limit
         if (lane == 0) {
-            PartFrame L = f, R = f;
-            L.e[dim] = mid;
+            PartFrame& R = st[sp];
+            PartFrame& L = st[sp + 1];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                L.b[d] = R.b[d] = f.b[d];
+                L.e[d] = R.e[d] = f.e[d];
+            }
+            L.e[dim] = mid;  // LDS, indexed at run time
             R.b[dim] = mid;
+            L.begin = f.begin;
+            R.end = f.end;
             L.end = R.begin = f.begin + nl;
             L.depth = R.depth = f.depth + 1;
             L.sunk = R.sunk = n <= TREE_BUCKET;
-            st[sp] = R;
-            st[sp + 1] = L;
         }
         sp += 2;
         __syncthreads();
     }
+    return true;
+}
+
+__global__ __launch_bounds__(64) void tree_partition_kernel(int C, const int32_t* __restrict__ cell_begin,
+                                                            const uint32_t* __restrict__ x,
+                                                            const uint32_t* __restrict__ y,
+                                                            const uint32_t* __restrict__ z, int4* __restrict__ pos,
+                                                            int4* __restrict__ tmp, int32_t* __restrict__ cnt,
+                                                            int32_t* __restrict__ flags) {
+    __shared__ PartFrame st[TREE_STACK];
+    __shared__ uint32_t xs[TREE_LDS_CAP], ys[TREE_LDS_CAP], zs[TREE_LDS_CAP];
+    __shared__ uint16_t ia[TREE_LDS_CAP], ib[TREE_LDS_CAP];
+    const int c = blockIdx.x;
+    const int ncell = C * C * C;
+    const int lane = (int)lane_id();
+    const int b0 = cell_begin[c], e0 = cell_begin[c + 1];
+    int nodes = 0, buckets = 0;
+    bool ok;
+    if (e0 - b0 <= TREE_LDS_CAP) {
+        for (int k = lane; k < e0 - b0; k += 64) {
+            xs[k] = x[b0 + k];
+            ys[k] = y[b0 + k];
+            zs[k] = z[b0 + k];
+            ia[k] = (uint16_t)k;
+        }
+        ok = partition_cell(PartLds{xs, ys, zs, ia, ib, b0, pos}, st, c, C, b0, e0, nodes, buckets);
+    } else {
+        for (int i = b0 + lane; i < e0; i += 64) pos[i] = make_int4((int)x[i], (int)y[i], (int)z[i], i);
+        ok = partition_cell(PartGlobal{pos, tmp}, st, c, C, b0, e0, nodes, buckets);
+    }
     if (lane == 0) {
         cnt[c] = nodes;
         cnt[ncell + c] = buckets;
-        if (bad) atomicOr(&flags[0], 1);
+        if (!ok) atomicOr(&flags[0], 1);
     }
 }
+
+// Read access to a cell's partitioned positions, pos[i] for absolute i:
+// from LDS for cells of <= TREE_LDS_CAP particles, else from the records.
+struct PosLds {
+    const uint32_t *xs, *ys, *zs;
+    int b0;
+    __device__ int4 operator[](int i) const {
+        const int k = i - b0;
+        return make_int4((int)xs[k], (int)ys[k], (int)zs[k], 0);
+    }
+};
+struct PosGlobal {
+    const int4* __restrict__ p;
+    __device__ int4 operator[](int i) const { return p[i]; }
+};
 
 // Centre of the bounding box of [begin, end) (the box centre when empty) and
 // the largest squared distance of a particle from it (tree.cpp:25-59,
 // 88-99); every lane returns the same values.
-__device__ double tree_bbox(const int4* __restrict__ pos, int begin, int end, const double* bb, const double* be,
-                            double* xc) {
+template <class P>
+__device__ double tree_bbox(const P& pos, int begin, int end, const double* bb, const double* be, double* xc) {
+    // min and max in fixed point: raw -> double is exact and monotonic, so
+    // these are the reference's FP64 extremes
     const int lane = (int)lane_id();
-    double mn[3] = {1.0, 1.0, 1.0}, mx[3] = {0.0, 0.0, 0.0};
+    uint32_t mn[3] = {~0u, ~0u, ~0u}, mx[3] = {0u, 0u, 0u};
     for (int i = begin + lane; i < end; i += 64) {
         const int4 v = pos[i];
+#pragma unroll
         for (int d = 0; d < 3; d++) {
-            const double w = raw_to_double(comp(v, d));
-            mx[d] = dmax(mx[d], w);
-            mn[d] = dmin(mn[d], w);
+            mx[d] = max(mx[d], comp(v, d));
+            mn[d] = min(mn[d], comp(v, d));
         }
     }
+#pragma unroll
     for (int d = 0; d < 3; d++) {
-        mn[d] = wave_dmin(mn[d]);
-        mx[d] = wave_dmax(mx[d]);
-        xc[d] = (end > begin) ? (mx[d] + mn[d]) * 0.5 : (bb[d] + be[d]) * 0.5;
+        const double lo = raw_to_double(wave_umin(mn[d])), hi = raw_to_double(wave_umax(mx[d]));
+        xc[d] = (end > begin) ? (hi + lo) * 0.5 : (bb[d] + be[d]) * 0.5;
     }
     double r2max = 0.0;
     for (int i = begin + lane; i < end; i += 64) {
@@ -220,8 +319,9 @@ __device__ double tree_bbox(const int4* __restrict__ pos, int begin, int end, co
 
 // A leaf of <= 32 particles (tree.cpp:24-100): lane k holds particle k; the
 // moments {N, -sum dx} are summed in particle order.
-__device__ tpm_tree_node tree_leaf(const int4* __restrict__ pos, int begin, int end, const double* bb,
-                                   const double* be, double hsoft) {
+template <class P>
+__device__ tpm_tree_node tree_leaf(const P& pos, int begin, int end, const double* bb, const double* be,
+                                   double hsoft) {
     const int lane = (int)lane_id();
     const int n = end - begin;
     double xc[3];
@@ -234,11 +334,11 @@ __device__ tpm_tree_node tree_leaf(const int4* __restrict__ pos, int begin, int 
         dx2 = (float)(raw_to_double(comp(v, 2)) - xc[2]);
     }
     float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
-    for (int k = 0; k < n; k++) {
+    for (int k = 0; k < n; k++) {  // k is wave-uniform: readlane, no LDS round trip
         m0 += 1.0f;
-        m1 += -__shfl(dx0, k);
-        m2 += -__shfl(dx1, k);
-        m3 += -__shfl(dx2, k);
+        m1 += -__int_as_float(__builtin_amdgcn_readlane(__float_as_int(dx0), k));
+        m2 += -__int_as_float(__builtin_amdgcn_readlane(__float_as_int(dx1), k));
+        m3 += -__int_as_float(__builtin_amdgcn_readlane(__float_as_int(dx2), k));
     }
     tpm_tree_node nd;
     for (int d = 0; d < 3; d++) nd.x[d] = tree_fixed(xc[d]);
@@ -311,28 +411,25 @@ __device__ tpm_tree_node tree_internal(const tpm_tree_node& c0, const tpm_tree_n
 
 struct NodeFrame {
     double b[3], e[3];
-    int begin, end, depth, sunk;
+    int begin, end;
+    int depth;
+    int sunk;   // inside a sink bucket already
     int state;  // 0: not entered; 1: children pending
     int which;  // 0: child 0 of its parent, 1: child 1
     int index;  // preorder index in the cell's tree
     int c1;     // child 1's index
-    tpm_tree_node child[2];
 };
 
-__global__ __launch_bounds__(64) void tree_nodes_kernel(int C, const int32_t* __restrict__ cell_begin,
-                                                        const int4* __restrict__ pos,
-                                                        const int32_t* __restrict__ node_begin,
-                                                        const int32_t* __restrict__ bucket_begin,
-                                                        tpm_tree_node* __restrict__ nodes,
-                                                        tpm_tree_bucket* __restrict__ buckets, double hsoft,
-                                                        int32_t* __restrict__ flags) {
-    __shared__ NodeFrame st[TREE_STACK];
-    const int c = blockIdx.x;
+// The numbering walk over one cell.  The stack holds the path from the root
+// and each ancestor's pending right child (<= 2 depth + 3 frames); a
+// finished node is handed to its parent through kids[parent depth][which],
+// which only the parent's two children write while the parent is on the
+// path.  Returns false when the walk disagrees with the partition's counts.
+template <class P>
+__device__ __forceinline__ bool number_cell(const P pos, NodeFrame* st, tpm_tree_node (*kids)[2], int c, int C, int b0, int e0,
+                            tpm_tree_node* __restrict__ out, tpm_tree_bucket* __restrict__ bout, int nmax, int bmax,
+                            double hsoft) {
     const int lane = (int)lane_id();
-    tpm_tree_node* out = nodes + node_begin[c];
-    tpm_tree_bucket* bout = buckets + bucket_begin[c];
-    // the partition kernel counted these; a walk that disagrees stops
-    const int nmax = node_begin[c + 1] - node_begin[c], bmax = bucket_begin[c + 1] - bucket_begin[c];
     if (lane == 0) {
         const double ninv = 1.0 / (double)C;
         const int ic[3] = {c % C, (c / C) % C, c / (C * C)};
@@ -341,8 +438,8 @@ __global__ __launch_bounds__(64) void tree_nodes_kernel(int C, const int32_t* __
             f.b[d] = ic[d] * ninv;
             f.e[d] = (ic[d] + 1) * ninv;
         }
-        f.begin = cell_begin[c];
-        f.end = cell_begin[c + 1];
+        f.begin = b0;
+        f.end = e0;
         f.depth = 0;
         f.sunk = 0;
         f.state = 0;
@@ -355,8 +452,9 @@ __global__ __launch_bounds__(64) void tree_nodes_kernel(int C, const int32_t* __
     int sp = 1, next = 0, nb = 0;
     while (sp > 0) {
         NodeFrame& f = st[sp - 1];
-        const int begin = f.begin, end = f.end, n = end - begin;
+        const int begin = f.begin, end = f.end, n = end - begin, depth = f.depth;
         double bb[3], be[3];
+#pragma unroll
         for (int d = 0; d < 3; d++) {
             bb[d] = f.b[d];
             be[d] = f.e[d];
@@ -365,10 +463,9 @@ __global__ __launch_bounds__(64) void tree_nodes_kernel(int C, const int32_t* __
         int index;
         if (f.state == 0) {
             index = next++;
-            if (index >= nmax || sp + 2 > TREE_STACK || (n <= TREE_BUCKET && !f.sunk && nb >= bmax)) {
-                if (lane == 0) atomicOr(&flags[2], 1);
-                break;
-            }
+            if (index >= nmax || sp + 2 > NODE_STACK || depth > TREE_DEPTH ||
+                (n <= TREE_BUCKET && !f.sunk && nb >= bmax))
+                return false;
             if (n <= TREE_BUCKET && !f.sunk) {
                 double xc[3];
                 const double r2max = tree_bbox(pos, begin, end, bb, be, xc);
@@ -386,39 +483,38 @@ __global__ __launch_bounds__(64) void tree_nodes_kernel(int C, const int32_t* __
             }
             if (n > TREE_LEAF) {
                 const int dim = split_dim(bb, be);
-                const double mid = (be[dim] + bb[dim]) / 2.0;
+                const double mid = (f.e[dim] + f.b[dim]) / 2.0;  // f is in LDS
                 const uint32_t thr = tree_fixed(mid);
-                int lo = begin, hi = end;  // first particle of the right half
-                while (lo < hi) {
-                    const int q = (lo + hi) >> 1;
-                    if (comp(pos[q], dim) < thr)
-                        lo = q + 1;
-                    else
-                        hi = q;
+                // first particle of the right half: the records are
+                // partitioned, so it is begin + the number left of thr
+                int lo = begin;
+                for (int i0 = begin; i0 < end; i0 += 64) {
+                    const int i = i0 + lane;
+                    lo += __popcll(__ballot(i < end && comp(pos[i], dim) < thr));
                 }
                 __syncthreads();
                 if (lane == 0) {
                     f.state = 1;
                     f.index = index;
-                    NodeFrame L, R;
+                    NodeFrame& R = st[sp];
+                    NodeFrame& L = st[sp + 1];
+#pragma unroll
                     for (int d = 0; d < 3; d++) {
                         L.b[d] = R.b[d] = bb[d];
                         L.e[d] = R.e[d] = be[d];
                     }
-                    L.e[dim] = mid;
+                    L.e[dim] = mid;  // LDS, indexed at run time
                     R.b[dim] = mid;
                     L.begin = begin;
                     L.end = R.begin = lo;
                     R.end = end;
-                    L.depth = R.depth = f.depth + 1;
+                    L.depth = R.depth = depth + 1;
                     L.sunk = R.sunk = n <= TREE_BUCKET;
                     L.state = R.state = 0;
                     L.which = 0;
                     R.which = 1;
                     L.index = R.index = 0;
                     L.c1 = R.c1 = -1;
-                    st[sp] = R;
-                    st[sp + 1] = L;
                 }
                 sp += 2;
                 __syncthreads();
@@ -427,7 +523,7 @@ __global__ __launch_bounds__(64) void tree_nodes_kernel(int C, const int32_t* __
             done = tree_leaf(pos, begin, end, bb, be, hsoft);
         } else {
             index = f.index;
-            done = tree_internal(f.child[0], f.child[1], bb, be, hsoft);
+            done = tree_internal(kids[depth][0], kids[depth][1], bb, be, hsoft);
             done.child[0] = index + 1;
             done.child[1] = f.c1;
         }
@@ -437,14 +533,46 @@ __global__ __launch_bounds__(64) void tree_nodes_kernel(int C, const int32_t* __
         if (lane == 0) {
             out[index] = done;
             if (sp > 1) {
-                NodeFrame& parent = st[which == 0 ? sp - 3 : sp - 2];
-                parent.child[which] = done;
-                if (which == 1) parent.c1 = index;
+                kids[depth - 1][which] = done;
+                if (which == 1) st[sp - 2].c1 = index;  // the parent is just below its last child
             }
         }
         sp--;
         __syncthreads();
     }
+    return true;
+}
+
+__global__ __launch_bounds__(64) void tree_nodes_kernel(int C, const int32_t* __restrict__ cell_begin,
+                                                        const int4* __restrict__ pos,
+                                                        const int32_t* __restrict__ node_begin,
+                                                        const int32_t* __restrict__ bucket_begin,
+                                                        tpm_tree_node* __restrict__ nodes,
+                                                        tpm_tree_bucket* __restrict__ buckets, double hsoft,
+                                                        int32_t* __restrict__ flags) {
+    __shared__ NodeFrame st[NODE_STACK];
+    __shared__ tpm_tree_node kids[TREE_DEPTH + 1][2];
+    __shared__ uint32_t xs[TREE_LDS_CAP], ys[TREE_LDS_CAP], zs[TREE_LDS_CAP];
+    const int c = blockIdx.x;
+    const int lane = (int)lane_id();
+    const int b0 = cell_begin[c], e0 = cell_begin[c + 1];
+    tpm_tree_node* out = nodes + node_begin[c];
+    tpm_tree_bucket* bout = buckets + bucket_begin[c];
+    // the partition kernel counted these; a walk that disagrees stops
+    const int nmax = node_begin[c + 1] - node_begin[c], bmax = bucket_begin[c + 1] - bucket_begin[c];
+    bool ok;
+    if (e0 - b0 <= TREE_LDS_CAP) {
+        for (int k = lane; k < e0 - b0; k += 64) {
+            const int4 v = pos[b0 + k];
+            xs[k] = (uint32_t)v.x;
+            ys[k] = (uint32_t)v.y;
+            zs[k] = (uint32_t)v.z;
+        }
+        ok = number_cell(PosLds{xs, ys, zs, b0}, st, kids, c, C, b0, e0, out, bout, nmax, bmax, hsoft);
+    } else {
+        ok = number_cell(PosGlobal{pos}, st, kids, c, C, b0, e0, out, bout, nmax, bmax, hsoft);
+    }
+    if (!ok && lane == 0) atomicOr(&flags[2], 1);
 }
 
 // ---------------------------------------------------------------------------
