@@ -335,3 +335,20 @@ def test_kick_is_deterministic(gpu_ctx):
         out.append(gpu_ctx.download())
     for k in ("x", "gx", "gy", "gz", "pot", "vx", "rung"):
         assert np.array_equal(out[0][k], out[1][k]), k
+
+
+def test_pair_rate_probe(gpu_ctx):
+    """The pair-body probe runs with the current parameters and reports a
+    rate; out-of-range occupancies and missing parameters are errors."""
+    fresh = T.Context()
+    try:
+        with pytest.raises(T.TpmError, match="parameters"):
+            fresh.probe_pair_rate(7)
+    finally:
+        fresh.close()
+    gpu_ctx.set_params(T.baseline_params(64, 64, 3))
+    with pytest.raises(T.TpmError, match="waves_per_simd"):
+        gpu_ctx.probe_pair_rate(9)
+    r4, r7 = gpu_ctx.probe_pair_rate(4), gpu_ctx.probe_pair_rate(7)
+    print("pair body rate", r4, r7, flush=True)
+    assert 1e11 < r4 < 1e14 and 1e11 < r7 < 1e14

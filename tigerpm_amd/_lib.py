@@ -163,6 +163,7 @@ def lib():
         "tpm_get_tuning": (_int, [vp, ctypes.POINTER(TpmTuning)]),
         "tpm_profile": (_int, [vp, _int]),
         "tpm_profile_read": (_int, [vp, _f64p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+        "tpm_probe_pair_rate": (_int, [vp, _int, ctypes.POINTER(ctypes.c_double)]),
         "tpm_batch_create": (_int, [_int, ctypes.POINTER(TpmBatchLimits), ctypes.POINTER(vp)]),
         "tpm_batch_destroy": (_int, [vp]),
         "tpm_batch_last_error": (ctypes.c_char_p, [vp]),
@@ -355,6 +356,13 @@ class Context:
         self._chk(self._L.tpm_profile_read(self._h, ms, ctypes.byref(npp), ctypes.byref(nst)))
         return dict(chainmesh_ms=ms[0], pm_ms=ms[1], pp_ms=ms[2], step_ms=ms[3], pp_launches=npp.value,
                     steps=nst.value)
+
+    def probe_pair_rate(self, waves_per_simd=7):
+        """Pairs/s of the direct kernel's pair body alone (the ceiling of its
+        pair loop at that occupancy)."""
+        r = ctypes.c_double()
+        self._chk(self._L.tpm_probe_pair_rate(self._h, int(waves_per_simd), ctypes.byref(r)))
+        return r.value
 
     def sync(self):
         self._chk(self._L.tpm_sync(self._h))
