@@ -39,11 +39,28 @@ __global__ void key_kernel(int64_t n, const uint32_t* __restrict__ x, const uint
     }
 }
 
-template <class T>
-__global__ void gather_kernel(int64_t n, const uint32_t* __restrict__ perm, const T* __restrict__ src,
-                              T* __restrict__ dst) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[perm[i]];
+// The seven particle arrays in sorted order, in one pass (perm read once).
+struct GatherArgs {
+    const uint32_t* perm;
+    const uint32_t *x, *y, *z;
+    const float *vx, *vy, *vz;
+    const int8_t* rung;
+    uint32_t *x2, *y2, *z2;
+    float *vx2, *vy2, *vz2;
+    int8_t* rung2;
+};
+
+__global__ void gather_kernel(int64_t n, GatherArgs a) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t j = a.perm[i];
+        a.x2[i] = a.x[j];
+        a.y2[i] = a.y[j];
+        a.z2[i] = a.z[j];
+        a.vx2[i] = a.vx[j];
+        a.vy2[i] = a.vy[j];
+        a.vz2[i] = a.vz[j];
+        a.rung2[i] = a.rung[j];
+    }
 }
 
 // cell_begin[c] = first sorted position with cell id >= c.  Every entry is
@@ -154,13 +171,9 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
                                            (size_t)n, 0, end_bit, s));
 
     const unsigned g = grid_for(n, 256);
-    gather_kernel<uint32_t><<<g, 256, 0, s>>>(n, ctx->perm, ctx->x, ctx->x2);
-    gather_kernel<uint32_t><<<g, 256, 0, s>>>(n, ctx->perm, ctx->y, ctx->y2);
-    gather_kernel<uint32_t><<<g, 256, 0, s>>>(n, ctx->perm, ctx->z, ctx->z2);
-    gather_kernel<float><<<g, 256, 0, s>>>(n, ctx->perm, ctx->vx, ctx->vx2);
-    gather_kernel<float><<<g, 256, 0, s>>>(n, ctx->perm, ctx->vy, ctx->vy2);
-    gather_kernel<float><<<g, 256, 0, s>>>(n, ctx->perm, ctx->vz, ctx->vz2);
-    gather_kernel<int8_t><<<g, 256, 0, s>>>(n, ctx->perm, ctx->rung, ctx->rung2);
+    gather_kernel<<<g, 256, 0, s>>>(n, GatherArgs{ctx->perm, ctx->x, ctx->y, ctx->z, ctx->vx, ctx->vy, ctx->vz,
+                                                  ctx->rung, ctx->x2, ctx->y2, ctx->z2, ctx->vx2, ctx->vy2,
+                                                  ctx->vz2, ctx->rung2});
     TPM_HIP(ctx, hipGetLastError());
     std::swap(ctx->x, ctx->x2);
     std::swap(ctx->y, ctx->y2);
@@ -378,28 +391,6 @@ __global__ __launch_bounds__(256) void bcell_order_kernel(int32_t ncell, const i
     }
 }
 
-struct GatherArgs {
-    const uint32_t* perm;
-    const uint32_t *x, *y, *z;
-    const float *vx, *vy, *vz;
-    const int8_t* rung;
-    uint32_t *x2, *y2, *z2;
-    float *vx2, *vy2, *vz2;
-    int8_t* rung2;
-};
-
-__global__ void bgather_kernel(int64_t n, GatherArgs a) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t j = a.perm[i];
-        a.x2[i] = a.x[j];
-        a.y2[i] = a.y[j];
-        a.z2[i] = a.z[j];
-        a.vx2[i] = a.vx[j];
-        a.vy2[i] = a.vy[j];
-        a.vz2[i] = a.vz[j];
-        a.rung2[i] = a.rung[j];
-    }
-}
 
 __global__ void btiles_count_kernel(int32_t ncell, int ts, const int32_t* __restrict__ cell_begin,
                                     int32_t* __restrict__ nt) {
@@ -463,7 +454,7 @@ tpm_status tpm_batch_chainmesh(tpm_batch* bt, const tpm_batch_layout& L) {
     if (L.n_live > 0) {
         GatherArgs g{bt->perm, bt->x,  bt->y,  bt->z,  bt->vx,  bt->vy,  bt->vz, bt->rung,
                      bt->x2,   bt->y2, bt->z2, bt->vx2, bt->vy2, bt->vz2, bt->rung2};
-        bgather_kernel<<<grid_for(L.n_live, 256), 256, 0, s>>>(L.n_live, g);
+        gather_kernel<<<grid_for(L.n_live, 256), 256, 0, s>>>(L.n_live, g);
         TPM_HIP(bt, hipGetLastError());
     }
     std::swap(bt->x, bt->x2);
