@@ -20,6 +20,14 @@
 
 namespace {
 
+// Onesweep radix sort with 9-bit digits: the 512^3 keys (26 bits) take three
+// passes instead of four (6.0 vs 6.5 ms for the whole chain mesh, same
+// permutation: the sort is stable, so the digit width cannot change it).
+using ChainSortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<512, 12>, rocprim::kernel_config<512, 12>, 9,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+
 __global__ void key_kernel(int64_t n, const uint32_t* __restrict__ x, const uint32_t* __restrict__ y,
                            const uint32_t* __restrict__ z, int C, const uint32_t* __restrict__ bnd,
                            uint32_t* __restrict__ key, uint32_t* __restrict__ idx) {
@@ -160,14 +168,14 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
     unsigned end_bit = 1;
     while ((uint64_t(1) << end_bit) < (uint64_t)ncell * 64) end_bit++;
     size_t need = 0;
-    TPM_HIP(ctx, rocprim::radix_sort_pairs(nullptr, need, ctx->keys, ctx->keys2, ctx->idx, ctx->perm, (size_t)n, 0,
+    TPM_HIP(ctx, rocprim::radix_sort_pairs<ChainSortConfig>(nullptr, need, ctx->keys, ctx->keys2, ctx->idx, ctx->perm, (size_t)n, 0,
                                            end_bit, s));
     if (need > ctx->sort_tmp_bytes) {
         if (ctx->sort_tmp) TPM_HIP(ctx, hipFree(ctx->sort_tmp));
         TPM_HIP(ctx, hipMalloc(&ctx->sort_tmp, need));
         ctx->sort_tmp_bytes = need;
     }
-    TPM_HIP(ctx, rocprim::radix_sort_pairs(ctx->sort_tmp, need, ctx->keys, ctx->keys2, ctx->idx, ctx->perm,
+    TPM_HIP(ctx, rocprim::radix_sort_pairs<ChainSortConfig>(ctx->sort_tmp, need, ctx->keys, ctx->keys2, ctx->idx, ctx->perm,
                                            (size_t)n, 0, end_bit, s));
 
     const unsigned g = grid_for(n, 256);

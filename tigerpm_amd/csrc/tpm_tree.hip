@@ -5,13 +5,16 @@
 // Build, one wave64 per chain cell:
 //   tree_partition_kernel: the recursive midpoint split of tree.cpp's sort()
 //     as a depth-first walk over an LDS stack.  Each split is a stable
-//     wave-wide partition (ballot + mbcnt) of int4 records (x, y, z,
-//     chain-mesh index) from one buffer into the other, so the leaves end in
-//     tree order; the kernel counts the cell's nodes and sink buckets.
+//     wave-wide partition (ballot + mbcnt) from one buffer into the other, so
+//     the leaves end in tree order as int4 records (x, y, z, chain-mesh
+//     index); the kernel counts the cell's nodes and sink buckets.  A cell of
+//     <= TREE_LDS_CAP particles is split in LDS (16-bit indices), a larger
+//     one in device memory.
 //   Exclusive scans of the counts give each cell's node and bucket base.
-//   tree_nodes_kernel: the same walk over the partitioned records (split
-//     points by binary search), nodes numbered in preorder and written after
-//     their children: a leaf's centre, radius and moments from its particles
+//   tree_nodes_kernel: the same walk over the partitioned positions (in LDS
+//     up to the same size; a split point is the ballot count of particles
+//     left of the plane), nodes numbered in preorder and written after their
+//     children: a leaf's centre, radius and moments from its particles
 //     (bounding box and farthest distance by wave reductions, exact in FP64;
 //     moment sums in particle order), an internal node from its two children
 //     as tree.cpp:101-171 does, and the sink buckets (tree.cpp:173-211).
@@ -657,10 +660,11 @@ __device__ __forceinline__ void pc_interaction(const WalkArgs& a, float dx, floa
 }
 
 struct WalkShared {
-    int stack[WALK_STACK];
-    int mlist[WALK_LIST];
+    int stack[WALK_STACK];  // (neighbour << 27) | node index within that tree, as unsigned bits
+    int mlist[WALK_LIST];   // global node indices
     int plist[WALK_LIST];
     int4 sbuf[WALK_SBUF];
+    int nb_base[27], nb_cnt[27];  // the neighbour trees' first node and node count
 };
 
 // One sink bucket of <= 32 K sinks.  Lane l holds sinks (l & 31) + 32 k,
@@ -710,9 +714,9 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
         const float sr = bk.radius;
         // Far dummy that pads a staged list to an even length: r >= rcut.
         const int4 far = make_int4(0x40000000, 0x40000000, 0x40000000, 0);
-        for (int nbr = 0; nbr < 27 && !overflow; nbr++) {
-            // x outermost, z innermost (kick_treepm.cu:930-940)
-            int jx = cx + nbr / 9 - 1, jy = cy + (nbr / 3) % 3 - 1, jz = cz + nbr % 3 - 1;
+        // the 27 neighbour trees (x outermost, z innermost, kick_treepm.cu:930-940)
+        if (lane < 27) {
+            int jx = cx + lane / 9 - 1, jy = cy + (lane / 3) % 3 - 1, jz = cz + lane % 3 - 1;
             jx += (jx < 0) ? C : 0;
             jx -= (jx >= C) ? C : 0;
             jy += (jy < 0) ? C : 0;
@@ -720,160 +724,170 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
             jz += (jz < 0) ? C : 0;
             jz -= (jz >= C) ? C : 0;
             const int tcell = (jz * C + jy) * C + jx;
-            const tpm_tree_node* tr = a.nodes + a.node_begin[tcell];
-            const int tn = a.node_begin[tcell + 1] - a.node_begin[tcell];
-            int nm = 0, np = 0;
-            // applies and empties the multipole list (each stream every other node)
-            auto flush_pc = [&]() {
-                // the next node's centre and moments load while this one is applied
-                uint32_t nx = 0, ny = 0, nz = 0;
-                float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
-                if (stream < nm) {
-                    const tpm_tree_node& nd = tr[sh.mlist[stream]];
-                    nx = nd.x[0], ny = nd.x[1], nz = nd.x[2];
-                    m0 = nd.m[0], m1 = nd.m[1], m2 = nd.m[2], m3 = nd.m[3];
-                }
-                for (int q = stream; q < nm; q += 2) {
-                    const uint32_t cx0 = nx, cy0 = ny, cz0 = nz;
-                    const float M[4] = {m0, m1, m2, m3};
-                    if (q + 2 < nm) {
-                        const tpm_tree_node& nn = tr[sh.mlist[q + 2]];
-                        nx = nn.x[0], ny = nn.x[1], nz = nn.x[2];
-                        m0 = nn.m[0], m1 = nn.m[1], m2 = nn.m[2], m3 = nn.m[3];
-                    }
+            sh.nb_base[lane] = a.node_begin[tcell];
+            sh.nb_cnt[lane] = a.node_begin[tcell + 1] - a.node_begin[tcell];
+        }
+        __syncthreads();
+        // all 27 trees are walked together from one stack, so a round takes
+        // up to 64 pending nodes of any of them
+        const bool root = lane < 27 && sh.nb_cnt[lane] > 0;
+        const unsigned long long mr = __ballot(root);
+        if (root) sh.stack[(int)count_below(mr)] = (int)((unsigned)lane << 27);
+        int tn = lane < 27 ? sh.nb_cnt[lane] : 0;
 #pragma unroll
-                    for (int k = 0; k < K; k++)
-                        pc_interaction(a, sdist(ox[k], bk.x[0], cx0), sdist(oy[k], bk.x[1], cy0),
-                                       sdist(oz[k], bk.x[2], cz0), M, cg[k][0],
-                                       cg[k][1], cg[k][2], cg[k][3]);
+        for (int o = 32; o >= 1; o >>= 1) tn += __shfl_xor(tn, o);
+        int sp = __popcll(mr), visited = 0, nm = 0, np = 0;
+        __syncthreads();
+        // applies and empties the multipole list (each stream every other node)
+        auto flush_pc = [&]() {
+            // the next node's centre and moments load while this one is applied
+            uint32_t nx = 0, ny = 0, nz = 0;
+            float m0 = 0.f, m1 = 0.f, m2 = 0.f, m3 = 0.f;
+            if (stream < nm) {
+                const tpm_tree_node& nd = a.nodes[sh.mlist[stream]];
+                nx = nd.x[0], ny = nd.x[1], nz = nd.x[2];
+                m0 = nd.m[0], m1 = nd.m[1], m2 = nd.m[2], m3 = nd.m[3];
+            }
+            for (int q = stream; q < nm; q += 2) {
+                const uint32_t cx0 = nx, cy0 = ny, cz0 = nz;
+                const float M[4] = {m0, m1, m2, m3};
+                if (q + 2 < nm) {
+                    const tpm_tree_node& nn = a.nodes[sh.mlist[q + 2]];
+                    nx = nn.x[0], ny = nn.x[1], nz = nn.x[2];
+                    m0 = nn.m[0], m1 = nn.m[1], m2 = nn.m[2], m3 = nn.m[3];
                 }
-                npc += (unsigned long long)nm * nact;
-                nm = 0;
-            };
-            // stages the leaf list's particles and applies them pair by pair
-            auto flush_pp = [&]() {
-                int q = 0;
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    pc_interaction(a, sdist(ox[k], bk.x[0], cx0), sdist(oy[k], bk.x[1], cy0),
+                                   sdist(oz[k], bk.x[2], cz0), M, cg[k][0], cg[k][1], cg[k][2], cg[k][3]);
+            }
+            npc += (unsigned long long)nm * nact;
+            nm = 0;
+        };
+        // stages the leaf list's particles and applies them pair by pair
+        auto flush_pp = [&]() {
+            int q = 0;
+            while (q < np) {
+                int nbuf = 0;
                 while (q < np) {
-                    int nbuf = 0;
-                    while (q < np) {
-                        const tpm_tree_node& nd = tr[__builtin_amdgcn_readfirstlane(sh.plist[q])];
-                        const int cnt = min(max(nd.end - nd.begin, 0), TREE_LEAF);
-                        if (nbuf + cnt > WALK_SBUF - 3) break;
-                        if (lane < cnt && nd.begin + lane < a.n) {
-                            const int4 v = a.pos[nd.begin + lane];
-                            sh.sbuf[nbuf + lane] = make_int4((int)((uint32_t)v.x - bk.x[0]),
-                                                             (int)((uint32_t)v.y - bk.x[1]),
-                                                             (int)((uint32_t)v.z - bk.x[2]), 0);
-                        }
-                        nbuf += cnt;
-                        q++;
+                    const tpm_tree_node& nd = a.nodes[__builtin_amdgcn_readfirstlane(sh.plist[q])];
+                    const int cnt = min(max(nd.end - nd.begin, 0), TREE_LEAF);
+                    if (nbuf + cnt > WALK_SBUF - 3) break;
+                    if (lane < cnt && nd.begin + lane < a.n) {
+                        const int4 v = a.pos[nd.begin + lane];
+                        sh.sbuf[nbuf + lane] = make_int4((int)((uint32_t)v.x - bk.x[0]),
+                                                         (int)((uint32_t)v.y - bk.x[1]),
+                                                         (int)((uint32_t)v.z - bk.x[2]), 0);
                     }
-                    if (lane == 0 && (nbuf & 1)) sh.sbuf[nbuf] = far;
-                    __syncthreads();
-                    const int padded = (nbuf + 1) & ~1;
-                    // one staged source ahead (the read past the end stays below
-                    // padded + 2 <= WALK_SBUF and is not used)
-                    int4 sv = sh.sbuf[stream];
-                    for (int j = stream; j < padded; j += 2) {
-                        const int4 sn = sh.sbuf[j + 2];
-#pragma unroll
-                        for (int k = 0; k < K; k++)
-                            pp_pair<false>(B, ox[k], oy[k], oz[k], sv, pg[k][0], pg[k][1], pg[k][2], pg[k][3]);
-                        sv = sn;
-                    }
-                    npairs += (unsigned long long)nbuf * nact;
-                    __syncthreads();
+                    nbuf += cnt;
+                    q++;
                 }
-                np = 0;
-            };
-            int sp = 1, visited = 0;
-            if (lane == 0) sh.stack[0] = 0;
-            __syncthreads();
-            while (sp > 0) {
-                // 64 nodes a round while the stack has room for their
-                // children; near the top one at a time, which grows the stack
-                // by at most the remaining tree depth (<= 50)
-                const int take = (sp + 128 <= WALK_STACK) ? (sp < 64 ? sp : 64) : 1;
-                visited += take;
-                if (visited > tn) {  // each node is met at most once
-                    overflow = true;
-                    break;
-                }
-                const int idx = lane < take ? sh.stack[sp - 1 - lane] : 0;
-                const bool valid = lane < take && idx >= 0 && idx < tn;
+                if (lane == 0 && (nbuf & 1)) sh.sbuf[nbuf] = far;
                 __syncthreads();
-                sp -= take;
-                bool is_multi = false, is_open = false, is_leaf = false;
-                tpm_tree_node nd;
-                if (valid) {
-                    nd = tr[idx];
-                    const float dx = dist(bk.x[0], nd.x[0]), dy = dist(bk.x[1], nd.x[1]), dz = dist(bk.x[2], nd.x[2]);
-                    const float R2 = sqr3_f(dx, dy, dz);
-                    const float R = sqrtf(R2);
-                    const bool cutoff = R - nd.radius - sr > rcut;
-                    const float rr = nd.radius + sr;
-                    const bool far_node = R2 > rr * rr * t2;
-                    const bool leaf = nd.end - nd.begin <= TREE_LEAF;
-                    is_multi = !cutoff && far_node;
-                    is_open = !cutoff && !far_node && !leaf;
-                    is_leaf = !cutoff && !far_node && leaf;
-                }
-                // near internal nodes: children onto the stack
-                const unsigned long long mo = __ballot(is_open);
-                const int nopen = __popcll(mo);
-                if (sp + 2 * nopen > WALK_STACK) {  // unreachable below the depth limit
-                    overflow = true;
-                    break;
-                }
-                if (is_open) {
-                    const int at = sp + 2 * (int)count_below(mo);
-                    sh.stack[at] = nd.child[1];
-                    sh.stack[at + 1] = nd.child[0];
-                }
-                sp += 2 * nopen;
-                // far nodes: the multipole list
-                const unsigned long long mm = __ballot(is_multi);
-                if (nm + __popcll(mm) > WALK_LIST) {
-                    __syncthreads();
-                    flush_pc();
-                }
-                if (is_multi) sh.mlist[nm + (int)count_below(mm)] = idx;
-                nm += __popcll(mm);
-                // near leaves: PC when every active sink is beyond r_src / theta
-                unsigned long long ml = __ballot(is_leaf);
-                while (ml) {
-                    const int j = __ffsll((long long)ml) - 1;
-                    ml &= ml - 1;
-                    const int lidx = __shfl(idx, j);
-                    const uint32_t lx = (uint32_t)__shfl((int)nd.x[0], j), ly = (uint32_t)__shfl((int)nd.x[1], j),
-                                   lz = (uint32_t)__shfl((int)nd.x[2], j);
-                    const float lr = __shfl(nd.radius, j);
-                    const float lim = lr * lr * t2;
-                    bool nearby = false;
+                const int padded = (nbuf + 1) & ~1;
+                // one staged source ahead (the read past the end stays below
+                // padded + 2 <= WALK_SBUF and is not used)
+                int4 sv = sh.sbuf[stream];
+                for (int j = stream; j < padded; j += 2) {
+                    const int4 sn = sh.sbuf[j + 2];
 #pragma unroll
                     for (int k = 0; k < K; k++)
-                        nearby |= ((act >> k) & 1u) &&
-                                  !(sqr3_f(sdist(ox[k], bk.x[0], lx), sdist(oy[k], bk.x[1], ly),
-                                           sdist(oz[k], bk.x[2], lz)) > lim);
-                    if (__ballot(nearby) != 0ull) {
-                        if (np == WALK_LIST) {
-                            __syncthreads();
-                            flush_pp();
-                        }
-                        if (lane == 0) sh.plist[np] = lidx;
-                        np++;
-                    } else {
-                        if (nm == WALK_LIST) {
-                            __syncthreads();
-                            flush_pc();
-                        }
-                        if (lane == 0) sh.mlist[nm] = lidx;
-                        nm++;
-                    }
+                        pp_pair<false>(B, ox[k], oy[k], oz[k], sv, pg[k][0], pg[k][1], pg[k][2], pg[k][3]);
+                    sv = sn;
                 }
+                npairs += (unsigned long long)nbuf * nact;
                 __syncthreads();
             }
+            np = 0;
+        };
+        while (sp > 0) {
+            // 64 nodes a round while the stack has room for their children;
+            // near the top one at a time, which grows the stack by at most
+            // the remaining tree depth (<= 50)
+            const int take = (sp + 128 <= WALK_STACK) ? (sp < 64 ? sp : 64) : 1;
+            visited += take;
+            if (visited > tn) {  // each node is met at most once
+                overflow = true;
+                break;
+            }
+            const unsigned e = lane < take ? (unsigned)sh.stack[sp - 1 - lane] : 0u;
+            const int nbr = (int)(e >> 27), li = (int)(e & 0x7ffffffu);
+            const bool valid = lane < take && nbr < 27 && li < sh.nb_cnt[nbr < 27 ? nbr : 0];
+            const int idx = valid ? sh.nb_base[nbr] + li : 0;  // global node index
             __syncthreads();
+            sp -= take;
+            bool is_multi = false, is_open = false, is_leaf = false;
+            tpm_tree_node nd;
+            if (valid) {
+                nd = a.nodes[idx];
+                const float dx = dist(bk.x[0], nd.x[0]), dy = dist(bk.x[1], nd.x[1]), dz = dist(bk.x[2], nd.x[2]);
+                const float R2 = sqr3_f(dx, dy, dz);
+                const float R = sqrtf(R2);
+                const bool cutoff = R - nd.radius - sr > rcut;
+                const float rr = nd.radius + sr;
+                const bool far_node = R2 > rr * rr * t2;
+                const bool leaf = nd.end - nd.begin <= TREE_LEAF;
+                is_multi = !cutoff && far_node;
+                is_open = !cutoff && !far_node && !leaf;
+                is_leaf = !cutoff && !far_node && leaf;
+            }
+            // near internal nodes: children onto the stack
+            const unsigned long long mo = __ballot(is_open);
+            const int nopen = __popcll(mo);
+            if (sp + 2 * nopen > WALK_STACK) {  // unreachable below the depth limit
+                overflow = true;
+                break;
+            }
+            if (is_open) {
+                const int at = sp + 2 * (int)count_below(mo);
+                sh.stack[at] = (int)(((unsigned)nbr << 27) | (unsigned)nd.child[1]);
+                sh.stack[at + 1] = (int)(((unsigned)nbr << 27) | (unsigned)nd.child[0]);
+            }
+            sp += 2 * nopen;
+            // far nodes: the multipole list
+            const unsigned long long mm = __ballot(is_multi);
+            if (nm + __popcll(mm) > WALK_LIST) {
+                __syncthreads();
+                flush_pc();
+            }
+            if (is_multi) sh.mlist[nm + (int)count_below(mm)] = idx;
+            nm += __popcll(mm);
+            // near leaves: PC when every active sink is beyond r_src / theta
+            unsigned long long ml = __ballot(is_leaf);
+            while (ml) {
+                const int j = __ffsll((long long)ml) - 1;
+                ml &= ml - 1;
+                const int lidx = __shfl(idx, j);
+                const uint32_t lx = (uint32_t)__shfl((int)nd.x[0], j), ly = (uint32_t)__shfl((int)nd.x[1], j),
+                               lz = (uint32_t)__shfl((int)nd.x[2], j);
+                const float lr = __shfl(nd.radius, j);
+                const float lim = lr * lr * t2;
+                bool nearby = false;
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    nearby |= ((act >> k) & 1u) &&
+                              !(sqr3_f(sdist(ox[k], bk.x[0], lx), sdist(oy[k], bk.x[1], ly),
+                                       sdist(oz[k], bk.x[2], lz)) > lim);
+                if (__ballot(nearby) != 0ull) {
+                    if (np == WALK_LIST) {
+                        __syncthreads();
+                        flush_pp();
+                    }
+                    if (lane == 0) sh.plist[np] = lidx;
+                    np++;
+                } else {
+                    if (nm == WALK_LIST) {
+                        __syncthreads();
+                        flush_pc();
+                    }
+                    if (lane == 0) sh.mlist[nm] = lidx;
+                    nm++;
+                }
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        if (!overflow) {
             flush_pc();
             flush_pp();
         }
