@@ -594,6 +594,7 @@ struct WalkArgs {
     int nbuckets, per_xcd;
     int n;                               // particles
     float rcut, theta2inv, inv2rs, tsp;  // box units
+    float e2;                            // -log2(e) inv2rs^2: exp(-r0^2) = exp2(r^2 e2)
     unsigned flags;
 };
 
@@ -606,9 +607,10 @@ __device__ __forceinline__ void pc_interaction(const WalkArgs& a, float dx, floa
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
     const float r0 = r * a.inv2rs;
-    // erfcexp (util.hpp:36-59)
-    const float t1 = 1.f / fmaf(0.3275911f, r0, 1.f);
-    const float exp0 = expf(-r0 * r0);
+    // erfcexp (util.hpp:36-59), with the hardware reciprocal and exp2 as in
+    // the pair law
+    const float t1 = __builtin_amdgcn_rcpf(fmaf(0.3275911f, r0, 1.f));
+    const float exp0 = __builtin_amdgcn_exp2f(r2 * a.e2);
     const float t2 = t1 * t1, t3 = t2 * t1, t4 = t2 * t2, t5 = t2 * t3;
     const float erfc0 = fmaf(0.254829592f, t1,
                              fmaf(-0.284496736f, t2, fmaf(1.421413741f, t3, fmaf(-1.453152027f, t4, 1.061405429f * t5)))) *
@@ -900,25 +902,37 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
         atomicAdd(&a.o.counters[0], npairs);
         atomicAdd(&a.o.counters[3], npc);
     }
-    // the two streams' sums, then (stream 0) short range in box units, mesh, kick
+    // the two streams' sums: short range in box units (pg), then the mesh
+    // force, each sink's six x planes split between the streams (cg)
 #pragma unroll
     for (int k = 0; k < K; k++)
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             pg[k][c] += __shfl_xor(pg[k][c], 32);
             cg[k][c] += __shfl_xor(cg[k][c], 32);
+            pg[k][c] = pg[k][c] * (c < 3 ? 0x1p64f : 0x1p32f) + cg[k][c];
+            cg[k][c] = 0.f;
         }
+    const bool use_mesh = a.phi && !(a.flags & TPM_SKIP_PM);
+    if (use_mesh) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if ((act >> k) & 1u)
+                mesh_interp6_part(a.phi, B.M, (uint32_t)ox[k] + bk.x[0], (uint32_t)oy[k] + bk.x[1],
+                                  (uint32_t)oz[k] + bk.x[2], stream, 2, cg[k][0], cg[k][1], cg[k][2], cg[k][3]);
+#pragma unroll
+            for (int c = 0; c < 4; c++) cg[k][c] += __shfl_xor(cg[k][c], 32);
+        }
+    }
     if (stream != 0) return;
+    const float Mf = (float)B.M;
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (!((act >> k) & 1u)) continue;
-        const float gx = pg[k][0] * 0x1p64f + cg[k][0], gy = pg[k][1] * 0x1p64f + cg[k][1];
-        const float gz = pg[k][2] * 0x1p64f + cg[k][2], ph = pg[k][3] * 0x1p32f + cg[k][3];
-        float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
-        if (a.phi && !(a.flags & TPM_SKIP_PM))
-            mesh_interp6(a.phi, B.M, (uint32_t)ox[k] + bk.x[0], (uint32_t)oy[k] + bk.x[1],
-                         (uint32_t)oz[k] + bk.x[2], pgx, pgy, pgz, pph);
-        kick_epilogue(B, a.flags, a.o, w[k], a.o.rung[w[k]], pgx, pgy, pgz, pph, gx, gy, gz, ph);
+        const float pgx = use_mesh ? -cg[k][0] * Mf : 0.f, pgy = use_mesh ? -cg[k][1] * Mf : 0.f;
+        const float pgz = use_mesh ? -cg[k][2] * Mf : 0.f, pph = use_mesh ? cg[k][3] : 0.f;
+        kick_epilogue(B, a.flags, a.o, w[k], a.o.rung[w[k]], pgx, pgy, pgz, pph, pg[k][0], pg[k][1], pg[k][2],
+                      pg[k][3]);
     }
 }
 
@@ -1066,6 +1080,7 @@ tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, in
     const float theta = 0.4f;
     a.theta2inv = 1.0f / (theta * theta);
     a.inv2rs = 1.0f / (float)ctx->p.rs / 2.0f;
+    a.e2 = (float)(-1.4426950408889634 / (4.0 * ctx->p.rs * ctx->p.rs));
     a.tsp = 2.0f / sqrtf((float)M_PI);
     a.flags = flags & (TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK);
     if (!ctx->ev[0]) {
