@@ -90,27 +90,46 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
         p[i] = v;
 }
 
-__global__ void tiles_per_cell_kernel(int64_t ncell, int ts, const int32_t* __restrict__ cell_begin,
+// Tile order: cells are visited in z-major blocks of TILE_BLOCK^3 cells
+// (blocks z-major, cells z-major inside a block), so the tiles the PP kernel
+// runs close together in time share their 27-cell neighbourhoods: a block's
+// sources (10^3 cells at 512^3, ~2 MB) and its stretch of the potential mesh
+// fit one XCD's L2.  rank(c) is this order's position of cell c, a bijection
+// onto [0, C^3) (edge blocks are smaller).  Each tile is computed
+// independently, so the order changes no result.
+#define TILE_BLOCK 8
+__device__ __forceinline__ int32_t tile_rank(int64_t c, int C) {
+    const int x = (int)(c % C), y = (int)((c / C) % C), z = (int)(c / ((int64_t)C * C));
+    const int bx = x / TILE_BLOCK, by = y / TILE_BLOCK, bz = z / TILE_BLOCK;
+    const int sy = min(TILE_BLOCK, C - by * TILE_BLOCK), sz = min(TILE_BLOCK, C - bz * TILE_BLOCK);
+    const int sx = min(TILE_BLOCK, C - bx * TILE_BLOCK);
+    const int64_t before = (int64_t)bz * TILE_BLOCK * C * C + (int64_t)sz * by * TILE_BLOCK * C +
+                           (int64_t)sz * sy * bx * TILE_BLOCK;
+    const int local = ((z - bz * TILE_BLOCK) * sy + (y - by * TILE_BLOCK)) * sx + (x - bx * TILE_BLOCK);
+    return (int32_t)(before + local);
+}
+
+__global__ void tiles_per_cell_kernel(int64_t ncell, int C, int ts, const int32_t* __restrict__ cell_begin,
                                       int32_t* __restrict__ nt) {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
         const int32_t cnt = cell_begin[c + 1] - cell_begin[c];
-        nt[c] = (cnt + ts - 1) / ts;
+        nt[tile_rank(c, C)] = (cnt + ts - 1) / ts;
     }
 }
-
-__global__ void tiles_write_kernel(int64_t ncell, int ts, const int32_t* __restrict__ cell_begin,
+__global__ void tiles_write_kernel(int64_t ncell, int C, int ts, const int32_t* __restrict__ cell_begin,
                                    const int32_t* __restrict__ tile_end, int4* __restrict__ tiles,
                                    int32_t* __restrict__ ntiles) {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
         const int32_t b = cell_begin[c], e = cell_begin[c + 1];
         const int32_t nt = (e - b + ts - 1) / ts;
-        const int32_t t0 = tile_end[c] - nt;
+        const int32_t r = tile_rank(c, C);
+        const int32_t t0 = tile_end[r] - nt;
         for (int32_t t = 0; t < nt; t++) {
             const int32_t tb = b + t * ts;
             const int32_t te = min(tb + ts, e);
             tiles[t0 + t] = make_int4((int)c, tb, te, 0);
         }
-        if (c == ncell - 1) *ntiles = tile_end[c];
+        if (r == ncell - 1) *ntiles = tile_end[r];
     }
 }
 
@@ -193,7 +212,7 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
 
     cell_bounds_kernel<<<g, 256, 0, s>>>(n, ctx->keys2, ncell, ctx->cell_begin);
     // tiles per cell, then their inclusive scan.
-    tiles_per_cell_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ts, ctx->cell_begin, ctx->tile_cnt);
+    tiles_per_cell_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, C, ts, ctx->cell_begin, ctx->tile_cnt);
     TPM_HIP(ctx, hipGetLastError());
     size_t need2 = 0;
     TPM_HIP(ctx, rocprim::inclusive_scan(nullptr, need2, ctx->tile_cnt, ctx->tile_end, (size_t)ncell,
@@ -205,7 +224,7 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
     }
     TPM_HIP(ctx, rocprim::inclusive_scan(ctx->scan_tmp, need2, ctx->tile_cnt, ctx->tile_end, (size_t)ncell,
                                          rocprim::plus<int32_t>(), s));
-    tiles_write_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, ts, ctx->cell_begin, ctx->tile_end, ctx->tiles,
+    tiles_write_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, C, ts, ctx->cell_begin, ctx->tile_end, ctx->tiles,
                                                             ctx->ntiles);
     TPM_HIP(ctx, hipGetLastError());
     ctx->chain_valid = true;
