@@ -166,6 +166,42 @@ def test_pp_only_vs_oracle(gpu_ctx, dist, tile):
     assert rel_rms(G, np.stack([hx, hy, hz], 1) * p.GM) < 2e-5
 
 
+def test_pp_on_a_larger_chain_mesh(gpu_ctx):
+    """chain_dim 17: tiles laid out in 8^3-cell blocks with partial edge
+    blocks.  Every particle is kicked by exactly one tile (all forces
+    nonzero; velocities of sampled sinks equal one half kick, not two), and
+    the sampled sinks' forces match the oracle."""
+    n_side = 96
+    n = n_side ** 3
+    x, y, z = clustered(n, 43)
+    p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, 3))
+    assert p.chain_dim == 17
+    zf = np.zeros(n, np.float32)
+    gpu_ctx.upload(x, y, z, zf, zf, zf, np.zeros(n, np.int8))
+    gpu_ctx.chainmesh_create()
+    print("pp chain_dim 17", flush=True)
+    st = gpu_ctx.kick(0, 1.0, 1.0, True, flags=T.TPM_SKIP_PM)
+    d = gpu_ctx.download()
+    G = np.stack([d["gx"], d["gy"], d["gz"]], 1)
+    assert np.all(np.abs(G).sum(1) > 0)
+    operm, ocb = O.chainmesh_sort(x, y, z, p.chain_dim)
+    assert np.array_equal(gpu_ctx.chainmesh_perm(), operm)
+    xs, ys, zs = x[operm], y[operm], z[operm]
+    assert st.candidate_pairs == O.candidate_pairs(p.chain_dim, ocb)
+    sinks = np.sort(np.random.default_rng(7).choice(n, 3000, replace=False))
+    op = oparams(p)
+    (gx, gy, gz, pot), _ = O.pp_sum(op, xs, ys, zs, ocb, sinks, fp64=False)
+    assert rel_rms(G[sinks], np.stack([gx, gy, gz], 1) * p.GM) < 1e-5
+    z3 = np.zeros(len(sinks), np.float32)
+    (og, opot, ov, orung) = O.kick_epilogue(op, (gx, gy, gz), pot, (z3, z3, z3), np.zeros(len(sinks), np.int8), 0,
+                                            1.0, 1.0, True)
+    assert rel_rms(d["pot"][sinks], opot) < 1e-5
+    same = d["rung"][sinks] == orung
+    assert same.mean() > 0.99
+    V = np.stack([d["vx"], d["vy"], d["vz"]], 1)[sinks][same]
+    assert rel_rms(V, np.stack(ov, 1)[same]) < 1e-5
+
+
 @pytest.mark.parametrize("order", [2, 3])
 def test_full_kick_vs_oracle(gpu_ctx, order):
     n_side = 32
