@@ -46,7 +46,7 @@
 #define TREE_STACK 64    // partition frames: a depth-first walk holds <= depth + 2
 #define NODE_STACK (2 * TREE_DEPTH + 4)  // numbering frames: the path and its pending right siblings
 #define TREE_LDS_CAP 512  // cells up to this many particles are built in LDS
-#define WALK_STACK 1024  // walk: pending nodes of one neighbour tree
+#define WALK_STACK 512   // walk: pending nodes of the 27 neighbour trees (>= 192 + depth suffices)
 #define WALK_LIST 128    // multipole / leaf list entries between flushes
 #define WALK_SBUF 256    // staged PP sources
 
@@ -665,7 +665,9 @@ struct WalkShared {
     int stack[WALK_STACK];  // (neighbour << 27) | node index within that tree, as unsigned bits
     int mlist[WALK_LIST];   // global node indices
     int plist[WALK_LIST];
-    int4 sbuf[WALK_SBUF];
+    int4 sbuf[WALK_SBUF];   // staged sources of the pair list's leaves
+    int4 cbuf[WALK_SBUF];   // those within reach of one 32-sink block
+    float box[4][6];        // each block's active sinks: bounding box of the offsets
     int nb_base[27], nb_cnt[27];  // the neighbour trees' first node and node count
 };
 
@@ -707,6 +709,26 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
     // distance() from slot k's sink to a point: the offset plus the centre's
     auto sdist = [&](int o, uint32_t centre, uint32_t to) { return dist((uint32_t)o + centre, to); };
     if (nact == 0) return;
+    // Bounding box of each block's active sinks (offsets as floats: <= 64
+    // raw units of rounding, inside the 512-unit margin of cull2s, so the
+    // cull below never drops a source within rcut of an active sink).
+    unsigned bact = 0u;  // blocks with an active sink
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const bool on = (act >> k) & 1u;
+        bact |= __ballot(on) ? 1u << k : 0u;
+        const float lo0 = (float)wave_min(on ? ox[k] : INT_MAX), hi0 = (float)wave_max(on ? ox[k] : INT_MIN);
+        const float lo1 = (float)wave_min(on ? oy[k] : INT_MAX), hi1 = (float)wave_max(on ? oy[k] : INT_MIN);
+        const float lo2 = (float)wave_min(on ? oz[k] : INT_MAX), hi2 = (float)wave_max(on ? oz[k] : INT_MIN);
+        if (lane == 0) {
+            sh.box[k][0] = lo0;
+            sh.box[k][1] = hi0;
+            sh.box[k][2] = lo1;
+            sh.box[k][3] = hi1;
+            sh.box[k][4] = lo2;
+            sh.box[k][5] = hi2;
+        }
+    }
     unsigned long long npairs = 0, npc = 0;
     bool overflow = false;
     if (!(a.flags & TPM_SKIP_PP)) {
@@ -784,21 +806,48 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                     nbuf += cnt;
                     q++;
                 }
-                if (lane == 0 && (nbuf & 1)) sh.sbuf[nbuf] = far;
                 __syncthreads();
-                const int padded = (nbuf + 1) & ~1;
-                // one staged source ahead (the read past the end stays below
-                // padded + 2 <= WALK_SBUF and is not used)
-                int4 sv = sh.sbuf[stream];
-                for (int j = stream; j < padded; j += 2) {
-                    const int4 sn = sh.sbuf[j + 2];
+                // Block by block: the staged sources within reach of the
+                // block's sinks (box distance < rcut + margin; a dropped pair
+                // has r >= rcut and adds exactly zero), then the pairs.  The
+                // count is the walk's: every staged source times every active
+                // sink, as the reference evaluates them.
 #pragma unroll
-                    for (int k = 0; k < K; k++)
+                for (int k = 0; k < K; k++) {
+                    if (!((bact >> k) & 1u)) continue;
+                    const float lx = sh.box[k][0], hx = sh.box[k][1], ly = sh.box[k][2];
+                    const float hy = sh.box[k][3], lz = sh.box[k][4], hz = sh.box[k][5];
+                    int m = 0;
+                    for (int i0 = 0; i0 < nbuf; i0 += 64) {
+                        const int i = i0 + lane;
+                        bool keep = false;
+                        int4 sv = far;
+                        if (i < nbuf) {
+                            sv = sh.sbuf[i];
+                            const float fx = (float)sv.x, fy = (float)sv.y, fz = (float)sv.z;
+                            const float ex = fmaxf(fmaxf(lx - fx, fx - hx), 0.f);
+                            const float ey = fmaxf(fmaxf(ly - fy, fy - hy), 0.f);
+                            const float ez = fmaxf(fmaxf(lz - fz, fz - hz), 0.f);
+                            keep = !B.cull || fmaf(ex, ex, fmaf(ey, ey, ez * ez)) < B.cull2s;
+                        }
+                        const unsigned long long mk = __ballot(keep);
+                        if (keep) sh.cbuf[m + (int)count_below(mk)] = sv;
+                        m += __popcll(mk);
+                    }
+                    if (lane == 0 && (m & 1)) sh.cbuf[m] = far;
+                    __syncthreads();
+                    const int padded = (m + 1) & ~1;
+                    // one source ahead (the read past the end stays below
+                    // padded + 2 <= WALK_SBUF and is not used)
+                    int4 sv = sh.cbuf[stream];
+                    for (int j = stream; j < padded; j += 2) {
+                        const int4 sn = sh.cbuf[j + 2];
                         pp_pair<false>(B, ox[k], oy[k], oz[k], sv, pg[k][0], pg[k][1], pg[k][2], pg[k][3]);
-                    sv = sn;
+                        sv = sn;
+                    }
+                    __syncthreads();
                 }
                 npairs += (unsigned long long)nbuf * nact;
-                __syncthreads();
             }
             np = 0;
         };
