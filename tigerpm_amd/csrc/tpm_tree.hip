@@ -24,18 +24,19 @@
 // Walk, one wave64 per sink bucket of <= 128 sinks: lane l holds sinks
 // (l & 31) + 32 k for k < K = ceil(sinks / 32) and takes every other
 // multipole node and staged particle (source stream l >> 5), so a bucket of
-// 89 sinks keeps 93 % of the lanes' sink slots busy.  For each of
-// the 27 neighbour trees: nodes are taken 64 at a time from an LDS stack,
-// each lane tests one (cutoff: R - r_src - r_snk > rcut; far: R^2 >
+// 89 sinks keeps 93 % of the lanes' sink slots busy.  The roots of the 27
+// neighbour trees share one LDS stack; nodes are taken 64 at a time, each
+// lane tests one (cutoff: R - r_src - r_snk > rcut; far: R^2 >
 // (r_src + r_snk)^2 / theta^2); far nodes go to the multipole list, near
 // internal nodes push their children, near leaves go to the particle list
 // unless every active sink is beyond r_src / theta of them
 // (kick_treepm.cu:630-680).  The multipole list is applied with
 // compute_pc_interaction (greens_function + M2L, kernels.hpp:13-88); the
-// leaves' particles are staged in LDS and applied with the pair law of the
-// direct kernel.  The decisions do not depend on the visiting order, so they
-// are the reference's breadth-first walk's.  Then the mesh force and the
-// kick epilogue, as in the direct kernel.
+// leaves' particles are staged in LDS and, for each block of 32 sinks, those
+// within reach of the block are applied with the pair law of the direct
+// kernel (the others would add exact zeros).  The decisions do not depend on
+// the visiting order, so they are the reference's breadth-first walk's.
+// Then the mesh force and the kick epilogue, as in the direct kernel.
 #include "tpm_pp_device.hpp"
 
 #include <vector>
