@@ -667,7 +667,7 @@ struct WalkShared {
     int mlist[WALK_LIST];   // global node indices
     int plist[WALK_LIST];
     int4 sbuf[WALK_SBUF];   // staged sources of the pair list's leaves
-    int4 cbuf[WALK_SBUF];   // those within reach of one 32-sink block
+    int4 cbuf[WALK_SBUF + 4];  // those within reach of one block (+ far padding)
     float box[4][6];        // each block's active sinks: bounding box of the offsets
     int nb_base[27], nb_cnt[27];  // the neighbour trees' first node and node count
 };
@@ -675,11 +675,14 @@ struct WalkShared {
 // One sink bucket of <= 32 K sinks.  Lane l holds sinks (l & 31) + 32 k,
 // k < K, and works on source stream l >> 5 (every other multipole node and
 // staged particle); the two streams' sums meet at the end.
-template <int K>
+// HALF: the last block holds the <= 16 remaining sinks, lane l sink
+// (l & 15) of it with source stream l >> 4 (four streams, half the pair
+// iterations of a 32-sink block).
+template <int K, bool HALF>
 __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bucket& bk, WalkShared& sh) {
     const tpm_box_dev& B = a.B;
     const int lane = (int)lane_id();
-    const int slot = lane & 31, stream = lane >> 5;
+    const int slot = lane & 31, stream = lane >> 5, s4 = lane >> 4;
     const int ns = bk.end - bk.begin;
     // per sink slot: offsets from the bucket centre (exact int32, as the
     // direct kernel's; the position is the centre plus the offset) and the
@@ -690,7 +693,8 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
     int nact = 0;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        const int j = slot + 32 * k;
+        const bool hb = HALF && k == K - 1;  // this block's sinks are held four times
+        const int j = hb ? 32 * k + (lane & 15) : slot + 32 * k;
         ox[k] = oy[k] = oz[k] = 0;
         w[k] = -1;
         bool on = false;
@@ -703,7 +707,7 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
             on = a.o.rung[v.w] >= B.min_rung;
         }
         act |= on ? 1u << k : 0u;
-        nact += __popcll(__ballot(on && stream == 0));
+        nact += __popcll(__ballot(on && (hb ? s4 == 0 : stream == 0)));
 #pragma unroll
         for (int c = 0; c < 4; c++) pg[k][c] = cg[k][c] = 0.f;
     }
@@ -782,9 +786,13 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                     m0 = nn.m[0], m1 = nn.m[1], m2 = nn.m[2], m3 = nn.m[3];
                 }
 #pragma unroll
-                for (int k = 0; k < K; k++)
+                for (int k = 0; k < K; k++) {
+                    // the half block's sinks count once per stream pair: lanes
+                    // with (l & 16) == 0 apply the node
+                    if (HALF && k == K - 1 && (lane & 16)) continue;
                     pc_interaction(a, sdist(ox[k], bk.x[0], cx0), sdist(oy[k], bk.x[1], cy0),
                                    sdist(oz[k], bk.x[2], cz0), M, cg[k][0], cg[k][1], cg[k][2], cg[k][3]);
+                }
             }
             npc += (unsigned long long)nm * nact;
             nm = 0;
@@ -835,14 +843,16 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                         if (keep) sh.cbuf[m + (int)count_below(mk)] = sv;
                         m += __popcll(mk);
                     }
-                    if (lane == 0 && (m & 1)) sh.cbuf[m] = far;
+                    const bool hb = HALF && k == K - 1;
+                    const int S = hb ? 4 : 2, st = hb ? s4 : stream;  // source streams
+                    const int padded = (m + S - 1) / S * S;
+                    if (lane < padded - m) sh.cbuf[m + lane] = far;
                     __syncthreads();
-                    const int padded = (m + 1) & ~1;
                     // one source ahead (the read past the end stays below
-                    // padded + 2 <= WALK_SBUF and is not used)
-                    int4 sv = sh.cbuf[stream];
-                    for (int j = stream; j < padded; j += 2) {
-                        const int4 sn = sh.cbuf[j + 2];
+                    // padded + 4 <= WALK_SBUF + 4 and is not used)
+                    int4 sv = sh.cbuf[st];
+                    for (int j = st; j < padded; j += S) {
+                        const int4 sn = sh.cbuf[j + S];
                         pp_pair<false>(B, ox[k], oy[k], oz[k], sv, pg[k][0], pg[k][1], pg[k][2], pg[k][3]);
                         sv = sn;
                     }
@@ -958,6 +968,10 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
     for (int k = 0; k < K; k++)
 #pragma unroll
         for (int c = 0; c < 4; c++) {
+            if (HALF && k == K - 1) {  // four streams
+                pg[k][c] += __shfl_xor(pg[k][c], 16);
+                cg[k][c] += __shfl_xor(cg[k][c], 16);
+            }
             pg[k][c] += __shfl_xor(pg[k][c], 32);
             cg[k][c] += __shfl_xor(cg[k][c], 32);
             pg[k][c] = pg[k][c] * (c < 3 ? 0x1p64f : 0x1p32f) + cg[k][c];
@@ -967,11 +981,16 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
     if (use_mesh) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
+            const bool hb = HALF && k == K - 1;
             if ((act >> k) & 1u)
                 mesh_interp6_part(a.phi, B.M, (uint32_t)ox[k] + bk.x[0], (uint32_t)oy[k] + bk.x[1],
-                                  (uint32_t)oz[k] + bk.x[2], stream, 2, cg[k][0], cg[k][1], cg[k][2], cg[k][3]);
+                                  (uint32_t)oz[k] + bk.x[2], hb ? s4 : stream, hb ? 4 : 2, cg[k][0], cg[k][1],
+                                  cg[k][2], cg[k][3]);
 #pragma unroll
-            for (int c = 0; c < 4; c++) cg[k][c] += __shfl_xor(cg[k][c], 32);
+            for (int c = 0; c < 4; c++) {
+                if (hb) cg[k][c] += __shfl_xor(cg[k][c], 16);
+                cg[k][c] += __shfl_xor(cg[k][c], 32);
+            }
         }
     }
     if (stream != 0) return;
@@ -979,6 +998,7 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
 #pragma unroll
     for (int k = 0; k < K; k++) {
         if (!((act >> k) & 1u)) continue;
+        if (HALF && k == K - 1 && lane >= 16) continue;  // the half block's copies
         const float pgx = use_mesh ? -cg[k][0] * Mf : 0.f, pgy = use_mesh ? -cg[k][1] * Mf : 0.f;
         const float pgz = use_mesh ? -cg[k][2] * Mf : 0.f, pph = use_mesh ? cg[k][3] : 0.f;
         kick_epilogue(B, a.flags, a.o, w[k], a.o.rung[w[k]], pgx, pgy, pgz, pph, pg[k][0], pg[k][1], pg[k][2],
@@ -995,21 +1015,19 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
     const int bi = (wg & 7) * a.per_xcd + k;
     if (bi >= a.nbuckets) return;
     const tpm_tree_bucket bk = a.buckets[bi];
-    switch ((bk.end - bk.begin + 31) >> 5) {  // sinks per lane
-        case 1:
-            walk_bucket<1>(a, bk, sh);
-            break;
-        case 2:
-            walk_bucket<2>(a, bk, sh);
-            break;
-        case 3:
-            walk_bucket<3>(a, bk, sh);
-            break;
-        case 4:
-            walk_bucket<4>(a, bk, sh);
-            break;
-        default:  // an empty bucket (buckets hold <= 128 sinks)
-            break;
+    const int ns = bk.end - bk.begin;
+    const int K = (ns + 31) >> 5;          // sinks per lane
+    const bool half = ns - 32 * (K - 1) <= 16;  // the last block fits 16 lanes
+    switch (2 * K + (half ? 1 : 0)) {
+        case 2: walk_bucket<1, false>(a, bk, sh); break;
+        case 3: walk_bucket<1, true>(a, bk, sh); break;
+        case 4: walk_bucket<2, false>(a, bk, sh); break;
+        case 5: walk_bucket<2, true>(a, bk, sh); break;
+        case 6: walk_bucket<3, false>(a, bk, sh); break;
+        case 7: walk_bucket<3, true>(a, bk, sh); break;
+        case 8: walk_bucket<4, false>(a, bk, sh); break;
+        case 9: walk_bucket<4, true>(a, bk, sh); break;
+        default: break;  // an empty bucket (buckets hold <= 128 sinks)
     }
 }
 
