@@ -596,6 +596,7 @@ struct WalkArgs {
     int n;                               // particles
     float rcut, theta2inv, inv2rs, tsp;  // box units
     float e2;                            // -log2(e) inv2rs^2: exp(-r0^2) = exp2(r^2 e2)
+    float thetainv;                      // 1 / theta
     unsigned flags;
 };
 
@@ -893,6 +894,34 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                 is_open = !cutoff && !far_node && !leaf;
                 is_leaf = !cutoff && !far_node && leaf;
             }
+            // A near leaf is decided here when every active block's box lies
+            // clearly within r_src / theta of it (some active sink is near:
+            // pairs) or clearly beyond (no sink is: multipole); margins of
+            // 512 raw units and 1e-5 cover the float offsets' rounding and the
+            // exact test's FP32 distances, so the outcome is the per-sink
+            // test's.  The rest are tested sink by sink below.
+            bool leaf_near = false, leaf_far = false;
+            if (is_leaf && B.cull) {
+                const float qx = (float)(int)(nd.x[0] - bk.x[0]), qy = (float)(int)(nd.x[1] - bk.x[1]);
+                const float qz = (float)(int)(nd.x[2] - bk.x[2]);
+                const float R = nd.radius * a.thetainv * 0x1p32f;
+                const float rin = R * (1.f - 1e-5f) - 512.f, rout = R * (1.f + 1e-5f) + 512.f;
+                bool anear = false, afar = true;
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    if (!((bact >> k) & 1u)) continue;
+                    const float lx = sh.box[k][0], hx = sh.box[k][1], ly = sh.box[k][2];
+                    const float hy = sh.box[k][3], lz = sh.box[k][4], hz = sh.box[k][5];
+                    const float ex = fmaxf(fmaxf(lx - qx, qx - hx), 0.f), ey = fmaxf(fmaxf(ly - qy, qy - hy), 0.f);
+                    const float ez = fmaxf(fmaxf(lz - qz, qz - hz), 0.f);
+                    const float fx = fmaxf(qx - lx, hx - qx), fy = fmaxf(qy - ly, hy - qy);
+                    const float fz = fmaxf(qz - lz, hz - qz);
+                    anear = anear || (rin > 0.f && fmaf(fx, fx, fmaf(fy, fy, fz * fz)) < rin * rin);
+                    afar = afar && fmaf(ex, ex, fmaf(ey, ey, ez * ez)) > rout * rout;
+                }
+                leaf_near = anear;
+                leaf_far = !anear && afar;
+            }
             // near internal nodes: children onto the stack
             const unsigned long long mo = __ballot(is_open);
             const int nopen = __popcll(mo);
@@ -906,16 +935,24 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                 sh.stack[at + 1] = (int)(((unsigned)nbr << 27) | (unsigned)nd.child[0]);
             }
             sp += 2 * nopen;
-            // far nodes: the multipole list
-            const unsigned long long mm = __ballot(is_multi);
+            // far nodes and clearly distant leaves: the multipole list
+            const unsigned long long mm = __ballot(is_multi || leaf_far);
             if (nm + __popcll(mm) > WALK_LIST) {
                 __syncthreads();
                 flush_pc();
             }
-            if (is_multi) sh.mlist[nm + (int)count_below(mm)] = idx;
+            if (is_multi || leaf_far) sh.mlist[nm + (int)count_below(mm)] = idx;
             nm += __popcll(mm);
-            // near leaves: PC when every active sink is beyond r_src / theta
-            unsigned long long ml = __ballot(is_leaf);
+            // clearly near leaves: the pair list
+            const unsigned long long mq = __ballot(leaf_near);
+            if (np + __popcll(mq) > WALK_LIST) {
+                __syncthreads();
+                flush_pp();
+            }
+            if (leaf_near) sh.plist[np + (int)count_below(mq)] = idx;
+            np += __popcll(mq);
+            // the other near leaves: PC when every active sink is beyond r_src / theta
+            unsigned long long ml = __ballot(is_leaf && !leaf_near && !leaf_far);
             while (ml) {
                 const int j = __ffsll((long long)ml) - 1;
                 ml &= ml - 1;
@@ -1147,6 +1184,7 @@ tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, in
     a.rcut = (float)ctx->p.rcut;
     const float theta = 0.4f;
     a.theta2inv = 1.0f / (theta * theta);
+    a.thetainv = 1.0f / theta;
     a.inv2rs = 1.0f / (float)ctx->p.rs / 2.0f;
     a.e2 = (float)(-1.4426950408889634 / (4.0 * ctx->p.rs * ctx->p.rs));
     a.tsp = 2.0f / sqrtf((float)M_PI);
