@@ -742,7 +742,8 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
         const int cx = bk.cell % C, cy = (bk.cell / C) % C, cz = bk.cell / (C * C);
         const float rcut = a.rcut, t2 = a.theta2inv;
         const float sr = bk.radius;
-        // Far dummy that pads a staged list to an even length: r >= rcut.
+        // Far dummy that pads a compacted list to a multiple of the stream
+        // count: r >= rcut from every sink.
         const int4 far = make_int4(0x40000000, 0x40000000, 0x40000000, 0);
         // the 27 neighbour trees (x outermost, z innermost, kick_treepm.cu:930-940)
         if (lane < 27) {
