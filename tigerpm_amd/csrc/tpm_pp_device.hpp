@@ -58,9 +58,10 @@ __device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, in
         t1 * fmaf(t1, fmaf(t1, fmaf(t1, fmaf(t1, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f),
                   0.254829592f);
     const float er = e * rinv;
-    // erfc(x) / r and (erfc(x) + 2 x / sqrt(pi) exp(-x^2)) / r^3
+    // erfc(x) / r and (erfc(x) + 2 x / sqrt(pi) exp(-x^2)) / r^3, the latter
+    // as (erfc(x) / r + (2 / sqrt(pi)) / (2 rs) exp(-x^2)) / r^2
     float rinv1 = poly * er;
-    float rinv3 = fmaf(a.tsp_inv2rs_s, r, poly) * er * (rinv * rinv);
+    float rinv3 = fmaf(a.tsp_inv2rs_s, e, rinv1) * (rinv * rinv);
     if (r2 <= a.h2s) {
         // Closer than h: Plummer polynomial plus the (erfc - 1) correction
         // (kick_treepm.cu:263-276).  At r = 0 (the sink itself) only d1
