@@ -3,8 +3,8 @@ TSC), checked through properties that do not need an O(N) oracle run:
 the force against the FP64 Ewald sum on sampled particles, Newton's third
 law for the pair forces (every pair is evaluated from both ends with exactly
 negated offsets), every particle kicked, the candidate-pair count equal to
-the closed form from the cell counts, and a repeated kick's forces
-bit-identical."""
+the closed form from the cell counts, a repeated kick's forces
+bit-identical, and the tree walk finishing with its multipole error."""
 import numpy as np
 import pytest
 
@@ -56,3 +56,14 @@ def run_full_size(gpu_ctx):
     e = gpu_ctx.download()
     for k in ("x", "y", "z", "gx", "gy", "gz", "pot"):
         assert np.array_equal(d[k], e[k]), k
+    # the reference's tree walk on the same particles: it finishes at this
+    # size (no stack or list exhaustion) and carries its multipole error
+    print("512^3 tree kick", flush=True)
+    gpu_ctx.chainmesh_create()
+    gpu_ctx.gravity_long_compute()
+    st = gpu_ctx.kick(flags=T.TPM_KICK_TREE | T.TPM_SKIP_KICK)
+    assert st.pc_interactions > 0 and st.executed_pairs > 0
+    t = gpu_ctx.download()
+    Gt = np.stack([t["gx"][s], t["gy"][s], t["gz"][s]], 1)
+    err = rel_rms(Gt, np.stack([gx, gy, gz], 1) * p.GM)
+    assert 1e-3 < err < 4e-3
