@@ -224,21 +224,19 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 7) void pp_kick_kernel(PPArgs a) 
                     sp += __shfl_xor(sp, o);
                 }
             }
-            if constexpr (!PERSIST) {
-                if (!active || stream != 0) return;
-            } else {
-                if (!active || stream != 0) goto next_tile;
-            }
             {
-                // Then the kick.
-                const float Mf = (float)B.M;
-                const float pgx = use_mesh ? -sx * Mf : 0.f, pgy = use_mesh ? -sy * Mf : 0.f;
-                const float pgz = use_mesh ? -sz * Mf : 0.f, pph = use_mesh ? sp : 0.f;
+                // Then the kick, by each active sink's stream-0 lane.
                 const tpm_kick_out o{a.gx, a.gy, a.gz, a.pot, a.vx, a.vy, a.vz, a.rung, a.counters};
-                kick_epilogue(B, a.flags, o, i, r, pgx, pgy, pgz, pph, gx, gy, gz, ph);
+                int nr = -1;
+                if (active && stream == 0) {
+                    const float Mf = (float)B.M;
+                    const float pgx = use_mesh ? -sx * Mf : 0.f, pgy = use_mesh ? -sy * Mf : 0.f;
+                    const float pgz = use_mesh ? -sz * Mf : 0.f, pph = use_mesh ? sp : 0.f;
+                    nr = kick_epilogue(B, a.flags, o, i, r, pgx, pgy, pgz, pph, gx, gy, gz, ph);
+                }
+                max_rung_update(o, nr);
             }
         }
-    next_tile:
         if constexpr (!PERSIST) return;
     }
 }

@@ -231,9 +231,11 @@ struct tpm_kick_out {
 // rung_dt[r] = 2^-r): total unit-mass force = mesh (pg*) + short range
 // (g*), scaled by GM; potential with phi0; velocity and rung update unless
 // TPM_SKIP_KICK.
-__device__ __forceinline__ void kick_epilogue(const tpm_box_dev& B, unsigned flags, const tpm_kick_out& o, int i,
-                                              int r, float pgx, float pgy, float pgz, float pph, float gx, float gy,
-                                              float gz, float ph) {
+// Returns the new rung (-1 without the kick); the caller reduces it over
+// the wave and updates the maximum rung once per wave.
+__device__ __forceinline__ int kick_epilogue(const tpm_box_dev& B, unsigned flags, const tpm_kick_out& o, int i,
+                                             int r, float pgx, float pgy, float pgz, float pph, float gx, float gy,
+                                             float gz, float ph) {
     const float GM = B.GM;
     const float g0 = (pgx + gx) * GM;
     const float g1 = (pgy + gy) * GM;
@@ -265,8 +267,16 @@ __device__ __forceinline__ void kick_epilogue(const tpm_box_dev& B, unsigned fla
         o.vy[i] = vy;
         o.vz[i] = vz;
         o.rung[i] = (int8_t)r;
-        if (o.counters) atomicMax(&o.counters[1], (unsigned long long)r);
+        return r;
     }
+    return -1;
+}
+
+// The maximum rung of a wave's kicked sinks into counters[1]: one atomic per
+// wave (all lanes must call it; lanes without a kick pass -1).
+__device__ __forceinline__ void max_rung_update(const tpm_kick_out& o, int r) {
+    const int m = wave_max(r);
+    if (lane_id() == 0 && m >= 0 && o.counters) atomicMax(&o.counters[1], (unsigned long long)m);
 }
 
 }  // namespace

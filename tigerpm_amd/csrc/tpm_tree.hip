@@ -1031,17 +1031,18 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
             }
         }
     }
-    if (stream != 0) return;
     const float Mf = (float)B.M;
+    int rmax = -1;  // the new rungs' maximum (the kicks are done by stream 0)
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        if (!((act >> k) & 1u)) continue;
+        if (stream != 0 || !((act >> k) & 1u)) continue;
         if (HALF && k == K - 1 && lane >= 16) continue;  // the half block's copies
         const float pgx = use_mesh ? -cg[k][0] * Mf : 0.f, pgy = use_mesh ? -cg[k][1] * Mf : 0.f;
         const float pgz = use_mesh ? -cg[k][2] * Mf : 0.f, pph = use_mesh ? cg[k][3] : 0.f;
-        kick_epilogue(B, a.flags, a.o, w[k], a.o.rung[w[k]], pgx, pgy, pgz, pph, pg[k][0], pg[k][1], pg[k][2],
-                      pg[k][3]);
+        rmax = max(rmax, kick_epilogue(B, a.flags, a.o, w[k], a.o.rung[w[k]], pgx, pgy, pgz, pph, pg[k][0],
+                                       pg[k][1], pg[k][2], pg[k][3]));
     }
+    max_rung_update(a.o, rmax);
 }
 
 __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
