@@ -461,11 +461,11 @@ def main():
         out["roofline"]["traffic_source"] = t["source"]
     # Ceiling of the pair loop at the kernel's occupancy: the same pair body
     # timed alone over sources held in LDS (tpm_probe_pair_rate).
-    body = ctx.probe_pair_rate(7)
+    body = ctx.probe_pair_rate(8)
     kernel_rate = st.executed_pairs / (pp_ms * 1e-3)
     out["roofline"]["pair_body_ceiling"] = {
         "pairs_per_s": body, "kernel_executed_pairs_per_s": kernel_rate, "frac": kernel_rate / body,
-        "waves_per_simd": 7, "note": "pp_kick_kernel's executed pairs/s over its pair body's rate alone"}
+        "waves_per_simd": 8, "note": "pp_kick_kernel's executed pairs/s over its pair body's rate alone"}
     log("pair body alone %.4g pairs/s; kernel %.4g executed pairs/s (%.3f)" % (body, kernel_rate, kernel_rate / body))
     if not args.no_ewald:
         log("FP64 Ewald on %d samples..." % w["samples"])

@@ -224,7 +224,7 @@ TPM_API tpm_status tpm_profile_read(tpm_ctx* ctx, double* ms, int64_t* pp_launch
 /* Issue-rate ceiling of the pair law.  No counterpart in the reference: a
  * measurement aid for the roofline.  Times the direct kernel's pair body
  * alone (the same device function, 16-sink x 4-stream lane layout, flush
- * loop and 7-wave launch bound) over 256 sources held in LDS, with the
+ * loop and 8-wave launch bound) over 256 sources held in LDS, with the
  * current parameters' constants, at waves_per_simd (1..8) waves per SIMD,
  * and returns pair evaluations per second.  Allocates a small buffer. */
 TPM_API tpm_status tpm_probe_pair_rate(tpm_ctx* ctx, int waves_per_simd, double* pairs_per_second);

@@ -385,6 +385,6 @@ def test_pair_rate_probe(gpu_ctx):
     gpu_ctx.set_params(T.baseline_params(64, 64, 3))
     with pytest.raises(T.TpmError, match="waves_per_simd"):
         gpu_ctx.probe_pair_rate(9)
-    r4, r7 = gpu_ctx.probe_pair_rate(4), gpu_ctx.probe_pair_rate(7)
-    print("pair body rate", r4, r7, flush=True)
-    assert 1e11 < r4 < 1e14 and 1e11 < r7 < 1e14
+    r4, r8 = gpu_ctx.probe_pair_rate(4), gpu_ctx.probe_pair_rate(8)
+    print("pair body rate", r4, r8, flush=True)
+    assert 1e11 < r4 < 1e14 and 1e11 < r8 < 1e14

@@ -357,7 +357,7 @@ class Context:
         return dict(chainmesh_ms=ms[0], pm_ms=ms[1], pp_ms=ms[2], step_ms=ms[3], pp_launches=npp.value,
                     steps=nst.value)
 
-    def probe_pair_rate(self, waves_per_simd=7):
+    def probe_pair_rate(self, waves_per_simd=8):
         """Pairs/s of the direct kernel's pair body alone (the ceiling of its
         pair loop at that occupancy)."""
         r = ctypes.c_double()

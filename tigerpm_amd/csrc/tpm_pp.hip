@@ -71,7 +71,9 @@ struct PPArgs {
 // BATCH = true: tile.w names the box whose constants apply (device table).
 // PME = true: kick_pme's pair law (the caller turns culling and phi0 off).
 template <int TS, bool BATCH, bool PERSIST, bool PME = false>
-__global__ __launch_bounds__(64, PERSIST ? 1 : 7) void pp_kick_kernel(PPArgs a) {
+// Occupancy: 8 waves per SIMD for one box (64 VGPRs, no spills); the
+// batched kernel keeps 7 (72 VGPRs), at 64 its box-table values would spill.
+__global__ __launch_bounds__(64, PERSIST ? 1 : (BATCH ? 7 : 8)) void pp_kick_kernel(PPArgs a) {
     constexpr int SPLIT = 64 / TS;  // source streams per wave
     __shared__ int4 sbuf[SBUF];
     const int nt = *a.ntiles;

@@ -193,8 +193,12 @@ __device__ __forceinline__ void mesh_interp6_part(const float* __restrict__ phi,
             v = fmaf(dwy[bb], s0, v);
             t = fmaf(wy[bb], s1, t);
         }
+        // x weight of this plane, made here: hoisted out of the loop (the
+        // plane index is per lane) the six candidates would spill.
+        float xa = X[0];
+        asm volatile("" : "+v"(xa));
         float wx, dwx;
-        interp6_weight(X[0], aa, &wx, &dwx);
+        interp6_weight(xa, aa, &wx, &dwx);
         sx = fmaf(dwx, u, sx);
         sy = fmaf(wx, v, sy);
         sz = fmaf(wx, t, sz);

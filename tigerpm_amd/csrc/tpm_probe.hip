@@ -3,7 +3,7 @@
 // pair_rate_kernel runs pp_pair (the device function pp_kick_kernel uses)
 // with pp_kick_kernel's 16-sink tile layout (lane l: sink l % 16, source
 // stream l / 16), its flush loop (two pairs per trip, the next source read
-// one step ahead from LDS) and its 7-wave launch bound, over 256 sources
+// one step ahead from LDS) and its 8-wave launch bound, over 256 sources
 // held in LDS, with nothing else in the loop: no staging, culling or
 // epilogue.  Sources fill a cube of side 2 rcut around the sinks, so about
 // half of the pairs are inside rcut and the softening branch is rare, as in
@@ -17,7 +17,7 @@ constexpr int PROBE_SRC = 256;
 constexpr int PROBE_TS = 16;
 constexpr int PROBE_SPLIT = 64 / PROBE_TS;
 
-__global__ __launch_bounds__(64, 7) void pair_rate_kernel(tpm_box_dev B, int iters, float* __restrict__ out) {
+__global__ __launch_bounds__(64, 8) void pair_rate_kernel(tpm_box_dev B, int iters, float* __restrict__ out) {
     __shared__ int4 sbuf[PROBE_SRC + PROBE_SPLIT];
     const int lane = (int)lane_id();
     const int slot = lane % PROBE_TS, stream = lane / PROBE_TS;
