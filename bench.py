@@ -166,6 +166,36 @@ def tree_leg(T, ctx, p, steps, nsamp, seed, ewald=True):
     return out
 
 
+def useful_pairs(ctx, p, nsamp, seed):
+    """Pairs inside rcut per particle, counted over sampled sinks from the
+    downloaded positions (minimum-image int32 offsets, as the kernel forms
+    them) and the 27 chain-mesh cells around each sink; beside it the
+    kernel's candidate and executed pairs per particle."""
+    d = ctx.download()
+    n = ctx.size()
+    C = p.chain_dim
+    cb = ctx.chainmesh_cells()
+    x, y, z = d["x"], d["y"], d["z"]
+    rng = np.random.default_rng(seed + 2000)
+    sinks = np.sort(rng.choice(n, min(nsamp, n), replace=False))
+    cell = np.searchsorted(cb, sinks, side="right") - 1
+    rc2 = (p.rcut * 2.0 ** 32) ** 2
+    counts = np.empty(len(sinks), np.int64)
+    for k, (i, c) in enumerate(zip(sinks, cell)):
+        cx, cy, cz = c % C, (c // C) % C, c // (C * C)
+        idx = np.concatenate([np.arange(cb[j], cb[j + 1]) for j in
+                              [(((cz + dz) % C) * C + (cy + dy) % C) * C + (cx + dx) % C
+                               for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]])
+        r2 = np.zeros(len(idx))
+        for a in (x, y, z):
+            off = (a[idx] - a[i]).view(np.int32).astype(np.float64)
+            r2 += off * off
+        counts[k] = int((r2 < rc2).sum()) - 1  # the sink itself
+    return {"samples": int(len(sinks)), "per_particle": float(counts.mean()),
+            "total_estimate": float(counts.mean()) * n,
+            "note": "pairs with r < rcut (sampled sinks); the roofline credits every candidate pair"}
+
+
 def ewald_error(ctx, d, nsamp, seed):
     """Vector rms force error of downloaded state d against the FP64 Ewald
     sum of the same particles (resident in ctx)."""
@@ -467,6 +497,12 @@ def main():
         "pairs_per_s": body, "kernel_executed_pairs_per_s": kernel_rate, "frac": kernel_rate / body,
         "waves_per_simd": 8, "note": "pp_kick_kernel's executed pairs/s over its pair body's rate alone"}
     log("pair body alone %.4g pairs/s; kernel %.4g executed pairs/s (%.3f)" % (body, kernel_rate, kernel_rate / body))
+    up = useful_pairs(ctx, p, 2048, w["seed"])
+    up["candidate_per_particle"] = W / n
+    up["executed_per_particle"] = st.executed_pairs / n
+    out["roofline"]["useful_pairs"] = up
+    log("pairs per particle: useful %.1f, executed %.1f, candidate %.1f"
+        % (up["per_particle"], up["executed_per_particle"], up["candidate_per_particle"]))
     if not args.no_ewald:
         log("FP64 Ewald on %d samples..." % w["samples"])
         out["accuracy"] = accuracy(ctx, p, w["samples"], w["seed"])

@@ -4,6 +4,7 @@ import os
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 import oracle as O
@@ -127,3 +128,36 @@ def test_batch_without_gpu_fails_cleanly():
     assert T.batch_arena_bytes(1 << 20, 1 << 20, 8, 3, 1) > 100 * (1 << 20)
     with pytest.raises(T.TpmError):
         T.Batch(1 << 16, 1 << 16, 4, 3, 0)
+
+
+def test_bench_useful_pair_count_matches_uniform_density():
+    """bench.py's pairs-inside-rcut count, on uniform particles sorted into
+    the chain mesh, equals the closed form (4/3) pi rcut^3 N per particle
+    within its sampling error."""
+    sys.path.insert(0, ROOT)
+    import bench
+    p = T.baseline_params(32, 32, 2)
+    n = 32 ** 3
+    rng = np.random.default_rng(3)
+    x, y, z = [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    key = O.chain_keys(x, y, z, p.chain_dim)
+    o = np.argsort(key, kind="stable")
+    x, y, z, key = x[o], y[o], z[o], key[o]
+    cb = np.searchsorted(key // 64, np.arange(p.chain_dim ** 3 + 1)).astype(np.int64)
+
+    class Resident:
+        params = p
+
+        def download(self):
+            return {"x": x, "y": y, "z": z}
+
+        def size(self):
+            return n
+
+        def chainmesh_cells(self):
+            return cb
+
+    r = bench.useful_pairs(Resident(), p, 1024, 1)
+    expect = 4.0 / 3.0 * np.pi * p.rcut ** 3 * n
+    # Poisson error of the mean over 1024 sinks is ~1 %
+    assert abs(r["per_particle"] / expect - 1) < 0.03
