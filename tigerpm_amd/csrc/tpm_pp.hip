@@ -70,10 +70,32 @@ struct PPArgs {
 //     and then helps the others.  Every queue ends, so every workgroup exits.
 // BATCH = true: tile.w names the box whose constants apply (device table).
 // PME = true: kick_pme's pair law (the caller turns culling and phi0 off).
+// The box record of a tile.  A batch's table is read word by word through
+// the constant address space, so the record arrives by scalar loads in SGPRs
+// (the table is not written during the launch); read as ordinary global
+// memory, its fields would take vector registers.
+template <bool BATCH>
+__device__ __forceinline__ tpm_box_dev tile_box(const PPArgs& a, int w) {
+    if constexpr (BATCH) {
+        static_assert(sizeof(tpm_box_dev) % 4 == 0, "box record of whole words");
+        constexpr int NW = (int)(sizeof(tpm_box_dev) / 4);
+        using cword = const __attribute__((address_space(4))) int;
+        cword* src = (cword*)(a.boxes + __builtin_amdgcn_readfirstlane(w));
+        int words[NW];
+#pragma unroll
+        for (int k = 0; k < NW; k++) words[k] = src[k];
+        tpm_box_dev b;
+        __builtin_memcpy(&b, words, sizeof b);
+        return b;
+    } else {
+        return a.box0;
+    }
+}
+
 template <int TS, bool BATCH, bool PERSIST, bool PME = false>
-// Occupancy: 8 waves per SIMD for one box (64 VGPRs, no spills); the
-// batched kernel keeps 7 (72 VGPRs), at 64 its box-table values would spill.
-__global__ __launch_bounds__(64, PERSIST ? 1 : (BATCH ? 7 : 8)) void pp_kick_kernel(PPArgs a) {
+// Occupancy: 8 waves per SIMD (64 VGPRs, no spills) for one box and for a
+// batch (whose box record tile_box keeps in SGPRs).
+__global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) {
     constexpr int SPLIT = 64 / TS;  // source streams per wave
     __shared__ int4 sbuf[SBUF];
     const int nt = *a.ntiles;
@@ -103,7 +125,7 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : (BATCH ? 7 : 8)) void pp_kick_ker
             }
         }
         const int4 tile = a.tiles[t];
-        const tpm_box_dev& B = BATCH ? a.boxes[__builtin_amdgcn_readfirstlane(tile.w)] : a.box0;
+        const tpm_box_dev B = tile_box<BATCH>(a, tile.w);
         const int lane = (int)lane_id();
         const int slot = lane % TS;
         const int stream = lane / TS;
