@@ -59,7 +59,8 @@ int64_t arena_bytes(const tpm_batch_limits& l) {
     const int64_t per_k = 8 + 4 + (l.interlace ? 8 : 0);  // spectrum, Green's, shifted spectrum
     const int64_t per_c = 4 * 4;                          // counts, cell_begin, tiles per cell, tile offsets
     return c.p * per_p + c.m * per_m + c.k * per_k + (c.c + 2) * per_c + c.t * 16 +
-           ((c.c + 2) / 2048 + 2) * 4 + c.nbox * (int64_t)(sizeof(tpm_box_dev) + sizeof(tpm_kregion)) + 256;
+           ((c.c + 2) / 2048 + 2) * 4 + c.nbox * (int64_t)(sizeof(tpm_box_dev) + sizeof(tpm_kregion)) +
+           (int64_t)sizeof(unsigned long long) * TPM_COUNTER_WORDS + 128;
 }
 
 template <class T>
@@ -150,7 +151,7 @@ tpm_status enqueue_step(tpm_batch* bt, const tpm_batch_layout& L, unsigned flags
         if (bt->stage_events) TPM_HIP(bt, hipEventRecord(bt->ev_stage[k], bt->stream));
         return TPM_OK;
     };
-    TPM_TRY(tpm_batch_fill_u64(bt, bt->counters, 16));
+    TPM_TRY(tpm_batch_fill_u64(bt, bt->counters, TPM_COUNTER_WORDS));
     TPM_TRY(tpm_batch_chainmesh(bt, L));
     TPM_TRY(mark(0));
     if (!(flags & TPM_SKIP_PM)) TPM_TRY(tpm_batch_pm(bt, L));
@@ -245,9 +246,12 @@ tpm_status run_step(tpm_batch* bt, unsigned flags, tpm_batch_stats* st) {
     after_sort(bt, !(flags & TPM_SKIP_KICK));
     if (st) {
         TPM_TRY(tpm_batch_candidates(bt, L));
-        unsigned long long hc[4];
-        TPM_HIP(bt, hipMemcpyAsync(hc, bt->counters, sizeof(hc), hipMemcpyDeviceToHost, bt->stream));
+        std::vector<unsigned long long> hw(TPM_COUNTER_WORDS);
+        TPM_HIP(bt, hipMemcpyAsync(hw.data(), bt->counters, sizeof(unsigned long long) * TPM_COUNTER_WORDS,
+                                   hipMemcpyDeviceToHost, bt->stream));
         TPM_HIP(bt, hipStreamSynchronize(bt->stream));
+        unsigned long long hc[4];
+        tpm_reduce_counters(hw.data(), hc);
         float ms = 0.f;
         TPM_HIP(bt, hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
         st->executed_pairs = (double)hc[0];
@@ -430,7 +434,7 @@ extern "C" tpm_status tpm_batch_create(int device, const tpm_batch_limits* l, tp
         alloc(bt, &bt->tiles, c.t) || alloc(bt, &bt->rho_q, c.m) || alloc(bt, &bt->rho, c.m) ||
         alloc(bt, &bt->phi, c.m) || alloc(bt, &bt->rho_hat, c.k) || alloc(bt, &bt->green, c.k) ||
         (l->interlace && alloc(bt, &bt->rho_hat1, c.k)) || alloc(bt, &bt->d_boxes, l->max_boxes) ||
-        alloc(bt, &bt->d_kreg, l->max_boxes) || alloc(bt, &bt->counters, 16))
+        alloc(bt, &bt->d_kreg, l->max_boxes) || alloc(bt, &bt->counters, TPM_COUNTER_WORDS))
         return fail(TPM_ERR_BUDGET);
     if (l->interlace && alloc(bt, &bt->rho1, c.m)) return fail(TPM_ERR_BUDGET);
     for (int k = 0; k < tpm_batch::NSIDE; k++)

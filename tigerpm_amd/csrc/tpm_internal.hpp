@@ -3,6 +3,27 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+// The direct kernel's per-kick counters are spread over slots on separate
+// 128-byte lines (slot = workgroup index mod TPM_COUNTER_SLOTS), so the
+// millions of per-wave atomics of a launch do not all meet on one address.
+constexpr int TPM_COUNTER_SLOTS = 64;
+constexpr int TPM_COUNTER_STRIDE = 16;
+constexpr int TPM_COUNTER_WORDS = TPM_COUNTER_SLOTS * TPM_COUNTER_STRIDE;
+
+// The counters of all slots (h: TPM_COUNTER_WORDS words copied from the
+// device) -> out[0] executed pairs, [1] max rung, [2] candidate pairs,
+// [3] multipole interactions.
+inline void tpm_reduce_counters(const unsigned long long* h, unsigned long long out[4]) {
+    out[0] = out[1] = out[2] = out[3] = 0;
+    for (int k = 0; k < TPM_COUNTER_SLOTS; k++) {
+        const unsigned long long* c = h + k * TPM_COUNTER_STRIDE;
+        out[0] += c[0];
+        out[1] = c[1] > out[1] ? c[1] : out[1];
+        out[2] += c[2];
+        out[3] += c[3];
+    }
+}
 #include <rocfft/rocfft.h>
 
 #include <array>
@@ -144,7 +165,10 @@ struct tpm_ctx {
     bool pm_valid = false;
 
     // Diagnostics.
-    unsigned long long* counters = nullptr;  // [0] executed pairs, [1] max rung
+    // TPM_COUNTER_SLOTS slots of TPM_COUNTER_STRIDE words: [0] executed
+    // pairs, [1] max rung, [2] candidate pairs, [3] multipole interactions
+    // (tpm_reduce_counters)
+    unsigned long long* counters = nullptr;
     double* dscratch = nullptr;
     hipEvent_t ev[8] = {};
 
@@ -375,7 +399,9 @@ struct tpm_batch {
     hipEvent_t ev_fork = nullptr, ev_join[NSIDE] = {};
     float* rho1 = nullptr;  // float image of the shifted deposit (interlace)
     tpm_ctx* scratch = nullptr;  // initial conditions are generated here
-    unsigned long long* counters = nullptr;  // [0] executed pairs [1] max rung [2] candidates [8..15] PP queues
+    // counter slots as in tpm_ctx (tpm_reduce_counters); words 8..15 of
+    // slot 0: the persistent PP launch's tile queues
+    unsigned long long* counters = nullptr;
     tpm_batch_tuning tune{16, 0, 0, 0};
     int persist_grid = 0;  // workgroups of the persistent PP launch
 

@@ -22,6 +22,8 @@
 // is the reference's up to exact power-of-two factors.  The self pair is
 // evaluated like any other source, as in the reference: inside the
 // softening branch it contributes -35/16 h^-1 to phi and nothing to g.
+#include <vector>
+
 #include "tpm_pp_device.hpp"
 
 #include <climits>
@@ -126,6 +128,8 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) 
         }
         const int4 tile = a.tiles[t];
         const tpm_box_dev B = tile_box<BATCH>(a, tile.w);
+        unsigned long long* const cnt =
+            a.counters ? a.counters + TPM_COUNTER_STRIDE * (blockIdx.x % TPM_COUNTER_SLOTS) : nullptr;
         const int lane = (int)lane_id();
         const int slot = lane % TS;
         const int stream = lane / TS;
@@ -225,8 +229,8 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) 
                     gz += __shfl_xor(gz, o);
                     ph += __shfl_xor(ph, o);
                 }
-                if (lane == 0 && a.counters)
-                    atomicAdd(&a.counters[0],
+                if (lane == 0 && cnt)
+                    atomicAdd(&cnt[0],
                               staged * (unsigned long long)__popcll(amask & ((TS == 64) ? ~0ull : ((1ull << TS) - 1))));
                 // back to box units: g in 2^-64, phi in 2^-32
                 gx *= 0x1p64f;
@@ -250,7 +254,7 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) 
             }
             {
                 // Then the kick, by each active sink's stream-0 lane.
-                const tpm_kick_out o{a.gx, a.gy, a.gz, a.pot, a.vx, a.vy, a.vz, a.rung, a.counters};
+                const tpm_kick_out o{a.gx, a.gy, a.gz, a.pot, a.vx, a.vy, a.vz, a.rung, cnt};
                 int nr = -1;
                 if (active && stream == 0) {
                     const float Mf = (float)B.M;
@@ -383,8 +387,8 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
         return tpm_fail(ctx, TPM_ERR_STATE, "tpm_kick: PM mesh not computed (call tpm_gravity_long_compute)");
     if (t0 <= 0.0) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: t0 must be positive");
     hipStream_t s = ctx->stream;
-    if (!ctx->counters) TPM_HIP(ctx, hipMalloc(&ctx->counters, sizeof(unsigned long long) * 4));
-    TPM_TRY(tpm_fill_u64(ctx, ctx->counters, 4));
+    if (!ctx->counters) TPM_HIP(ctx, hipMalloc(&ctx->counters, sizeof(unsigned long long) * TPM_COUNTER_WORDS));
+    TPM_TRY(tpm_fill_u64(ctx, ctx->counters, TPM_COUNTER_WORDS));
 
     PPArgs a{};
     a.x = ctx->x;
@@ -441,13 +445,16 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
         candidate_pairs_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, 0, nullptr, ctx->C, ctx->cell_begin,
                                                                     ctx->counters + 2);
         TPM_HIP(ctx, hipGetLastError());
-        unsigned long long hc[4];
+        std::vector<unsigned long long> hw(TPM_COUNTER_WORDS);
         int32_t nt = 0;
-        TPM_HIP(ctx, hipMemcpyAsync(hc, ctx->counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+        TPM_HIP(ctx, hipMemcpyAsync(hw.data(), ctx->counters, sizeof(unsigned long long) * TPM_COUNTER_WORDS,
+                                    hipMemcpyDeviceToHost, s));
         TPM_HIP(ctx, hipMemcpyAsync(&nt, ctx->ntiles, sizeof(nt), hipMemcpyDeviceToHost, s));
         TPM_HIP(ctx, hipStreamSynchronize(s));
         float ms = 0.f;
         TPM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+        unsigned long long hc[4];
+        tpm_reduce_counters(hw.data(), hc);
         stats->executed_pairs = (double)hc[0];
         stats->max_rung = (int)hc[1];
         stats->candidate_pairs = (double)hc[2];

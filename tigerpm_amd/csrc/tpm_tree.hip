@@ -996,9 +996,12 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
         if (lane == 0) atomicOr(&a.flags_dev[1], 1);
         return;
     }
-    if (lane == 0 && a.o.counters) {
-        atomicAdd(&a.o.counters[0], npairs);
-        atomicAdd(&a.o.counters[3], npc);
+    // this workgroup's counter slot (tpm_reduce_counters)
+    tpm_kick_out o = a.o;
+    if (o.counters) o.counters += TPM_COUNTER_STRIDE * (blockIdx.x % TPM_COUNTER_SLOTS);
+    if (lane == 0 && o.counters) {
+        atomicAdd(&o.counters[0], npairs);
+        atomicAdd(&o.counters[3], npc);
     }
     // the two streams' sums: short range in box units (pg), then the mesh
     // force, each sink's six x planes split between the streams (cg)
@@ -1042,7 +1045,7 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
         rmax = max(rmax, kick_epilogue(B, a.flags, a.o, w[k], a.o.rung[w[k]], pgx, pgy, pgz, pph, pg[k][0],
                                        pg[k][1], pg[k][2], pg[k][3]));
     }
-    max_rung_update(a.o, rmax);
+    max_rung_update(o, rmax);
 }
 
 __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
@@ -1165,8 +1168,8 @@ tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, in
     if (t0 <= 0.0) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: t0 must be positive");
     TPM_TRY(tpm_tree_ensure(ctx));
     hipStream_t s = ctx->stream;
-    if (!ctx->counters) TPM_HIP(ctx, hipMalloc(&ctx->counters, sizeof(unsigned long long) * 4));
-    TPM_TRY(tpm_fill_u64(ctx, ctx->counters, 4));
+    if (!ctx->counters) TPM_HIP(ctx, hipMalloc(&ctx->counters, sizeof(unsigned long long) * TPM_COUNTER_WORDS));
+    TPM_TRY(tpm_fill_u64(ctx, ctx->counters, TPM_COUNTER_WORDS));
     tree_fill_i32_kernel<<<1, 64, 0, s>>>(ctx->tree_flags, 4, 0);
     WalkArgs a{};
     a.pos = ctx->tree_pos;
@@ -1209,12 +1212,15 @@ tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, in
     TPM_HIP(ctx, hipEventRecord(e1, s));
     int32_t fl[2] = {0, 0};
     TPM_HIP(ctx, hipMemcpyAsync(fl, ctx->tree_flags, sizeof(fl), hipMemcpyDeviceToHost, s));
-    unsigned long long hc[4] = {0, 0, 0, 0};
+    std::vector<unsigned long long> hw(TPM_COUNTER_WORDS, 0ull);
     if (stats) {
         TPM_TRY(tpm_candidate_pairs(ctx, ctx->counters + 2));
-        TPM_HIP(ctx, hipMemcpyAsync(hc, ctx->counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+        TPM_HIP(ctx, hipMemcpyAsync(hw.data(), ctx->counters, sizeof(unsigned long long) * TPM_COUNTER_WORDS,
+                                    hipMemcpyDeviceToHost, s));
     }
     TPM_HIP(ctx, hipStreamSynchronize(s));
+    unsigned long long hc[4];
+    tpm_reduce_counters(hw.data(), hc);
     if (fl[1])
         return tpm_fail(ctx, TPM_ERR_HIP, "tpm_kick: tree walk stopped (stack exhausted or a node met twice)");
     if (stats) {
