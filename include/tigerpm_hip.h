@@ -85,6 +85,18 @@ typedef struct {
     double n_total;/* unit masses in the box (reference parts_dim^3) */
 } tpm_params;
 
+/* FP64 global sums over every resident particle of a box, with masses
+ * m = GM (no reference counterpart; the reference prints no conserved
+ * quantities): momentum m sum v, force m sum g, kinetic energy
+ * m/2 sum |v|^2, potential energy m/2 sum phi.  Fixed reduction order: the
+ * same particles in the same order give the same bits. */
+typedef struct {
+    double momentum[3];
+    double force[3];
+    double kinetic;
+    double potential;
+} tpm_moments;
+
 /* Per-call statistics (the reference prints similar diagnostics,
  * kick_treepm.cu:1084-1087).  Times are HIP-event durations on the context
  * stream, milliseconds. */
@@ -100,6 +112,7 @@ typedef struct {
     int max_rung;
     int pad_;
     double pc_interactions;  /* TPM_KICK_TREE: sink x multipole interactions */
+    tpm_moments moments;     /* after the kick, over every resident particle */
 } tpm_stats;
 
 /* Launch-shape tuning (no reference counterpart: the reference fixes 32-thread
@@ -138,16 +151,39 @@ TPM_API tpm_status tpm_particles_download(tpm_ctx* ctx, uint32_t* x, uint32_t* y
                                   float* vz, int8_t* rung, float* gx, float* gy, float* gz, float* pot);
 TPM_API int64_t tpm_particles_size(const tpm_ctx* ctx);
 
+/* Particle order the caller sees.  The device always works in its own
+ * order: after a chain mesh, cells z-major and inside a cell by 4x4x4
+ * sub-cell Morton code (the PP kernel's tiles need compact groups of sinks).
+ *   TPM_ORDER_DEVICE (default): the caller sees that order -- downloads,
+ *     tpm_chainmesh_perm and the sample indices of tpm_ewald use it.
+ *   TPM_ORDER_REFERENCE: every tpm_chainmesh_create also computes, on the
+ *     device, the order in which chainmesh_create's recursive partition
+ *     (chainmesh.cpp:168-213 with the two-pointer particles_sort,
+ *     particles.cpp:414-445) leaves the caller's previous order, bit for
+ *     bit; the caller sees that order from then on: downloads are gathered
+ *     into it, tpm_ewald's sample indices refer to it and
+ *     tpm_chainmesh_perm returns the reference's permutation.  Uploads and
+ *     initial conditions define a new caller order (identical to the
+ *     device's).  Costs 3 ceil(log2 C) scan passes over the particles per
+ *     chain-mesh build.
+ * The caller's order replaces particles.hpp's global arrays' order, which
+ * chainmesh_create sorts in place in the reference. */
+#define TPM_ORDER_DEVICE 0
+#define TPM_ORDER_REFERENCE 1
+TPM_API tpm_status tpm_set_host_order(tpm_ctx* ctx, int order);
+
 /* chainmesh_create (chainmesh.cpp:139-166, sort :168-213): reorders the
  * resident particles into chain-cell order (cells z-major, the reference's
  * partition order; in-cell order by 4x4x4 sub-cell Morton code then previous
- * order) and builds the cell table. */
+ * order) and builds the cell table.  Cell ranges are the reference's. */
 TPM_API tpm_status tpm_chainmesh_create(tpm_ctx* ctx);
 /* chainmesh_get (chainmesh.cpp:48-51): cell (ix, iy, iz), periodic. */
 TPM_API tpm_status tpm_chainmesh_get(tpm_ctx* ctx, int ix, int iy, int iz, int64_t* pbegin, int64_t* pend);
 /* All C^3 + 1 cell offsets, cell id = (iz*C + iy)*C + ix. */
 TPM_API tpm_status tpm_chainmesh_cells(tpm_ctx* ctx, int64_t* cell_begin);
-/* perm[new] = index before the last tpm_chainmesh_create. */
+/* perm[new] = index before the last tpm_chainmesh_create, both in the
+ * caller's order (TPM_ORDER_REFERENCE: the reference partition's
+ * permutation). */
 TPM_API tpm_status tpm_chainmesh_perm(tpm_ctx* ctx, int64_t* perm);
 
 /* gravity_long_compute (gravity_long.cpp:36-48) with the PME filter
@@ -169,6 +205,33 @@ TPM_API tpm_status tpm_gravity_long_get_green(tpm_ctx* ctx, float* table);
  * stats may be NULL. */
 TPM_API tpm_status tpm_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                     tpm_stats* stats);
+
+/* The FP64 momentum and energy sums of the resident particles' current v,
+ * g and phi (the same values tpm_stats.moments holds after a kick). */
+TPM_API tpm_status tpm_moments_get(tpm_ctx* ctx, tpm_moments* out);
+
+/* kick_pme(range<int> box, ...) (kick_pme.hpp:15, kick_pme.cu:390-621) and
+ * the box argument of kick_treepm (kick_treepm.hpp:14,
+ * kick_treepm.cu:761-1089): tpm_kick restricted to the sinks of the chain
+ * cells in [lo, hi) (cell coordinates x, y, z; 0 <= lo < hi <= C).  Sources
+ * are every resident particle, as for the whole kick, so each sink gets the
+ * same bits as in tpm_kick.  stats (may be NULL) count the box's sinks only;
+ * stats->moments are over every resident particle. */
+TPM_API tpm_status tpm_kick_box(tpm_ctx* ctx, const int32_t lo[3], const int32_t hi[3], int min_rung, double scale,
+                                double t0, int first_call, unsigned flags, tpm_stats* stats);
+/* Device budget of one kick (the reference's `mem_required > free_mem`
+ * test, kick_treepm.cu:815-823, kick_pme.cu:422-429), in bytes; 0 (the
+ * default) = no limit.  A kick's working set is what its launch touches:
+ * 12 B per source particle of the box padded by one cell, 41 B per sink
+ * (position, velocity, rung, force and potential), 16 B per sink tile,
+ * 4 B per node of the potential mesh under the padded box, 8 B per cell of
+ * the padded box, and on the tree path 48 B per tree node of the padded box
+ * and 32 B per sink bucket.  While a box's working set exceeds the budget,
+ * tpm_kick, tpm_kick_box and tpm_kick_treepm split it in halves on its
+ * longest axis (range::split, range.hpp:118-133) and kick the halves one
+ * after the other; every sink gets the same bits as unsplit.  A single cell
+ * over the budget is TPM_ERR_BUDGET. */
+TPM_API tpm_status tpm_set_budget(tpm_ctx* ctx, int64_t bytes);
 
 /* Per-cell trees (tree_node / sink_bucket, tree.hpp:17-36).  Nodes of one
  * cell are stored in preorder (a node before its subtrees, child 0 first);
@@ -193,18 +256,27 @@ typedef struct {
 /* tree_create (tree.cpp:216-245) for every chain cell, as the loop of
  * kick_treepm_begin (kick_treepm.cpp:26-44) does, on the device: each cell's
  * particles are split at the midpoint of the longest side of its box until
- * <= 32 remain (x < the fixed32 midpoint goes left, each side keeps its
- * order); sink buckets are the first nodes of <= 128 particles.  Needs
- * tpm_chainmesh_create.  A tree deeper than 50 levels is TPM_ERR_INVALID
- * (the reference aborts, tree.cpp:16-19). */
+ * <= 32 remain, by the reference's two-pointer particles_sort
+ * (particles.cpp:414-445: x < the fixed32 midpoint goes left) starting from
+ * the caller's order; sink buckets are the first nodes of <= 128
+ * particles.  Needs tpm_chainmesh_create.  A tree deeper than 50 levels is
+ * TPM_ERR_INVALID (the reference aborts, tree.cpp:16-19). */
 TPM_API tpm_status tpm_tree_build(tpm_ctx* ctx);
 /* Sizes of the built trees (any pointer may be NULL). */
 TPM_API tpm_status tpm_tree_info(tpm_ctx* ctx, int64_t* n_nodes, int64_t* n_buckets);
 /* Copies the trees to the host: node_begin and bucket_begin get C^3 + 1
- * offsets, order[i] = chain-mesh index of tree position i.  Any pointer may
- * be NULL. */
+ * offsets, order[i] = index (in the caller's order) of the particle at tree
+ * position i.  Any pointer may be NULL. */
 TPM_API tpm_status tpm_tree_download(tpm_ctx* ctx, int64_t* node_begin, int64_t* bucket_begin, tpm_tree_node* nodes,
                                      tpm_tree_bucket* buckets, int32_t* order);
+
+/* tree_create's in-place reordering (tree.cpp:101-107 calls particles_sort
+ * on the caller's arrays): with TPM_ORDER_REFERENCE the caller's order
+ * becomes the trees' order, cell by cell, as kick_treepm_begin leaves the
+ * reference's global arrays (kick_treepm.cpp:26-44).  The trees are built
+ * from the caller's order (tpm_tree_build), so their order is the
+ * reference's two-pointer partitions of it.  Later downloads use it. */
+TPM_API tpm_status tpm_tree_sort_particles(tpm_ctx* ctx);
 
 /* One whole step: chain mesh + PM + kick (kick_treepm_test without IC and
  * comparison, test.cpp:237-290). */
@@ -236,6 +308,14 @@ TPM_API tpm_status tpm_probe_pair_rate(tpm_ctx* ctx, int waves_per_simd, double*
  * background term.  alpha <= 0 picks the split automatically. */
 TPM_API tpm_status tpm_ewald(tpm_ctx* ctx, int64_t nsamples, const int64_t* samples, double alpha, double* phi,
                      double* gx, double* gy, double* gz);
+/* gravity_short_ewald_call_kernel(sinkx, sinky, sinkz) (gravity_short.hpp:
+ * 25-26, gravity_short.cu:10-91) at arbitrary fixed32 positions: the FP64
+ * periodic potential and force of all resident unit masses at each position,
+ * every particle softened inside hsoft (a particle sitting exactly at the
+ * position adds -35/16 h^-1 to phi and nothing to g), no particle left out.
+ * Host arrays of nsamples raw positions. */
+TPM_API tpm_status tpm_ewald_at(tpm_ctx* ctx, int64_t nsamples, const uint32_t* sx, const uint32_t* sy,
+                                const uint32_t* sz, double alpha, double* phi, double* gx, double* gy, double* gz);
 
 /* Zel'dovich initial conditions on the device: n_side^3 particles on the
  * lattice (i+1/2)/n_side displaced by a Gaussian field with power
@@ -346,6 +426,9 @@ TPM_API tpm_status tpm_batch_download(tpm_batch* batch, int id, uint32_t* x, uin
 /* Copies a box's state into a context on the same device (parameters
  * included), e.g. for tpm_ewald or to kick it alone. */
 TPM_API tpm_status tpm_batch_to_ctx(tpm_batch* batch, int id, tpm_ctx* ctx);
+/* The FP64 momentum and energy sums of one box's particles after its last
+ * step (tpm_moments, masses = the box's GM). */
+TPM_API tpm_status tpm_batch_moments(tpm_batch* batch, int id, tpm_moments* out);
 TPM_API tpm_status tpm_batch_set_tuning(tpm_batch* batch, const tpm_batch_tuning* tuning);
 TPM_API tpm_status tpm_batch_get_tuning(const tpm_batch* batch, tpm_batch_tuning* tuning);
 /* Autotuner: times the step over the resident boxes for every tile size and

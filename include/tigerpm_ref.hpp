@@ -151,6 +151,68 @@ void chainmesh_exchange_end();
 range<int> chainmesh_interior_box();
 chaincell chainmesh_get(const std::array<int, NDIM>& cell);
 
+// ---- kernels.hpp:6-10, tree.hpp:17-110 ------------------------------------------
+#define NCHILD 2
+#define MULTIPOLE_SIZE 4
+using multipole = std::array<float, MULTIPOLE_SIZE>;  // {N, -sum dx, -sum dy, -sum dz}
+
+struct tree_node {
+    std::array<fixed32, NDIM> x;  // expansion centre
+    float radius;
+    std::array<int, NCHILD> children;  // within the cell's tree; -1 for a leaf
+    int src_begin;
+    int src_end;
+    int snk_begin;
+    int snk_end;
+    multipole multi;
+};
+
+struct sink_bucket {
+    int src_begin;
+    int src_end;
+    int snk_begin;
+    int snk_end;
+    float radius;
+    std::array<fixed32, NDIM> x;
+};
+
+// One chain cell's tree, nodes in preorder (the reference's tree class
+// holds the same nodes behind a raw pointer with a device copy; here the
+// device keeps its own copy).
+class tree {
+public:
+    std::vector<tree_node> nodes;
+    size_t size() const { return nodes.size(); }
+    void resize(int new_size) { nodes.resize(new_size); }
+    int allocate() {
+        nodes.emplace_back();
+        return (int)nodes.size() - 1;
+    }
+    fixed32 get_x(int dim, int i) const { return nodes[i].x[dim]; }
+    int get_pbegin(int i) const { return nodes[i].src_begin; }
+    int get_pend(int i) const { return nodes[i].src_end; }
+    int get_snk_begin(int i) const { return nodes[i].snk_begin; }
+    int get_snk_end(int i) const { return nodes[i].snk_end; }
+    multipole get_multipole(int i) const { return nodes[i].multi; }
+    float get_mass(int i) const { return nodes[i].multi[0]; }
+    bool is_leaf(int i) const { return nodes[i].children[0] == -1; }
+    float get_radius(int i) const { return nodes[i].radius; }
+    std::array<int, NCHILD> get_children(int i) const { return nodes[i].children; }
+    void set(tree_node node, int i) { nodes[i] = node; }
+    void adjust_src_indexes(int dif) {
+        for (auto& n : nodes) {
+            n.src_begin += dif;
+            n.src_end += dif;
+        }
+    }
+    void adjust_snk_indexes(int dif) {
+        for (auto& n : nodes) {
+            n.snk_begin += dif;
+            n.snk_end += dif;
+        }
+    }
+};
+
 // ---- gravity_long.hpp:17-25 --------------------------------------------------
 using gravity_long_type = int;
 #define GRAVITY_LONG_PM 0
@@ -160,12 +222,39 @@ std::vector<float> gravity_long_get_phi(const range<int>& box);  // periodic
 range<int> gravity_long_get_phi_box();
 std::pair<float, std::array<float, NDIM>> gravity_long_force_at(const std::array<double, NDIM>& pos);
 
+// tree.hpp:108: the tree and sink buckets of one chain cell, its particles
+// reordered in place (particles_X/U/R, [cell.pbegin, cell.pend)) into tree
+// order, as the reference's two-pointer partitions leave them.  The trees
+// are built on the device from the current chain mesh (tpm_tree_build);
+// node and bucket ranges index the global particle arrays.
+std::pair<tree, std::vector<sink_bucket>> tree_create(const std::array<int, NDIM>& cell_index, chaincell cell);
+
 // ---- kick_treepm.hpp / kick_pme.hpp / kick_pm.hpp ----------------------------
+// kick_treepm_begin: the trees of every cell (particles into tree order, as
+// tree_create for each cell of the interior box) and the kick of the
+// interior box.  The short-range sum is options::short_range: "direct" (the
+// erfc-split pair force over the 27 neighbour cells, the default) or "tree"
+// (the theta = 0.4 walk of the trees with monopole + dipole multipoles, the
+// reference's kick_treepm kernel).
 void kick_treepm_begin(int min_rung, double scale, double t0, bool first_call);
+// kick_treepm.hpp:14: the kick of the sinks in box (chain cells), with the
+// trees of box.pad(1) and the sink buckets of box as tree_create made them
+// (their sizes are checked against the device's copies, which the walk
+// uses).  Over the device budget (set_device_budget) the box is split in
+// halves and kicked one half after the other (kick_treepm.cu:815-823).
+void kick_treepm(std::vector<tree> trees, std::vector<std::vector<sink_bucket>> buckets, range<int> box, int min_rung,
+                 double scale, double t0, bool first_call);
 void kick_treepm_end();
+// kick_pme.hpp:15: kick_pme's pair law over the sinks of box (split over
+// the device budget as kick_pme.cu:422-429 does).
+void kick_pme(range<int> box, int min_rung, double scale, double t0, bool first_call);
 void kick_pme_begin(int min_rung, double scale, double t0, bool first_call);
 void kick_pme_end();
 void kick_pm();
+// Device memory a kick's working set may use (bytes, 0 = no limit): the
+// reference compares with 85 % of the free device memory
+// (kick_treepm.cu:817); here the budget is explicit (tpm_set_budget).
+void set_device_budget(int64_t bytes);
 
 // ---- gravity_short.hpp:25-28 -------------------------------------------------
 std::pair<std::vector<double>, std::array<std::vector<double>, NDIM>> gravity_short_ewald_call_kernel(

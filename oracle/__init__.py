@@ -93,6 +93,10 @@ def lib():
             "orc_ewald_direct": (None, [c_i64, u32p, u32p, u32p, c_i64, i64p, c_dbl, f64p, f64p, f64p, f64p]),
             "orc_ewald_sf": (None, [c_i64, u32p, u32p, u32p, c_i64, i64p, c_dbl, c_dbl, f64p, f64p, f64p,
                                     f64p]),
+            "orc_ewald_sf_at": (None, [c_i64, u32p, u32p, u32p, c_i64, u32p, u32p, u32p, c_dbl, c_dbl, f64p, f64p,
+                                       f64p, f64p]),
+            "orc_ewald_direct_at": (None, [c_i64, u32p, u32p, u32p, c_i64, u32p, u32p, u32p, c_dbl, f64p, f64p,
+                                           f64p, f64p]),
             "orc_tree_build": (c_int, [ctypes.POINTER(OrcParams), c_i64, u32p, u32p, u32p, i64p, i32p, u32p, u32p,
                                        u32p, i64p, i64p, ctypes.c_void_p, c_i64, ctypes.c_void_p, c_i64]),
             "orc_tree_sum": (c_int, [ctypes.POINTER(OrcParams), c_i64, u32p, u32p, u32p, i64p, i8p, c_int, c_int,
@@ -300,6 +304,21 @@ def ewald_sf(x, y, z, samples, hsoft, alpha=10.0):
     samples = np.ascontiguousarray(samples, np.int64)
     out = [np.empty(len(samples), np.float64) for _ in range(4)]
     lib().orc_ewald_sf(len(x), x, y, z, len(samples), samples, hsoft, alpha, *out)
+    return out
+
+
+def ewald_sf_at(x, y, z, sx, sy, sz, hsoft, alpha=10.0):
+    """Ewald potential and force of all particles at fixed32 positions."""
+    x, y, z, sx, sy, sz = _u32(x), _u32(y), _u32(z), _u32(sx), _u32(sy), _u32(sz)
+    out = [np.empty(len(sx), np.float64) for _ in range(4)]
+    lib().orc_ewald_sf_at(len(x), x, y, z, len(sx), sx, sy, sz, hsoft, alpha, *out)
+    return out
+
+
+def ewald_direct_at(x, y, z, sx, sy, sz, hsoft):
+    x, y, z, sx, sy, sz = _u32(x), _u32(y), _u32(z), _u32(sx), _u32(sy), _u32(sz)
+    out = [np.empty(len(sx), np.float64) for _ in range(4)]
+    lib().orc_ewald_direct_at(len(x), x, y, z, len(sx), sx, sy, sz, hsoft, *out)
     return out
 
 

@@ -956,24 +956,34 @@ typedef struct {
     int err;
 } tree_cell;
 
-/* particles_sort (particles.cpp:414-445) puts x < xmid on the left.  Here
- * each side keeps its previous relative order (a stable partition); that
- * fixes the order of the particles inside a leaf, which the reference leaves
- * to its swap sequence. */
+/* particles_sort (particles.cpp:414-445), called by tree.cpp's sort()
+ * (tree.cpp:101-107) on the longest side's midpoint: lo walks up, and a
+ * particle with x >= xmid that lo meets is swapped with the next particle
+ * below the top with x < xmid.  Returns the first position of the right
+ * side.  The swap sequence fixes the order of the particles in a leaf. */
 static int tree_partition(tree_cell* t, int begin, int end, uint32_t thr, int dim) {
-    int nl = 0;
-    for (int i = begin; i < end; i++) nl += t->pos[dim][i] < thr;
-    int l = 0, r = nl;
-    for (int i = begin; i < end; i++) {
-        const int k = (t->pos[dim][i] < thr) ? l++ : r++;
-        for (int d = 0; d < 3; d++) t->tpos[d][k] = t->pos[d][i];
-        t->tidx[k] = t->idx[i];
+    int lo = begin, hi = end;
+    const uint32_t* xd = t->pos[dim];
+    while (lo < hi) {
+        if (xd[lo] >= thr) {
+            while (lo != hi) {
+                hi--;
+                if (xd[hi] < thr) {
+                    for (int d = 0; d < 3; d++) {
+                        const uint32_t v = t->pos[d][hi];
+                        t->pos[d][hi] = t->pos[d][lo];
+                        t->pos[d][lo] = v;
+                    }
+                    const int32_t k = t->idx[hi];
+                    t->idx[hi] = t->idx[lo];
+                    t->idx[lo] = k;
+                    break;
+                }
+            }
+        }
+        lo++;
     }
-    for (int k = 0; k < end - begin; k++) {
-        for (int d = 0; d < 3; d++) t->pos[d][begin + k] = t->tpos[d][k];
-        t->idx[begin + k] = t->tidx[k];
-    }
-    return begin + nl;
+    return hi;
 }
 
 /* Centre of the bounding box of [begin, end) (the box centre when empty)
@@ -1773,9 +1783,9 @@ ORC_API void orc_ewald_direct(int64_t n, const uint32_t* x, const uint32_t* y, c
  * reciprocal space through S(h) = sum_j exp(2 pi i h.x_j) over the half
  * space 0 < |h| <= 5.2 alpha/pi.  Same outputs and conventions as
  * orc_ewald_direct. */
-ORC_API void orc_ewald_sf(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int64_t ns,
-                          const int64_t* samples, double hsoft, double alpha, double* phi, double* gx, double* gy,
-                          double* gz) {
+static void ewald_sf_core(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int64_t ns,
+                          const int64_t* samples, const uint32_t* sx, const uint32_t* sy, const uint32_t* sz,
+                          double hsoft, double alpha, double* phi, double* gx, double* gy, double* gz) {
     const double a = alpha;
     const double rc = 4.8 / a;
     const double rc2 = rc * rc;
@@ -1820,14 +1830,17 @@ ORC_API void orc_ewald_sf(int64_t n, const uint32_t* x, const uint32_t* y, const
     }
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t s = 0; s < ns; s++) {
-        const int64_t i = samples[s];
-        double P = (double)n * M_PI / (a * a) + tsp * a;
+        /* a sample particle: its own pair is left out and tsp a restores the
+         * image + background term; a position: every particle counts */
+        const int64_t i = samples ? samples[s] : -1;
+        const uint32_t qx = samples ? x[i] : sx[s], qy = samples ? y[i] : sy[s], qz = samples ? z[i] : sz[s];
+        double P = (double)n * M_PI / (a * a) + (samples ? tsp * a : 0.0);
         double G[3] = {0, 0, 0};
         for (int64_t j = 0; j < n; j++) {
             if (j == i) continue;
-            const double dx = (double)(int32_t)(x[i] - x[j]) * 0x1p-32;
-            const double dy = (double)(int32_t)(y[i] - y[j]) * 0x1p-32;
-            const double dz = (double)(int32_t)(z[i] - z[j]) * 0x1p-32;
+            const double dx = (double)(int32_t)(qx - x[j]) * 0x1p-32;
+            const double dy = (double)(int32_t)(qy - y[j]) * 0x1p-32;
+            const double dz = (double)(int32_t)(qz - z[j]) * 0x1p-32;
             const double r2 = dx * dx + dy * dy + dz * dz;
             if (r2 >= rc2) continue;
             const double r = sqrt(r2);
@@ -1857,7 +1870,7 @@ ORC_API void orc_ewald_sf(int64_t n, const uint32_t* x, const uint32_t* y, const
                 G[2] -= dr * dz / r;
             }
         }
-        const double px = (double)x[i] * 0x1p-32, py = (double)y[i] * 0x1p-32, pz = (double)z[i] * 0x1p-32;
+        const double px = (double)qx * 0x1p-32, py = (double)qy * 0x1p-32, pz = (double)qz * 0x1p-32;
         for (int k = 0; k < nh; k++) {
             const int hx = hl[3 * k], hy = hl[3 * k + 1], hz = hl[3 * k + 2];
             const int h2 = hx * hx + hy * hy + hz * hz;
@@ -1879,6 +1892,63 @@ ORC_API void orc_ewald_sf(int64_t n, const uint32_t* x, const uint32_t* y, const
     free(hl);
     free(Sre);
     free(Sim);
+}
+
+ORC_API void orc_ewald_sf(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int64_t ns,
+                          const int64_t* samples, double hsoft, double alpha, double* phi, double* gx, double* gy,
+                          double* gz) {
+    ewald_sf_core(n, x, y, z, ns, samples, NULL, NULL, NULL, hsoft, alpha, phi, gx, gy, gz);
+}
+
+/* The same sum at arbitrary positions (gravity_short_ewald_call_kernel's
+ * sinks, gravity_short.cu:10-91): the potential and force of all n unit
+ * masses at each position, each softened inside hsoft; a particle sitting
+ * exactly at the position adds -35/16 h^-1 to phi and nothing to g. */
+ORC_API void orc_ewald_sf_at(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int64_t ns,
+                             const uint32_t* sx, const uint32_t* sy, const uint32_t* sz, double hsoft, double alpha,
+                             double* phi, double* gx, double* gy, double* gz) {
+    ewald_sf_core(n, x, y, z, ns, NULL, sx, sy, sz, hsoft, alpha, phi, gx, gy, gz);
+}
+
+/* Pair-by-pair Ewald sum at arbitrary positions (alpha 2, images |n| <= 4,
+ * |h| <= 4), the independent check of orc_ewald_sf_at: sum over every
+ * particle of psi_soft(q - x_j), where inside hsoft the Newtonian -1/r of
+ * psi is replaced by the softened polynomial (at r = 0: psi's regular
+ * part minus 35/16 h^-1). */
+ORC_API void orc_ewald_direct_at(int64_t n, const uint32_t* x, const uint32_t* y, const uint32_t* z, int64_t ns,
+                                 const uint32_t* sx, const uint32_t* sy, const uint32_t* sz, double hsoft,
+                                 double* phi, double* gx, double* gy, double* gz) {
+    const double a = 2.0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t s = 0; s < ns; s++) {
+        double P = 0.0, G[3] = {0, 0, 0};
+        for (int64_t j = 0; j < n; j++) {
+            const double dx = (double)(int32_t)(sx[s] - x[j]) * 0x1p-32;
+            const double dy = (double)(int32_t)(sy[s] - y[j]) * 0x1p-32;
+            const double dz = (double)(int32_t)(sz[s] - z[j]) * 0x1p-32;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            double ps, gr[3];
+            if (r2 < hsoft * hsoft) {
+                ewald_psi(dx, dy, dz, a, 4, 4, 1, &ps, gr);
+                double d0, d1;
+                soft_poly(r2, hsoft, &d0, &d1);
+                P += ps - d0;
+                G[0] -= gr[0] + d1 * dx;
+                G[1] -= gr[1] + d1 * dy;
+                G[2] -= gr[2] + d1 * dz;
+            } else {
+                ewald_psi(dx, dy, dz, a, 4, 4, 0, &ps, gr);
+                P += ps;
+                G[0] -= gr[0];
+                G[1] -= gr[1];
+                G[2] -= gr[2];
+            }
+        }
+        phi[s] = P;
+        gx[s] = G[0];
+        gy[s] = G[1];
+        gz[s] = G[2];
+    }
 }
 
 ORC_API int orc_num_threads(void) { return omp_get_max_threads(); }

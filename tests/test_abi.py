@@ -77,6 +77,7 @@ def test_struct_layouts_match_header(tmp_path):
     """sizeof / offsetof of every ABI struct, from the C header compiled by
     gcc, equal the ctypes mirrors."""
     structs = {"tpm_params": T.TpmParams, "tpm_stats": T.TpmStats, "tpm_tuning": T.TpmTuning,
+               "tpm_moments": T.TpmMoments,
                "tpm_batch_limits": T.TpmBatchLimits, "tpm_box_spec": T.TpmBoxSpec,
                "tpm_batch_tuning": T.TpmBatchTuning, "tpm_batch_stats": T.TpmBatchStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tigerpm_hip.h"', "int main(void) {"]

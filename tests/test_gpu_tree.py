@@ -268,3 +268,42 @@ def test_depth_limit_is_an_error(gpu_ctx):
     gpu_ctx.chainmesh_create()
     with pytest.raises(T.TpmError, match="depth"):
         gpu_ctx.tree_build()
+
+
+@pytest.mark.parametrize("kind,n_side", [("clustered", 24), ("uniform", 20)])
+def test_trees_from_the_reference_order(kind, n_side):
+    """With TPM_ORDER_REFERENCE the trees start from the order the reference
+    partition leaves the caller's arrays in, as tree_create does
+    (kick_treepm.cpp:26-44 after chainmesh_create): node for node and in
+    particle order bit-identical to the oracle's two-pointer trees of that
+    order; tpm_tree_sort_particles then leaves the caller's arrays in tree
+    order, and a kick's outputs follow it."""
+    n = n_side ** 3
+    x, y, z = positions(kind, n, 11)
+    with T.Context(0) as ctx:
+        p = ctx.set_params(T.baseline_params(n_side, n_side, 4))
+        ctx.set_host_order(T.TPM_ORDER_REFERENCE)
+        ctx.upload(x, y, z)
+        ctx.chainmesh_create()
+        idx, rb, _ = O.chainmesh_partition_ref(x, y, z, p.chain_dim)
+        cb = np.concatenate([rb, [n]]).astype(np.int64)
+        xr, yr, zr = x[idx], y[idx], z[idx]
+        ctx.tree_build()
+        g = ctx.tree_download()
+        o = O.tree_build(oparams(p), xr, yr, zr, cb)
+        assert np.array_equal(g["order"], o["order"])
+        assert np.array_equal(g["node_begin"], o["node_begin"])
+        assert g["nodes"].tobytes() == o["nodes"].tobytes()
+        ctx.tree_sort_particles()
+        d = ctx.download()
+        for k, t in (("x", "tx"), ("y", "ty"), ("z", "tz")):
+            assert np.array_equal(d[k], o[t]), k
+        # the tree order again is its own partition: rebuilt trees keep it
+        ctx.chainmesh_create()
+        assert np.array_equal(ctx.chainmesh_perm(), np.arange(n))
+        ctx.tree_build()
+        assert np.array_equal(ctx.tree_download()["order"], np.arange(n))
+        ctx.gravity_long_compute()
+        ctx.kick(0, 1.0, 1.0, True, flags=T.TPM_KICK_TREE)
+        k1 = ctx.download()
+    assert np.array_equal(k1["x"], o["tx"])

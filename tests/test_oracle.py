@@ -92,6 +92,35 @@ def test_chainmesh_sort_equals_reference_partition(C, n):
     assert np.all(np.diff(perm)[tie] > 0)
 
 
+def test_reference_partition_hand_worked():
+    """The reference partition's order, worked out by hand from the
+    two-pointer particles_sort (particles.cpp:414-445) and the recursion of
+    chainmesh.cpp:168-213 (z split first, then y, then x; C = 2, split plane
+    fixed32(0.5) = 2^31; a coordinate >= the plane goes right)."""
+    L, H = 0x10000000, 0x90000000
+    # (1) one particle per cell.  z split of [0,6): right = {0, 2}; left
+    # count 4; misplaced right at 0, 2 trade with left at 5, 4 ->
+    # [5, 1, 4, 3, 2, 0].  y split of [5, 1, 4, 3]: lo meets 1 at position 1,
+    # hi finds 4 at position 2 -> [5, 4, 1, 3]; y split of [2, 0]: no move.
+    # x splits: [5, 4] -> [4, 5]; [1, 3] and [2, 0] stay.
+    x = np.array([H, L, L, H, L, H], np.uint32)
+    y = np.array([L, H, L, H, L, L], np.uint32)
+    z = np.array([H, L, H, L, L, L], np.uint32)
+    idx, cb, ce = O.chainmesh_partition_ref(x, y, z, 2)
+    assert list(idx) == [4, 5, 1, 3, 2, 0]
+    assert list(cb[:6]) == [0, 1, 2, 3, 4, 5] and list(ce[:6]) == [1, 2, 3, 4, 5, 6]
+    # (2) several particles per cell, all at x, y < 1/2: z classes
+    # [R, R, L, L, R, L] -> positions 0, 1 trade with 5, 3 -> [5, 3, 2, 1, 4, 0]
+    # (not the stable order [2, 3, 5 | 0, 1, 4]); the y and x splits keep it.
+    # A particle exactly on the plane (raw 2^31) goes right.
+    x = np.full(6, L, np.uint32)
+    y = np.full(6, L, np.uint32)
+    z = np.array([H, 2 ** 31, L, L, H, L], np.uint32)
+    idx, cb, ce = O.chainmesh_partition_ref(x, y, z, 2)
+    assert list(idx) == [5, 3, 2, 1, 4, 0]
+    assert (cb[0], ce[0]) == (0, 3) and (cb[4], ce[4]) == (3, 6)
+
+
 def test_chainmesh_edge_cases():
     e = np.zeros(0, np.uint32)
     perm, cb = O.chainmesh_sort(e, e, e, 5)
@@ -396,6 +425,39 @@ def test_ewald_structure_factor_equals_direct():
     b = O.ewald_sf(x, y, z, s, 1e-4, 9.6)
     assert rel_rms(np.stack(b[1:]), np.stack(a[1:])) < 1e-9
     assert rel_rms(b[0], a[0]) < 1e-9
+
+
+def test_ewald_at_positions():
+    """The field of all particles at arbitrary positions: structure-factor
+    form equals the pair-by-pair sum; at a particle's own position it is the
+    per-particle sum plus that particle's softened self potential
+    (-35/16 h^-1, no force); and g = -grad phi (central differences)."""
+    n, h = 200, 2e-4
+    x, y, z = rand_u32(n, 61), rand_u32(n, 62), rand_u32(n, 63)
+    x[1], y[1], z[1] = x[0] + 20000, y[0], z[0]  # a pair inside the softening
+    sx, sy, sz = rand_u32(40, 64), rand_u32(40, 65), rand_u32(40, 66)
+    sx[:3], sy[:3], sz[:3] = x[:3], y[:3], z[:3]  # on particles
+    sx[3], sy[3], sz[3] = x[0] + 3000, y[0] - 7000, z[0]  # inside the softening of two
+    a = O.ewald_direct_at(x, y, z, sx, sy, sz, h)
+    b = O.ewald_sf_at(x, y, z, sx, sy, sz, h, 9.6)
+    assert rel_rms(np.stack(b[1:]), np.stack(a[1:])) < 1e-9
+    assert rel_rms(b[0], a[0]) < 1e-9
+    own = O.ewald_sf(x, y, z, np.arange(3), h, 9.6)
+    assert np.allclose(b[0][:3], own[0] - 35.0 / 16.0 / h, rtol=0, atol=1e-8 * abs(own[0]).max())
+    for d in range(3):
+        assert np.allclose(b[1 + d][:3], own[1 + d], rtol=1e-10, atol=1e-9 * np.abs(own[1 + d]).max())
+    # g = -grad phi away from particles: step of 2^12 raw units
+    far = slice(4, 40)
+    step = 4096
+    for d, arr in enumerate((sx, sy, sz)):
+        up = [sx.copy(), sy.copy(), sz.copy()]
+        dn = [sx.copy(), sy.copy(), sz.copy()]
+        up[d] = (arr.astype(np.int64) + step).astype(np.uint32)
+        dn[d] = (arr.astype(np.int64) - step).astype(np.uint32)
+        pu = O.ewald_sf_at(x, y, z, *up, h, 9.6)[0]
+        pd = O.ewald_sf_at(x, y, z, *dn, h, 9.6)[0]
+        fd = -(pu - pd) / (2 * step * 2.0 ** -32)
+        assert rel_rms(fd[far], b[1 + d][far]) < 1e-4
 
 
 # ------------------------------------------------------------ whole kick

@@ -25,8 +25,19 @@ MUTANTS = {
     "cell_order_transposed": ("const uint32_t cid = ((uint32_t)cz * C + cy) * C + cx;",
                               "const uint32_t cid = ((uint32_t)cx * C + cy) * C + cz;"),
     "ewald_kspace_force_sign": ("G[0] -= 4.0 * c * hx * im;", "G[0] += 4.0 * c * hx * im;"),
-    "ewald_self_constant_dropped": ("double P = (double)n * M_PI / (a * a) + tsp * a;",
+    "ewald_self_constant_dropped": ("double P = (double)n * M_PI / (a * a) + (samples ? tsp * a : 0.0);",
                                     "double P = (double)n * M_PI / (a * a);"),
+    "ewald_at_self_constant_kept": ("double P = (double)n * M_PI / (a * a) + (samples ? tsp * a : 0.0);",
+                                    "double P = (double)n * M_PI / (a * a) + tsp * a;"),
+    "tree_partition_idx_not_swapped": ("                    t->idx[hi] = t->idx[lo];\n                    t->idx[lo] = k;",
+                                       "                    (void)k;"),
+    "partition_plane_goes_left": ("if (xd[pc->idx[lo]] >= xmid) {", "if (xd[pc->idx[lo]] > xmid) {"),
+    "partition_stable_instead_of_swaps": ("                    int64_t t = pc->idx[hi];\n"
+                                          "                    pc->idx[hi] = pc->idx[lo];\n"
+                                          "                    pc->idx[lo] = t;",
+                                          "                    int64_t t = pc->idx[hi];\n"
+                                          "                    for (int64_t q = hi; q > lo; q--) pc->idx[q] = pc->idx[q - 1];\n"
+                                          "                    pc->idx[lo] = t;"),
     "rung_rounding": ("float nr = fmaxf(ceilf(log2f(t0f)", "float nr = fmaxf(floorf(log2f(t0f)"),
     "pme_soft_variable": ("        const float q2 = q * q;\n        rinv3 = +15.0f / 8.0f;",
                           "        const float q2 = q;\n        rinv3 = +15.0f / 8.0f;"),

@@ -361,3 +361,34 @@ def test_depth_limit_is_an_error():
     perm, cb = O.chainmesh_sort(x, y, z, p.chain_dim)
     with pytest.raises(RuntimeError):
         O.tree_build(P, x[perm], y[perm], z[perm], cb)
+
+
+def test_tree_leaf_order_hand_worked():
+    """The particle order a split leaves, worked out by hand from the
+    two-pointer particles_sort (particles.cpp:414-445) that tree.cpp:101-107
+    calls.  One chain cell [0, 1/3)^3 (C = 3), 34 particles, so the root
+    splits once, on x (the first of three equal sides) at x = 1/6:
+    particles 0 and 33 lie right of the plane, 1..32 left.  lo meets 0 at
+    position 0 and hi, walking down from 34, finds 32 (33 is right): they
+    trade places; lo then walks to 32 = hi and stops.  Order:
+    [32, 1, 2, ..., 31 | 0, 33] (a stable partition would give
+    [1, ..., 32 | 0, 33])."""
+    C = 3
+    n = 34
+    left_x, right_x = fixed(0.05), fixed(0.25)
+    rng = np.random.default_rng(5)
+    x = np.full(n, left_x, np.uint32)
+    x[0] = x[33] = right_x
+    x[1:33] += np.arange(32, dtype=np.uint32) * 1000  # distinct positions
+    y = (rng.random(n) * 0.3 * 2 ** 32).astype(np.uint32)
+    z = (rng.random(n) * 0.3 * 2 ** 32).astype(np.uint32)
+    cb = np.zeros(C ** 3 + 1, np.int64)
+    cb[1:] = n
+    p = baseline_params(C * 8, C * 8, 4)
+    P = O.make_params(p.mesh_dim, C, p.order, p.rs, 0.3, p.hsoft, p.GM, p.eta, n)
+    t = O.tree_build(P, x, y, z, cb)
+    assert list(t["order"]) == [32] + list(range(1, 32)) + [0, 33]
+    root = t["nodes"][0]
+    assert (root["begin"], root["end"]) == (0, 34)
+    kids = [t["nodes"][c] for c in root["child"]]
+    assert [(k["begin"], k["end"]) for k in kids] == [(0, 32), (32, 34)]

@@ -11,6 +11,7 @@ _LIB = os.path.join(_HERE, "libtigerpm_hip.so")
 _HEADER = os.path.join(_ROOT, "include", "tigerpm_hip.h")
 
 TPM_CIC, TPM_TSC, TPM_CLOUD4 = 2, 3, 4
+TPM_ORDER_DEVICE, TPM_ORDER_REFERENCE = 0, 1
 TPM_SKIP_PP, TPM_SKIP_PM, TPM_SKIP_KICK = 1, 2, 4
 TPM_KICK_PME = 16  # kick_pme's pair law (no cutoff, plain softening, no phi0)
 TPM_KICK_TREE = 32  # kick_treepm's tree walk: per-cell trees, theta = 0.4 multipoles + PP
@@ -42,14 +43,26 @@ class TpmParams(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TpmMoments(ctypes.Structure):
+    _fields_ = [("momentum", ctypes.c_double * 3), ("force", ctypes.c_double * 3), ("kinetic", ctypes.c_double),
+                ("potential", ctypes.c_double)]
+
+    def as_dict(self):
+        return dict(momentum=list(self.momentum), force=list(self.force), kinetic=self.kinetic,
+                    potential=self.potential)
+
+
 class TpmStats(ctypes.Structure):
     _fields_ = [("candidate_pairs", ctypes.c_double), ("executed_pairs", ctypes.c_double),
                 ("ms_chainmesh", ctypes.c_double), ("ms_pm", ctypes.c_double), ("ms_pp", ctypes.c_double),
                 ("ms_total", ctypes.c_double), ("n_active", ctypes.c_int64), ("n_tiles", ctypes.c_int64),
-                ("max_rung", ctypes.c_int), ("pad_", ctypes.c_int), ("pc_interactions", ctypes.c_double)]
+                ("max_rung", ctypes.c_int), ("pad_", ctypes.c_int), ("pc_interactions", ctypes.c_double),
+                ("moments", TpmMoments)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad_", "moments")}
+        d["moments"] = self.moments.as_dict()
+        return d
 
 
 class TpmTuning(ctypes.Structure):
@@ -157,6 +170,14 @@ def lib():
         "tpm_tree_info": (_int, [vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
         "tpm_tree_download": (_int, [vp, vp, vp, vp, vp, vp]),
         "tpm_ewald": (_int, [vp, _i64, _i64p, _dbl, _f64p, _f64p, _f64p, _f64p]),
+        "tpm_ewald_at": (_int, [vp, _i64, _u32p, _u32p, _u32p, _dbl, _f64p, _f64p, _f64p, _f64p]),
+        "tpm_set_host_order": (_int, [vp, _int]),
+        "tpm_kick_box": (_int, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _int, _dbl, _dbl,
+                                _int, ctypes.c_uint, ctypes.POINTER(TpmStats)]),
+        "tpm_set_budget": (_int, [vp, _i64]),
+        "tpm_tree_sort_particles": (_int, [vp]),
+        "tpm_moments_get": (_int, [vp, ctypes.POINTER(TpmMoments)]),
+        "tpm_batch_moments": (_int, [vp, _int, ctypes.POINTER(TpmMoments)]),
         "tpm_ic_zeldovich": (_int, [vp, _int, ctypes.c_uint64, _dbl, _dbl]),
         "tpm_ic_uniform": (_int, [vp, _i64, ctypes.c_uint64]),
         "tpm_set_tuning": (_int, [vp, ctypes.POINTER(TpmTuning)]),
@@ -312,6 +333,20 @@ class Context:
                                    int(flags), ctypes.byref(st)))
         return st
 
+    def kick_box(self, lo, hi, min_rung=0, scale=1.0, t0=1.0, first_call=True, flags=0):
+        """The kick of the sinks in chain cells [lo, hi) (x, y, z)."""
+        st = TpmStats()
+        a = (ctypes.c_int32 * 3)(*[int(v) for v in lo])
+        b = (ctypes.c_int32 * 3)(*[int(v) for v in hi])
+        self._chk(self._L.tpm_kick_box(self._h, a, b, int(min_rung), float(scale), float(t0), int(bool(first_call)),
+                                       int(flags), ctypes.byref(st)))
+        return st
+
+    def set_budget(self, nbytes):
+        """Device budget of one kick's working set (0: none); kicks over it
+        are split into sub-boxes."""
+        self._chk(self._L.tpm_set_budget(self._h, int(nbytes)))
+
     def tree_build(self):
         self._chk(self._L.tpm_tree_build(self._h))
 
@@ -328,6 +363,10 @@ class Context:
                                                                              "buckets", "order")]))
         return out
 
+    def tree_sort_particles(self):
+        """The caller's order becomes the trees' order (TPM_ORDER_REFERENCE)."""
+        self._chk(self._L.tpm_tree_sort_particles(self._h))
+
     def kick_treepm(self, min_rung=0, scale=1.0, t0=1.0, first_call=True, stats=True):
         st = TpmStats()
         self._chk(self._L.tpm_kick_treepm(self._h, int(min_rung), float(scale), float(t0), int(bool(first_call)),
@@ -339,6 +378,24 @@ class Context:
         out = [np.empty(len(samples), np.float64) for _ in range(4)]
         self._chk(self._L.tpm_ewald(self._h, len(samples), samples, float(alpha), *out))
         return out
+
+    def ewald_at(self, sx, sy, sz, alpha=0.0):
+        """FP64 Ewald potential and force of all resident particles at fixed32
+        positions (nothing left out; softened inside hsoft)."""
+        sx, sy, sz = [np.ascontiguousarray(a, np.uint32) for a in (sx, sy, sz)]
+        out = [np.empty(len(sx), np.float64) for _ in range(4)]
+        self._chk(self._L.tpm_ewald_at(self._h, len(sx), sx, sy, sz, float(alpha), *out))
+        return out
+
+    def set_host_order(self, order):
+        """TPM_ORDER_DEVICE or TPM_ORDER_REFERENCE (the reference partition's
+        order after each chain mesh)."""
+        self._chk(self._L.tpm_set_host_order(self._h, int(order)))
+
+    def moments(self):
+        m = TpmMoments()
+        self._chk(self._L.tpm_moments_get(self._h, ctypes.byref(m)))
+        return m.as_dict()
 
     def ic_zeldovich(self, n_side, seed, rms_disp, spectral_index=-2.0):
         self._chk(self._L.tpm_ic_zeldovich(self._h, int(n_side), int(seed), float(rms_disp), float(spectral_index)))
@@ -466,6 +523,11 @@ class Batch:
         k = ["x", "y", "z", "vx", "vy", "vz", "rung", "gx", "gy", "gz", "pot"]
         self._chk(self._L.tpm_batch_download(self._h, int(box), *[_ptr(out[a]) for a in k]))
         return out
+
+    def moments(self, box):
+        m = TpmMoments()
+        self._chk(self._L.tpm_batch_moments(self._h, int(box), ctypes.byref(m)))
+        return m.as_dict()
 
     def to_ctx(self, box, ctx):
         self._chk(self._L.tpm_batch_to_ctx(self._h, int(box), ctx._h))

@@ -7,6 +7,18 @@
 
 tpm_status tpm_fft_release(tpm_ctx* ctx);
 
+// dst[r] = src[map[r]]: device arrays in the caller's order.
+__global__ static void gather_u32_kernel(int64_t n, const uint32_t* __restrict__ map, const uint32_t* __restrict__ src,
+                                         uint32_t* __restrict__ dst) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        dst[r] = src[map[r]];
+}
+__global__ static void gather_u8_kernel(int64_t n, const uint32_t* __restrict__ map, const uint8_t* __restrict__ src,
+                                        uint8_t* __restrict__ dst) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        dst[r] = src[map[r]];
+}
+
 static std::once_flag g_fft_once;
 static bool g_fft_ok = false;
 
@@ -40,7 +52,9 @@ extern "C" tpm_status tpm_ctx_destroy(tpm_ctx* ctx) {
                     ctx->tiles, ctx->ntiles,  ctx->scan_tmp, ctx->rho_q,     ctx->rho,      ctx->rho_hat, ctx->rho_hat1,
                     ctx->phi,   ctx->green,   ctx->counters, ctx->dscratch, ctx->tree_pos, ctx->tree_tmp,
                     ctx->tree_cnt, ctx->node_begin, ctx->bucket_begin, ctx->tree_part, ctx->tree_flags, ctx->nodes,
-                    ctx->buckets};
+                    ctx->buckets, ctx->h2d, ctx->h2d2, ctx->ref_perm, ctx->ref_cid, ctx->ref_cid2, ctx->ref_idx2,
+                    ctx->ref_posa, ctx->ref_posb, ctx->ref_inv, ctx->ref_flag, ctx->ref_scan, ctx->ref_scan_tmp,
+                    ctx->ref_tab, ctx->moment_buf, ctx->box_tiles, ctx->box_count, ctx->box_buckets, ctx->tree_moved};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ctx->ev)
@@ -124,6 +138,21 @@ extern "C" tpm_status tpm_get_tuning(const tpm_ctx* ctx, tpm_tuning* t) {
     return TPM_OK;
 }
 
+extern "C" tpm_status tpm_set_host_order(tpm_ctx* ctx, int order) {
+    if (!ctx) return TPM_ERR_INVALID;
+    if (order != TPM_ORDER_DEVICE && order != TPM_ORDER_REFERENCE)
+        return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_set_host_order: TPM_ORDER_DEVICE or TPM_ORDER_REFERENCE");
+    (void)hipSetDevice(ctx->device);
+    TPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // from now on the caller's order is the device's current one (trees
+    // start from the caller's order: build them again)
+    ctx->host_order = order;
+    ctx->h2d_identity = true;
+    ctx->ref_valid = false;
+    ctx->tree_epoch = -1;
+    return TPM_OK;
+}
+
 tpm_status tpm_ensure_particles(tpm_ctx* ctx, int64_t n) {
     if (n > (int64_t)INT32_MAX - 1024) return tpm_fail(ctx, TPM_ERR_INVALID, "particle count exceeds 2^31");
     if (n <= ctx->cap && ctx->x) return TPM_OK;
@@ -182,6 +211,8 @@ extern "C" tpm_status tpm_particles_upload(tpm_ctx* ctx, int64_t n, const uint32
     ctx->n = n;
     ctx->chain_valid = false;
     ctx->pm_valid = false;
+    ctx->h2d_identity = true;
+    ctx->ref_valid = false;
     return TPM_OK;
 }
 
@@ -199,8 +230,25 @@ extern "C" tpm_status tpm_particles_download(tpm_ctx* ctx, uint32_t* x, uint32_t
             size_t bytes;
         } cp[] = {{x, ctx->x, 4}, {y, ctx->y, 4}, {z, ctx->z, 4},   {vx, ctx->vx, 4},   {vy, ctx->vy, 4},   {vz, ctx->vz, 4},
                   {rung, ctx->rung, 1}, {gx, ctx->gx, 4}, {gy, ctx->gy, 4}, {gz, ctx->gz, 4}, {pot, ctx->pot, 4}};
-        for (auto& c : cp)
-            if (c.dst) TPM_HIP(ctx, hipMemcpyAsync(c.dst, c.src, c.bytes * n, hipMemcpyDeviceToHost, s));
+        // The caller's order differs from the device's: gather each array
+        // through h2d into a staging buffer first (stream order keeps the
+        // staging buffer's copy ahead of the next gather).
+        const bool mapped = ctx->host_order == TPM_ORDER_REFERENCE && !ctx->h2d_identity;
+        for (auto& c : cp) {
+            if (!c.dst) continue;
+            const void* src = c.src;
+            if (mapped) {
+                if (c.bytes == 4)
+                    gather_u32_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->h2d, (const uint32_t*)c.src,
+                                                                       ctx->ref_cid2);
+                else
+                    gather_u8_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->h2d, (const uint8_t*)c.src,
+                                                                      (uint8_t*)ctx->ref_cid2);
+                TPM_HIP(ctx, hipGetLastError());
+                src = ctx->ref_cid2;
+            }
+            TPM_HIP(ctx, hipMemcpyAsync(c.dst, src, c.bytes * n, hipMemcpyDeviceToHost, s));
+        }
     }
     TPM_HIP(ctx, hipStreamSynchronize(s));
     return TPM_OK;
@@ -247,9 +295,13 @@ extern "C" tpm_status tpm_chainmesh_perm(tpm_ctx* ctx, int64_t* perm) {
     if (!ctx || !perm) return TPM_ERR_INVALID;
     if (!ctx->chain_valid) return tpm_fail(ctx, TPM_ERR_STATE, "chain mesh not built");
     (void)hipSetDevice(ctx->device);
+    const bool ref = ctx->host_order == TPM_ORDER_REFERENCE;
+    if (ref && !ctx->ref_valid)
+        return tpm_fail(ctx, TPM_ERR_STATE, "tpm_chainmesh_perm: the chain mesh was built before TPM_ORDER_REFERENCE");
     std::vector<uint32_t> h(ctx->n);
     if (ctx->n > 0) {
-        TPM_HIP(ctx, hipMemcpyAsync(h.data(), ctx->perm, sizeof(uint32_t) * ctx->n, hipMemcpyDeviceToHost, ctx->stream));
+        TPM_HIP(ctx, hipMemcpyAsync(h.data(), ref ? ctx->ref_perm : ctx->perm, sizeof(uint32_t) * ctx->n,
+                                    hipMemcpyDeviceToHost, ctx->stream));
         TPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     for (int64_t i = 0; i < ctx->n; i++) perm[i] = h[i];
@@ -290,21 +342,205 @@ extern "C" tpm_status tpm_gravity_long_get_green(tpm_ctx* ctx, float* table) {
     return copy_mesh(ctx, table, ctx->green, sizeof(float) * M * M * (M / 2 + 1));
 }
 
-extern "C" tpm_status tpm_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
-                               tpm_stats* stats) {
-    if (!ctx) return TPM_ERR_INVALID;
+// ---------------------------------------------------------------- box kicks
+namespace {
+
+struct BoxCost {
+    int64_t bytes = 0;   // working set of one launch over the box
+    int64_t tiles = 0;   // sink tiles of the box's cells
+};
+
+// The working set of a kick of the sinks in box (the bytes its launch
+// touches; the terms of mem_requirements, kick_pme.cu:110-124 and
+// kick_treepm.cu:140-157, with this build's layout), from the host copy of
+// the cell table and, on the tree path, the node / bucket offsets.
+BoxCost box_cost(const tpm_ctx* ctx, const tpm_cell_box& b, const std::vector<int32_t>& cb,
+                 const std::vector<int32_t>* nodes, const std::vector<int32_t>* buckets) {
+    const int C = ctx->C, M = ctx->M, ts = ctx->tile;
+    BoxCost r;
+    int64_t sinks = 0, sources = 0, pad_cells = 0, tree_nodes = 0, tree_buckets = 0;
+    auto cid = [C](int x, int y, int z) { return ((int64_t)z * C + y) * C + x; };
+    for (int z = b.lo[2]; z < b.hi[2]; z++)
+        for (int y = b.lo[1]; y < b.hi[1]; y++)
+            for (int x = b.lo[0]; x < b.hi[0]; x++) {
+                const int64_t c = cid(x, y, z);
+                const int64_t n = cb[c + 1] - cb[c];
+                sinks += n;
+                r.tiles += (n + ts - 1) / ts;
+                if (buckets) tree_buckets += (*buckets)[c + 1] - (*buckets)[c];
+            }
+    // the box padded by one cell, periodic, each cell once
+    std::vector<int> ax[3];
+    for (int d = 0; d < 3; d++) {
+        const int span = b.hi[d] - b.lo[d];
+        if (span + 2 >= C) {
+            for (int i = 0; i < C; i++) ax[d].push_back(i);
+        } else {
+            for (int i = b.lo[d] - 1; i <= b.hi[d]; i++) ax[d].push_back((i + C) % C);
+        }
+    }
+    for (int z : ax[2])
+        for (int y : ax[1])
+            for (int x : ax[0]) {
+                const int64_t c = cid(x, y, z);
+                sources += cb[c + 1] - cb[c];
+                pad_cells++;
+                if (nodes) tree_nodes += (*nodes)[c + 1] - (*nodes)[c];
+            }
+    // potential mesh nodes under the padded box: the 6-point stencil reaches
+    // 3 nodes beyond a sink's cell (PHI_BW, tigerpm.hpp:41)
+    int64_t phivol = 1;
+    for (int d = 0; d < 3; d++) {
+        const int64_t w = ((int64_t)(b.hi[d] - b.lo[d]) * M + C - 1) / C + 1 + 6;
+        phivol *= std::min<int64_t>(w, M);
+    }
+    r.bytes = 12 * sources + 41 * sinks + 16 * r.tiles + 4 * phivol + 8 * pad_cells + 48 * tree_nodes +
+              (int64_t)sizeof(tpm_tree_bucket) * tree_buckets;
+    return r;
+}
+
+struct BoxKick {
+    tpm_ctx* ctx;
+    int min_rung;
+    double scale, t0;
+    int first_call;
+    unsigned flags;
+    tpm_stats* acc;  // accumulated statistics, or nullptr
+    std::vector<int32_t> cb, nodes, buckets;
+};
+
+tpm_status box_kick_rec(BoxKick& K, const tpm_cell_box& b) {
+    tpm_ctx* ctx = K.ctx;
+    const bool tree = (K.flags & TPM_KICK_TREE) != 0;
+    const BoxCost cost = box_cost(ctx, b, K.cb, tree ? &K.nodes : nullptr, tree ? &K.buckets : nullptr);
+    if (ctx->budget > 0 && cost.bytes > ctx->budget) {
+        const int vol = (b.hi[0] - b.lo[0]) * (b.hi[1] - b.lo[1]) * (b.hi[2] - b.lo[2]);
+        if (vol <= 1)
+            return tpm_fail(ctx, TPM_ERR_BUDGET,
+                            "kick: one chain cell needs " + std::to_string(cost.bytes) + " bytes, budget " +
+                                std::to_string(ctx->budget));
+        // range::split (range.hpp:118-133): the first longest axis, at its middle
+        int dim = 0, span = 0;
+        for (int d = 0; d < 3; d++)
+            if (b.hi[d] - b.lo[d] > span) {
+                span = b.hi[d] - b.lo[d];
+                dim = d;
+            }
+        const int mid = (b.hi[dim] + b.lo[dim]) / 2;
+        tpm_cell_box l = b, r = b;
+        l.hi[dim] = r.lo[dim] = mid;
+        TPM_TRY(box_kick_rec(K, l));
+        return box_kick_rec(K, r);
+    }
+    tpm_stats st{};
+    tpm_stats* sp = K.acc ? &st : nullptr;
+    if (tree)
+        TPM_TRY(tpm_tree_kick(ctx, K.min_rung, K.scale, K.t0, K.first_call, K.flags, sp, &b));
+    else
+        TPM_TRY(tpm_pp_kick(ctx, K.min_rung, K.scale, K.t0, K.first_call, K.flags, sp, &b, cost.tiles));
+    if (K.acc) {
+        K.acc->candidate_pairs += st.candidate_pairs;
+        K.acc->executed_pairs += st.executed_pairs;
+        K.acc->pc_interactions += st.pc_interactions;
+        K.acc->ms_pp += st.ms_pp;
+        K.acc->n_tiles += st.n_tiles;
+        K.acc->max_rung = std::max(K.acc->max_rung, st.max_rung);
+    }
+    return TPM_OK;
+}
+
+}  // namespace
+
+tpm_status tpm_kick_any(tpm_ctx* ctx, const tpm_cell_box* box, int min_rung, double scale, double t0, int first_call,
+                        unsigned flags, tpm_stats* stats) {
+    const bool tree = (flags & TPM_KICK_TREE) != 0;
+    if (!box && ctx->budget <= 0) {
+        // the whole box in one launch
+        if (tree) return tpm_tree_kick(ctx, min_rung, scale, t0, first_call, flags, stats);
+        return tpm_pp_kick(ctx, min_rung, scale, t0, first_call, flags, stats);
+    }
+    if (!ctx->chain_valid || ctx->chain_C != ctx->C || (!tree && ctx->chain_tile != ctx->tile))
+        return tpm_fail(ctx, TPM_ERR_STATE, "tpm_kick: chain mesh not built (call tpm_chainmesh_create)");
+    const int C = ctx->C;
+    tpm_cell_box b{{0, 0, 0}, {C, C, C}};
+    if (box) {
+        b = *box;
+        for (int d = 0; d < 3; d++)
+            if (b.lo[d] < 0 || b.hi[d] > C || b.lo[d] >= b.hi[d])
+                return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick_box: need 0 <= lo < hi <= chain_dim on every axis");
+    }
+    if (tree) TPM_TRY(tpm_tree_ensure(ctx));
+    BoxKick K{ctx, min_rung, scale, t0, first_call, flags, stats, {}, {}, {}};
+    const int64_t ncell = (int64_t)C * C * C;
+    hipStream_t s = ctx->stream;
+    if (ctx->h_cells_epoch != ctx->chain_epoch) {
+        ctx->h_cells.resize((size_t)ncell + 1);
+        TPM_HIP(ctx, hipMemcpyAsync(ctx->h_cells.data(), ctx->cell_begin, sizeof(int32_t) * (ncell + 1),
+                                    hipMemcpyDeviceToHost, s));
+        TPM_HIP(ctx, hipStreamSynchronize(s));
+        ctx->h_cells_epoch = ctx->chain_epoch;
+    }
+    K.cb = ctx->h_cells;
+    if (tree) {
+        K.nodes.resize((size_t)ncell + 1);
+        K.buckets.resize((size_t)ncell + 1);
+        TPM_HIP(ctx, hipMemcpyAsync(K.nodes.data(), ctx->node_begin, sizeof(int32_t) * (ncell + 1),
+                                    hipMemcpyDeviceToHost, s));
+        TPM_HIP(ctx, hipMemcpyAsync(K.buckets.data(), ctx->bucket_begin, sizeof(int32_t) * (ncell + 1),
+                                    hipMemcpyDeviceToHost, s));
+        TPM_HIP(ctx, hipStreamSynchronize(s));
+    }
+    return box_kick_rec(K, b);
+}
+
+extern "C" tpm_status tpm_set_budget(tpm_ctx* ctx, int64_t bytes) {
+    if (!ctx || bytes < 0) return TPM_ERR_INVALID;
+    ctx->budget = bytes;
+    return TPM_OK;
+}
+
+static tpm_status check_kick_flags(tpm_ctx* ctx, unsigned flags) {
     if (flags & ~(TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK | TPM_KICK_PME | TPM_KICK_TREE))
         return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: unknown flags");
     if ((flags & TPM_KICK_PME) && (flags & TPM_KICK_TREE))
         return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: TPM_KICK_PME and TPM_KICK_TREE exclude each other");
+    return TPM_OK;
+}
+
+extern "C" tpm_status tpm_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
+                               tpm_stats* stats) {
+    if (!ctx) return TPM_ERR_INVALID;
+    TPM_TRY(check_kick_flags(ctx, flags));
     (void)hipSetDevice(ctx->device);
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    if (flags & TPM_KICK_TREE)
-        TPM_TRY(tpm_tree_kick(ctx, min_rung, scale, t0, first_call, flags, stats));
-    else
-        TPM_TRY(tpm_pp_kick(ctx, min_rung, scale, t0, first_call, flags, stats));
-    if (stats) stats->ms_total = stats->ms_pp;
+    TPM_TRY(tpm_kick_any(ctx, nullptr, min_rung, scale, t0, first_call, flags, stats));
+    if (stats) {
+        stats->ms_total = stats->ms_pp;
+        TPM_TRY(tpm_ctx_moments(ctx, &stats->moments));
+    }
     return TPM_OK;
+}
+
+extern "C" tpm_status tpm_kick_box(tpm_ctx* ctx, const int32_t lo[3], const int32_t hi[3], int min_rung, double scale,
+                                   double t0, int first_call, unsigned flags, tpm_stats* stats) {
+    if (!ctx || !lo || !hi) return TPM_ERR_INVALID;
+    TPM_TRY(check_kick_flags(ctx, flags));
+    (void)hipSetDevice(ctx->device);
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    const tpm_cell_box b{{lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+    TPM_TRY(tpm_kick_any(ctx, &b, min_rung, scale, t0, first_call, flags, stats));
+    if (stats) {
+        stats->ms_total = stats->ms_pp;
+        TPM_TRY(tpm_ctx_moments(ctx, &stats->moments));
+    }
+    return TPM_OK;
+}
+
+extern "C" tpm_status tpm_moments_get(tpm_ctx* ctx, tpm_moments* out) {
+    if (!ctx || !out) return TPM_ERR_INVALID;
+    if (!ctx->params_set) return tpm_fail(ctx, TPM_ERR_STATE, "tpm_moments_get: parameters not set (GM)");
+    (void)hipSetDevice(ctx->device);
+    return tpm_ctx_moments(ctx, out);
 }
 
 extern "C" tpm_status tpm_kick_treepm(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call,
@@ -329,7 +565,7 @@ extern "C" tpm_status tpm_kick_treepm(tpm_ctx* ctx, int min_rung, double scale, 
     TPM_HIP(ctx, hipEventRecord(ev[1], s));
     TPM_TRY(tpm_pm_solve(ctx));
     TPM_HIP(ctx, hipEventRecord(ev[2], s));
-    TPM_TRY(tpm_pp_kick(ctx, min_rung, scale, t0, first_call, 0u, stats));
+    TPM_TRY(tpm_kick_any(ctx, nullptr, min_rung, scale, t0, first_call, 0u, stats));
     TPM_HIP(ctx, hipEventRecord(ev[3], s));
     if (stats) {
         TPM_HIP(ctx, hipEventSynchronize(ev[3]));
@@ -340,6 +576,7 @@ extern "C" tpm_status tpm_kick_treepm(tpm_ctx* ctx, int min_rung, double scale, 
         stats->ms_chainmesh = a;
         stats->ms_pm = b;
         stats->ms_total = c;
+        TPM_TRY(tpm_ctx_moments(ctx, &stats->moments));
     }
     return TPM_OK;
 }
@@ -386,7 +623,17 @@ extern "C" tpm_status tpm_ewald(tpm_ctx* ctx, int64_t nsamples, const int64_t* s
     if (!ctx || nsamples < 0 || (nsamples > 0 && (!samples || !phi || !gx || !gy || !gz))) return TPM_ERR_INVALID;
     if (!ctx->params_set) return tpm_fail(ctx, TPM_ERR_STATE, "tpm_ewald: parameters not set (hsoft)");
     (void)hipSetDevice(ctx->device);
-    return tpm_ewald_run(ctx, nsamples, samples, alpha, phi, gx, gy, gz);
+    return tpm_ewald_run(ctx, nsamples, samples, nullptr, alpha, phi, gx, gy, gz);
+}
+
+extern "C" tpm_status tpm_ewald_at(tpm_ctx* ctx, int64_t nsamples, const uint32_t* sx, const uint32_t* sy,
+                                   const uint32_t* sz, double alpha, double* phi, double* gx, double* gy, double* gz) {
+    if (!ctx || nsamples < 0 || (nsamples > 0 && (!sx || !sy || !sz || !phi || !gx || !gy || !gz)))
+        return TPM_ERR_INVALID;
+    if (!ctx->params_set) return tpm_fail(ctx, TPM_ERR_STATE, "tpm_ewald_at: parameters not set (hsoft)");
+    (void)hipSetDevice(ctx->device);
+    const uint32_t* pos[3] = {sx, sy, sz};
+    return tpm_ewald_run(ctx, nsamples, nullptr, pos, alpha, phi, gx, gy, gz);
 }
 
 extern "C" tpm_status tpm_ic_zeldovich(tpm_ctx* ctx, int n_side, uint64_t seed, double rms_disp,

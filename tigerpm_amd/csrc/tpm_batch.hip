@@ -434,7 +434,8 @@ extern "C" tpm_status tpm_batch_create(int device, const tpm_batch_limits* l, tp
         alloc(bt, &bt->tiles, c.t) || alloc(bt, &bt->rho_q, c.m) || alloc(bt, &bt->rho, c.m) ||
         alloc(bt, &bt->phi, c.m) || alloc(bt, &bt->rho_hat, c.k) || alloc(bt, &bt->green, c.k) ||
         (l->interlace && alloc(bt, &bt->rho_hat1, c.k)) || alloc(bt, &bt->d_boxes, l->max_boxes) ||
-        alloc(bt, &bt->d_kreg, l->max_boxes) || alloc(bt, &bt->counters, TPM_COUNTER_WORDS))
+        alloc(bt, &bt->d_kreg, l->max_boxes) || alloc(bt, &bt->counters, TPM_COUNTER_WORDS) ||
+        alloc(bt, &bt->moment_buf, 8 * TPM_MOMENT_BLOCKS + 8))
         return fail(TPM_ERR_BUDGET);
     if (l->interlace && alloc(bt, &bt->rho1, c.m)) return fail(TPM_ERR_BUDGET);
     for (int k = 0; k < tpm_batch::NSIDE; k++)
@@ -480,7 +481,7 @@ extern "C" tpm_status tpm_batch_destroy(tpm_batch* bt) {
                     bt->gx,    bt->gy,       bt->gz,       bt->pot,      bt->key,    bt->slot,    bt->tmp,
                     bt->perm,  bt->cnt,      bt->cell_begin, bt->tile_cnt, bt->tile_off, bt->scan_part, bt->tiles,
                     bt->rho_q, bt->rho,      bt->phi,      bt->rho_hat,  bt->rho_hat1, bt->green,  bt->d_boxes,
-                    bt->d_kreg, bt->counters};
+                    bt->d_kreg, bt->counters, bt->moment_buf};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (bt->h_boxes) (void)hipHostFree(bt->h_boxes);
@@ -616,6 +617,27 @@ extern "C" tpm_status tpm_batch_download(tpm_batch* bt, int id, uint32_t* x, uin
     return TPM_OK;
 }
 
+extern "C" tpm_status tpm_batch_moments(tpm_batch* bt, int id, tpm_moments* out) {
+    if (!bt || !out) return TPM_ERR_INVALID;
+    tpm_batch_box* b = find_box(bt, id);
+    if (!b) return tpm_fail(bt, TPM_ERR_INVALID, "tpm_batch_moments: no such box");
+    (void)hipSetDevice(bt->device);
+    const int64_t o = b->p0, n = b->n;
+    double* res = bt->moment_buf + 8 * TPM_MOMENT_BLOCKS;
+    TPM_TRY(tpm_moments_enqueue(bt->err, bt->stream, n, bt->vx + o, bt->vy + o, bt->vz + o, bt->gx + o, bt->gy + o,
+                                bt->gz + o, bt->pot + o, b->spec.params.GM, bt->moment_buf, res));
+    double h[8];
+    TPM_HIP(bt, hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, bt->stream));
+    TPM_HIP(bt, hipStreamSynchronize(bt->stream));
+    for (int d = 0; d < 3; d++) {
+        out->momentum[d] = h[d];
+        out->force[d] = h[3 + d];
+    }
+    out->kinetic = h[6];
+    out->potential = h[7];
+    return TPM_OK;
+}
+
 extern "C" tpm_status tpm_batch_to_ctx(tpm_batch* bt, int id, tpm_ctx* ctx) {
     if (!bt || !ctx) return TPM_ERR_INVALID;
     tpm_batch_box* b = find_box(bt, id);
@@ -642,6 +664,8 @@ extern "C" tpm_status tpm_batch_to_ctx(tpm_batch* bt, int id, tpm_ctx* ctx) {
     TPM_HIP(bt, hipStreamSynchronize(st));
     ctx->n = n;
     ctx->chain_valid = false;
+    ctx->h2d_identity = true;
+    ctx->ref_valid = false;
     ctx->pm_valid = false;
     return TPM_OK;
 }

@@ -133,7 +133,241 @@ __global__ void tiles_write_kernel(int64_t ncell, int C, int ts, const int32_t* 
     }
 }
 
+// ---------------------------------------------------------------------------
+// The order of the reference partition (chainmesh.cpp:168-213 with the
+// two-pointer particles_sort, particles.cpp:414-445), found level by level.
+//
+// The recursion splits a box of chain cells on its last axis that is more
+// than one cell wide (z until the box is one z plane, then y, then x) at
+// mid = begin + span / 2.  particles_sort partitions the box's particle range
+// [s, e) in place: lo walks up, and every particle lo meets that belongs to
+// the right is swapped with the next particle that belongs to the left,
+// searched downwards from the top.  The sweep has a closed form: with m =
+// s + (particles of the left box), the k-th right particle of [s, m)
+// (counted upwards) trades places with the k-th left particle of [m, e)
+// (counted downwards); nothing else moves.  Every box of the recursion is a
+// contiguous run of cell ids (z-major order), so s, m and e are entries of
+// the finished cell table.  One level moves every range of that depth at
+// once: a flag per position, one exclusive scan of
+// (right-in-left << 32 | left-in-right), the positions of the misplaced
+// particles by scan index, and the exchange.
+// ---------------------------------------------------------------------------
+#define REF_LEVELS 10  // bisection levels per axis: ceil(log2(TPM_MAX_CHAIN)) <= 9
+
+// Box of the recursion at level j for the particle of cell c: its cell-id
+// range [c0, c1) split at cm; false when the particle's cell is final.
+__device__ __forceinline__ bool ref_box(uint32_t c, int C, int j, const int4* __restrict__ tab, int64_t* c0,
+                                        int64_t* cm, int64_t* c1, bool* right) {
+    const int ix = (int)(c % (uint32_t)C), iy = (int)((c / (uint32_t)C) % (uint32_t)C),
+              iz = (int)(c / ((uint32_t)C * (uint32_t)C));
+    const int dz = tab[iz * REF_LEVELS].w, dy = tab[iy * REF_LEVELS].w, dx = tab[ix * REF_LEVELS].w;
+    int4 t;
+    int64_t base, stride;
+    int coord;
+    if (j < dz) {
+        t = tab[iz * REF_LEVELS + j];
+        base = 0;
+        stride = (int64_t)C * C;
+        coord = iz;
+    } else if (j < dz + dy) {
+        t = tab[iy * REF_LEVELS + (j - dz)];
+        base = (int64_t)iz * C * C;
+        stride = C;
+        coord = iy;
+    } else if (j < dz + dy + dx) {
+        t = tab[ix * REF_LEVELS + (j - dz - dy)];
+        base = ((int64_t)iz * C + iy) * C;
+        stride = 1;
+        coord = ix;
+    } else {
+        return false;
+    }
+    *c0 = base + t.x * stride;
+    *cm = base + t.y * stride;
+    *c1 = base + t.z * stride;
+    *right = coord >= t.y;
+    return true;
+}
+
+constexpr unsigned long long REF_LO = 0xffffffffull;
+
+// cid[h] = cell of the caller's particle h, idx[h] = h.
+__global__ void ref_init_kernel(int64_t n, const uint32_t* __restrict__ key, const uint32_t* __restrict__ h2d,
+                                uint32_t* __restrict__ cid, uint32_t* __restrict__ idx) {
+    for (int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; h < n; h += (int64_t)gridDim.x * blockDim.x) {
+        cid[h] = key[h2d ? h2d[h] : (uint32_t)h] >> 6;
+        idx[h] = (uint32_t)h;
+    }
+}
+
+__global__ void ref_flag_kernel(int64_t n, int C, int j, const int4* __restrict__ tab,
+                                const int32_t* __restrict__ cb, const uint32_t* __restrict__ cid,
+                                unsigned long long* __restrict__ flag) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p <= n; p += (int64_t)gridDim.x * blockDim.x) {
+        unsigned long long f = 0ull;
+        int64_t c0, cm, c1;
+        bool right;
+        if (p < n && ref_box(cid[p], C, j, tab, &c0, &cm, &c1, &right)) {
+            const int64_t m = cb[cm];
+            if (p < m && right) f = 1ull << 32;
+            if (p >= m && !right) f = 1ull;
+        }
+        flag[p] = f;
+    }
+}
+
+__global__ void ref_scatter_kernel(int64_t n, const unsigned long long* __restrict__ flag,
+                                   const unsigned long long* __restrict__ scan, uint32_t* __restrict__ posa,
+                                   uint32_t* __restrict__ posb) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long f = flag[p];
+        if (f >> 32) posa[scan[p] >> 32] = (uint32_t)p;
+        if (f & REF_LO) posb[scan[p] & REF_LO] = (uint32_t)p;
+    }
+}
+
+__global__ void ref_move_kernel(int64_t n, int C, int j, const int4* __restrict__ tab, const int32_t* __restrict__ cb,
+                                const unsigned long long* __restrict__ flag,
+                                const unsigned long long* __restrict__ scan, const uint32_t* __restrict__ posa,
+                                const uint32_t* __restrict__ posb, const uint32_t* __restrict__ cid,
+                                const uint32_t* __restrict__ idx, uint32_t* __restrict__ cid2,
+                                uint32_t* __restrict__ idx2) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = cid[p];
+        const unsigned long long f = flag[p];
+        int64_t dst = p;
+        if (f) {
+            int64_t c0, cm, c1;
+            bool right;
+            ref_box(c, C, j, tab, &c0, &cm, &c1, &right);
+            const unsigned long long ss = scan[cb[c0]], se = scan[cb[c1]];
+            if (f >> 32) {
+                // k-th right particle of [s, m) <-> k-th left particle of [m, e) from the top
+                const unsigned long long k = (scan[p] >> 32) - (ss >> 32);
+                dst = posb[(se & REF_LO) - 1ull - k];
+            } else {
+                const unsigned long long k = (se & REF_LO) - 1ull - (scan[p] & REF_LO);
+                dst = posa[(ss >> 32) + k];
+            }
+        }
+        cid2[dst] = c;
+        idx2[dst] = idx[p];
+    }
+}
+
+// inv[perm[d]] = d: device position after the sort of the particle at
+// device position perm[d] before it.
+__global__ void invert_kernel(int64_t n, const uint32_t* __restrict__ perm, uint32_t* __restrict__ inv) {
+    for (int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; d < n; d += (int64_t)gridDim.x * blockDim.x)
+        inv[perm[d]] = (uint32_t)d;
+}
+
+// h2d_new[r] = device position of the particle the caller now sees at r.
+__global__ void ref_compose_kernel(int64_t n, const uint32_t* __restrict__ ref, const uint32_t* __restrict__ h2d_old,
+                                   const uint32_t* __restrict__ inv, uint32_t* __restrict__ h2d_new) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t h = ref[r];
+        h2d_new[r] = inv[h2d_old ? h2d_old[h] : h];
+    }
+}
+
 }  // namespace
+
+// Per axis coordinate i and bisection level k (rows of REF_LEVELS): the
+// interval [lo, hi) of the reference partition's boxes that holds i and its
+// split point mid = lo + (hi - lo) / 2 (chainmesh.cpp:189-201); w = the
+// number of levels until i's interval is one cell wide.
+static void ref_axis_table(int C, std::vector<int4>& tab) {
+    tab.assign((size_t)C * REF_LEVELS, make_int4(0, 0, 0, 0));
+    for (int i = 0; i < C; i++) {
+        int lo = 0, hi = C, k = 0;
+        while (hi - lo > 1) {
+            const int mid = lo + (hi - lo) / 2;
+            tab[(size_t)i * REF_LEVELS + k] = make_int4(lo, mid, hi, 0);
+            if (i < mid)
+                hi = mid;
+            else
+                lo = mid;
+            k++;
+        }
+        for (int q = 0; q < REF_LEVELS; q++) tab[(size_t)i * REF_LEVELS + q].w = k;
+    }
+}
+
+// The reference partition of the caller's order, after tpm_chainmesh_build
+// has sorted and gathered (ctx->keys: cell keys in the device order before
+// the sort, ctx->perm: the sort's permutation, ctx->cell_begin: the table).
+static tpm_status ref_order(tpm_ctx* ctx) {
+    const int C = ctx->C;
+    const int64_t n = ctx->n;
+    hipStream_t s = ctx->stream;
+    if (n > ctx->ref_cap || !ctx->h2d) {
+        uint32_t** u32s[] = {&ctx->h2d,      &ctx->h2d2,     &ctx->ref_perm, &ctx->ref_cid, &ctx->ref_cid2,
+                             &ctx->ref_idx2, &ctx->ref_posa, &ctx->ref_posb, &ctx->ref_inv};
+        for (uint32_t** b : u32s) {
+            if (*b) TPM_HIP(ctx, hipFree(*b));
+            *b = nullptr;
+            TPM_HIP(ctx, hipMalloc(b, sizeof(uint32_t) * std::max<int64_t>(n, 1)));
+        }
+        unsigned long long** u64s[] = {&ctx->ref_flag, &ctx->ref_scan};
+        for (unsigned long long** b : u64s) {
+            if (*b) TPM_HIP(ctx, hipFree(*b));
+            *b = nullptr;
+            TPM_HIP(ctx, hipMalloc(b, sizeof(unsigned long long) * (n + 1)));
+        }
+        ctx->ref_cap = std::max<int64_t>(n, 1);
+        ctx->h2d_identity = true;  // the old map went with its buffer; only reached after an upload
+    }
+    if (ctx->ref_tab_C != C) {
+        std::vector<int4> tab;
+        ref_axis_table(C, tab);
+        if (ctx->ref_tab) TPM_HIP(ctx, hipFree(ctx->ref_tab));
+        ctx->ref_tab = nullptr;
+        TPM_HIP(ctx, hipMalloc(&ctx->ref_tab, sizeof(int4) * tab.size()));
+        TPM_HIP(ctx, hipMemcpy(ctx->ref_tab, tab.data(), sizeof(int4) * tab.size(), hipMemcpyHostToDevice));
+        ctx->ref_tab_C = C;
+    }
+    int depth = 0;
+    while ((1 << depth) < C) depth++;
+    const uint32_t* old_h2d = ctx->h2d_identity ? nullptr : ctx->h2d;
+    const unsigned g = grid_for(n, 256), g1 = grid_for(n + 1, 256);
+    uint32_t *cid = ctx->ref_cid, *cid2 = ctx->ref_cid2, *idx = ctx->ref_perm, *idx2 = ctx->ref_idx2;
+    ref_init_kernel<<<g, 256, 0, s>>>(n, ctx->keys, old_h2d, cid, idx);
+    TPM_HIP(ctx, hipGetLastError());
+    size_t need = 0;
+    TPM_HIP(ctx, rocprim::exclusive_scan(nullptr, need, ctx->ref_flag, ctx->ref_scan, 0ull, (size_t)(n + 1),
+                                         rocprim::plus<unsigned long long>(), s));
+    if (need > ctx->ref_scan_tmp_bytes) {
+        if (ctx->ref_scan_tmp) TPM_HIP(ctx, hipFree(ctx->ref_scan_tmp));
+        ctx->ref_scan_tmp = nullptr;
+        TPM_HIP(ctx, hipMalloc(&ctx->ref_scan_tmp, need));
+        ctx->ref_scan_tmp_bytes = need;
+    }
+    for (int j = 0; j < 3 * depth; j++) {
+        ref_flag_kernel<<<g1, 256, 0, s>>>(n, C, j, ctx->ref_tab, ctx->cell_begin, cid, ctx->ref_flag);
+        TPM_HIP(ctx, hipGetLastError());
+        TPM_HIP(ctx, rocprim::exclusive_scan(ctx->ref_scan_tmp, need, ctx->ref_flag, ctx->ref_scan, 0ull,
+                                             (size_t)(n + 1), rocprim::plus<unsigned long long>(), s));
+        ref_scatter_kernel<<<g, 256, 0, s>>>(n, ctx->ref_flag, ctx->ref_scan, ctx->ref_posa, ctx->ref_posb);
+        ref_move_kernel<<<g, 256, 0, s>>>(n, C, j, ctx->ref_tab, ctx->cell_begin, ctx->ref_flag, ctx->ref_scan,
+                                          ctx->ref_posa, ctx->ref_posb, cid, idx, cid2, idx2);
+        TPM_HIP(ctx, hipGetLastError());
+        std::swap(cid, cid2);
+        std::swap(idx, idx2);
+    }
+    // idx now holds the reference permutation; keep it as ref_perm
+    ctx->ref_perm = idx;
+    ctx->ref_idx2 = idx2;
+    ctx->ref_cid = cid;
+    ctx->ref_cid2 = cid2;
+    invert_kernel<<<g, 256, 0, s>>>(n, ctx->perm, ctx->ref_inv);
+    ref_compose_kernel<<<g, 256, 0, s>>>(n, ctx->ref_perm, old_h2d, ctx->ref_inv, ctx->h2d2);
+    TPM_HIP(ctx, hipGetLastError());
+    std::swap(ctx->h2d, ctx->h2d2);
+    ctx->h2d_identity = false;
+    ctx->ref_valid = true;
+    return TPM_OK;
+}
 
 tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
     if (!ctx->params_set) return tpm_fail(ctx, TPM_ERR_STATE, "tpm_chainmesh_create: parameters not set");
@@ -178,6 +412,7 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
         ctx->chain_C = C;
         ctx->chain_tile = ts;
         ctx->chain_epoch++;
+        ctx->ref_valid = ctx->host_order == TPM_ORDER_REFERENCE;
         return TPM_OK;
     }
 
@@ -231,6 +466,8 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
     ctx->chain_C = C;
     ctx->chain_tile = ts;
     ctx->chain_epoch++;
+    ctx->ref_valid = false;
+    if (ctx->host_order == TPM_ORDER_REFERENCE) TPM_TRY(ref_order(ctx));
     return TPM_OK;
 }
 

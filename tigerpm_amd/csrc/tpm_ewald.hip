@@ -9,8 +9,10 @@
 // through the structure factor S(h) = sum_j exp(2 pi i h.x_j) over the half
 // space 0 < |h| <= 5.2 alpha/pi.  All arithmetic is FP64; source phases are
 // formed exactly in 32-bit fixed point (h.x mod 1) before sincospi.
-// Unit masses; phi excludes the sample's own singular term but includes its
-// image/background term, the convention of the CPU restatement.
+// Unit masses.  At a sample particle, phi excludes the particle's own
+// singular term but includes its image/background term (the convention of
+// the CPU restatement); at an arbitrary position every particle counts,
+// softened inside hsoft.
 #include "tpm_internal.hpp"
 
 namespace {
@@ -57,16 +59,15 @@ __global__ __launch_bounds__(256) void structure_factor_kernel(int64_t n, const 
 }
 
 // Reciprocal-space potential and force at each sample.
-__global__ void kspace_kernel(int64_t ns, const int64_t* __restrict__ samples, const uint32_t* __restrict__ x,
-                              const uint32_t* __restrict__ y, const uint32_t* __restrict__ z, int nh,
+__global__ void kspace_kernel(int64_t ns, const uint32_t* __restrict__ spx, const uint32_t* __restrict__ spy,
+                              const uint32_t* __restrict__ spz, int nh,
                               const int4* __restrict__ hl, const double* __restrict__ coef,
                               const double* __restrict__ Sre, const double* __restrict__ Sim,
                               double* __restrict__ phi, double* __restrict__ gx, double* __restrict__ gy,
                               double* __restrict__ gz) {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= ns) return;
-    const int64_t i = samples[s];
-    const uint32_t xi = x[i], yi = y[i], zi = z[i];
+    const uint32_t xi = spx[s], yi = spy[s], zi = spz[s];
     double P = 0.0, G0 = 0.0, G1 = 0.0, G2 = 0.0;
     for (int k = 0; k < nh; k++) {
         const int4 hv = hl[k];
@@ -89,10 +90,15 @@ __global__ void kspace_kernel(int64_t ns, const int64_t* __restrict__ samples, c
 
 // Real space on the minimum image: 64 samples per block (one per lane) over
 // a chunk of sources staged in LDS.
+// excl[s] (when given) is the particle sitting at sample s, whose own pair
+// is left out.
 __global__ __launch_bounds__(64) void realspace_kernel(int64_t n, const uint32_t* __restrict__ x,
                                                        const uint32_t* __restrict__ y,
                                                        const uint32_t* __restrict__ z, int64_t ns,
-                                                       const int64_t* __restrict__ samples, int64_t per_block,
+                                                       const uint32_t* __restrict__ spx,
+                                                       const uint32_t* __restrict__ spy,
+                                                       const uint32_t* __restrict__ spz,
+                                                       const int64_t* __restrict__ excl, int64_t per_block,
                                                        double alpha, double hsoft, double* __restrict__ phi,
                                                        double* __restrict__ gx, double* __restrict__ gy,
                                                        double* __restrict__ gz) {
@@ -102,10 +108,10 @@ __global__ __launch_bounds__(64) void realspace_kernel(int64_t n, const uint32_t
     int64_t i = -1;
     uint32_t xi = 0, yi = 0, zi = 0;
     if (ok) {
-        i = samples[s];
-        xi = x[i];
-        yi = y[i];
-        zi = z[i];
+        i = excl ? excl[s] : -1;
+        xi = spx[s];
+        yi = spy[s];
+        zi = spz[s];
     }
     const double a = alpha;
     const double rc = 4.8 / a;
@@ -175,6 +181,22 @@ __global__ void ewald_init_kernel(int64_t ns, double c, double* phi, double* gx,
     }
 }
 
+// Positions of the sample particles (their device positions through map,
+// when the caller's order differs from the device's).
+__global__ void sample_pos_kernel(int64_t ns, const uint32_t* __restrict__ map, int64_t* __restrict__ samp,
+                                  const uint32_t* __restrict__ x, const uint32_t* __restrict__ y,
+                                  const uint32_t* __restrict__ z, uint32_t* __restrict__ spx,
+                                  uint32_t* __restrict__ spy, uint32_t* __restrict__ spz) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    int64_t i = samp[s];
+    if (map) i = map[i];
+    samp[s] = i;
+    spx[s] = x[i];
+    spy[s] = y[i];
+    spz[s] = z[i];
+}
+
 __global__ void fill_f64_kernel(double* p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0.0;
@@ -182,11 +204,13 @@ __global__ void fill_f64_kernel(double* p, int64_t n) {
 
 }  // namespace
 
-tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, double alpha, double* phi, double* gx,
-                         double* gy, double* gz) {
+tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, const uint32_t* const pos[3],
+                         double alpha, double* phi, double* gx, double* gy, double* gz) {
     if (ns <= 0) return TPM_OK;
-    for (int64_t s = 0; s < ns; s++)
-        if (samples[s] < 0 || samples[s] >= ctx->n) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_ewald: bad sample index");
+    if (samples)
+        for (int64_t s = 0; s < ns; s++)
+            if (samples[s] < 0 || samples[s] >= ctx->n)
+                return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_ewald: bad sample index");
     if (alpha <= 0.0) alpha = 9.6;
     if (alpha < 9.6) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_ewald: alpha must be >= 9.6 (minimum image only)");
     const int hmax = (int)floor(5.2 * alpha / M_PI);
@@ -206,12 +230,14 @@ tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, doubl
     int4* d_hl = nullptr;
     double *d_coef = nullptr, *d_S = nullptr, *d_out = nullptr;
     int64_t* d_samp = nullptr;
+    uint32_t* d_pos = nullptr;
     auto cleanup = [&]() {
         if (d_hl) (void)hipFree(d_hl);
         if (d_coef) (void)hipFree(d_coef);
         if (d_S) (void)hipFree(d_S);
         if (d_out) (void)hipFree(d_out);
         if (d_samp) (void)hipFree(d_samp);
+        if (d_pos) (void)hipFree(d_pos);
     };
 #define EW_HIP(call)                                                   \
     do {                                                               \
@@ -226,18 +252,30 @@ tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, doubl
     EW_HIP(hipMalloc(&d_coef, sizeof(double) * nh));
     EW_HIP(hipMalloc(&d_S, sizeof(double) * 2 * nh));
     EW_HIP(hipMalloc(&d_out, sizeof(double) * 4 * ns));
-    EW_HIP(hipMalloc(&d_samp, sizeof(int64_t) * ns));
+    EW_HIP(hipMalloc(&d_pos, sizeof(uint32_t) * 3 * ns));
     EW_HIP(hipMemcpyAsync(d_hl, hl.data(), sizeof(int4) * nh, hipMemcpyHostToDevice, st));
     EW_HIP(hipMemcpyAsync(d_coef, coef.data(), sizeof(double) * nh, hipMemcpyHostToDevice, st));
-    EW_HIP(hipMemcpyAsync(d_samp, samples, sizeof(int64_t) * ns, hipMemcpyHostToDevice, st));
+    uint32_t *spx = d_pos, *spy = d_pos + ns, *spz = d_pos + 2 * ns;
+    if (samples) {
+        EW_HIP(hipMalloc(&d_samp, sizeof(int64_t) * ns));
+        EW_HIP(hipMemcpyAsync(d_samp, samples, sizeof(int64_t) * ns, hipMemcpyHostToDevice, st));
+        const uint32_t* map =
+            ctx->host_order == TPM_ORDER_REFERENCE && !ctx->h2d_identity ? ctx->h2d : nullptr;
+        sample_pos_kernel<<<grid_for(ns, 256), 256, 0, st>>>(ns, map, d_samp, ctx->x, ctx->y, ctx->z, spx, spy, spz);
+    } else {
+        for (int d = 0; d < 3; d++)
+            EW_HIP(hipMemcpyAsync(d_pos + d * ns, pos[d], sizeof(uint32_t) * ns, hipMemcpyHostToDevice, st));
+    }
     fill_f64_kernel<<<grid_for(2 * nh, 256), 256, 0, st>>>(d_S, 2 * nh);
     double* o_phi = d_out;
     double* o_gx = d_out + ns;
     double* o_gy = d_out + 2 * ns;
     double* o_gz = d_out + 3 * ns;
+    // background of the n sources; for a sample particle, + 2 alpha / sqrt(pi)
+    // restores its image term (its own pair is left out of the real space)
     const double tsp = 2.0 / sqrt(M_PI);
-    ewald_init_kernel<<<grid_for(ns, 256), 256, 0, st>>>(ns, (double)ctx->n * M_PI / (alpha * alpha) + tsp * alpha,
-                                                         o_phi, o_gx, o_gy, o_gz);
+    ewald_init_kernel<<<grid_for(ns, 256), 256, 0, st>>>(
+        ns, (double)ctx->n * M_PI / (alpha * alpha) + (samples ? tsp * alpha : 0.0), o_phi, o_gx, o_gy, o_gz);
     EW_HIP(hipGetLastError());
     // S(h): source chunks of per_block particles x h blocks of 256.
     const int64_t n = ctx->n;
@@ -247,16 +285,16 @@ tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, doubl
     dim3 g1((unsigned)sf_chunks, (unsigned)((nh + 255) / 256));
     structure_factor_kernel<<<g1, 256, 0, st>>>(n, ctx->x, ctx->y, ctx->z, nh, d_hl, sf_per, d_S, d_S + nh);
     EW_HIP(hipGetLastError());
-    kspace_kernel<<<grid_for(ns, 64), 64, 0, st>>>(ns, d_samp, ctx->x, ctx->y, ctx->z, nh, d_hl, d_coef, d_S,
-                                                   d_S + nh, o_phi, o_gx, o_gy, o_gz);
+    kspace_kernel<<<grid_for(ns, 64), 64, 0, st>>>(ns, spx, spy, spz, nh, d_hl, d_coef, d_S, d_S + nh, o_phi, o_gx,
+                                                   o_gy, o_gz);
     EW_HIP(hipGetLastError());
     const int64_t sblocks = (ns + 63) / 64;
     int64_t rs_chunks = std::max<int64_t>(1, std::min<int64_t>((n + 4095) / 4096, (8192 + sblocks - 1) / sblocks));
     const int64_t rs_per = (n + rs_chunks - 1) / rs_chunks;
     rs_chunks = (n + rs_per - 1) / rs_per;
     dim3 g2((unsigned)rs_chunks, (unsigned)sblocks);
-    realspace_kernel<<<g2, 64, 0, st>>>(n, ctx->x, ctx->y, ctx->z, ns, d_samp, rs_per, alpha, ctx->p.hsoft, o_phi,
-                                        o_gx, o_gy, o_gz);
+    realspace_kernel<<<g2, 64, 0, st>>>(n, ctx->x, ctx->y, ctx->z, ns, spx, spy, spz, d_samp, rs_per, alpha,
+                                        ctx->p.hsoft, o_phi, o_gx, o_gy, o_gz);
     EW_HIP(hipGetLastError());
     EW_HIP(hipMemcpyAsync(phi, o_phi, sizeof(double) * ns, hipMemcpyDeviceToHost, st));
     EW_HIP(hipMemcpyAsync(gx, o_gx, sizeof(double) * ns, hipMemcpyDeviceToHost, st));

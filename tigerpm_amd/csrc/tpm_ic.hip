@@ -136,6 +136,8 @@ tpm_status tpm_ic_uniform_run(tpm_ctx* ctx, int64_t n, uint64_t seed) {
     TPM_HIP(ctx, hipGetLastError());
     TPM_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->chain_valid = false;
+    ctx->h2d_identity = true;
+    ctx->ref_valid = false;
     ctx->pm_valid = false;
     return TPM_OK;
 }
@@ -186,6 +188,8 @@ tpm_status tpm_ic_zeldovich_run(tpm_ctx* ctx, int n_side, uint64_t seed, double 
     TPM_HIP(ctx, hipStreamSynchronize(s));
     ctx->n = n;
     ctx->chain_valid = false;
+    ctx->h2d_identity = true;
+    ctx->ref_valid = false;
     ctx->pm_valid = false;  // phi was used as scratch
     return TPM_OK;
 }

@@ -133,9 +133,29 @@ struct tpm_ctx {
     int deposit_block = 0;  // chain cells per deposit block edge; 0 auto, -1 per-particle atomics
     int64_t chain_epoch = 0;  // counts chain-mesh builds (trees remember the one they follow)
 
+    // The order the caller sees (tpm_set_host_order).  With
+    // TPM_ORDER_REFERENCE each chain-mesh build also finds the order the
+    // reference partition leaves the caller's previous order in: h2d[r] is
+    // the device position of the caller's particle r (h2d_identity: the two
+    // orders agree, as after an upload), ref_perm[r] the caller's position of
+    // that particle before the build.
+    int host_order = TPM_ORDER_DEVICE;
+    bool h2d_identity = true;
+    bool ref_valid = false;        // ref_perm belongs to the current chain mesh
+    int64_t ref_cap = 0;
+    uint32_t *h2d = nullptr, *h2d2 = nullptr, *ref_perm = nullptr;
+    uint32_t *ref_cid = nullptr, *ref_cid2 = nullptr, *ref_idx2 = nullptr;
+    uint32_t *ref_posa = nullptr, *ref_posb = nullptr, *ref_inv = nullptr;
+    unsigned long long *ref_flag = nullptr, *ref_scan = nullptr;
+    void* ref_scan_tmp = nullptr;
+    size_t ref_scan_tmp_bytes = 0;
+    int4* ref_tab = nullptr;       // per axis coordinate and level: (lo, mid, hi, depth)
+    int ref_tab_C = 0;
+
     // Per-cell trees (tpm_tree.hip).
     int4* tree_pos = nullptr;        // tree order: x, y, z, chain-mesh index
     int4* tree_tmp = nullptr;        // partition ping-pong buffer
+    int32_t* tree_moved = nullptr;   // partition scratch of cells built in device memory
     int64_t tree_pos_cap = 0;
     int32_t* tree_cnt = nullptr;     // per cell: node counts [0, ncell), bucket counts [ncell, 2 ncell)
     int32_t* node_begin = nullptr;   // ncell + 1 exclusive offsets
@@ -170,7 +190,19 @@ struct tpm_ctx {
     // (tpm_reduce_counters)
     unsigned long long* counters = nullptr;
     double* dscratch = nullptr;
+    double* moment_buf = nullptr;  // TPM_MOMENT_BLOCKS x 8 partials + 8 results
     hipEvent_t ev[8] = {};
+
+    // Kicks restricted to a box of chain cells (tpm_kick_box) and the
+    // device budget that splits them (tpm_set_budget).
+    int64_t budget = 0;              // bytes of one kick's working set; 0 = no limit
+    int4* box_tiles = nullptr;       // the tiles of the box being kicked
+    int64_t box_tiles_cap = 0;
+    int32_t* box_count = nullptr;    // device scalar: tiles in box_tiles
+    int32_t* box_buckets = nullptr;  // tree walk: the buckets of the box
+    int64_t box_buckets_cap = 0;
+    std::vector<int32_t> h_cells;    // host copy of cell_begin ...
+    int64_t h_cells_epoch = -1;      // ... of this chain-mesh build
 
     // Launch timing log (tpm_profile).
     bool prof_on = false;
@@ -402,6 +434,7 @@ struct tpm_batch {
     // counter slots as in tpm_ctx (tpm_reduce_counters); words 8..15 of
     // slot 0: the persistent PP launch's tile queues
     unsigned long long* counters = nullptr;
+    double* moment_buf = nullptr;  // tpm_batch_moments: partials + results
     tpm_batch_tuning tune{16, 0, 0, 0};
     int persist_grid = 0;  // workgroups of the persistent PP launch
 
@@ -437,17 +470,29 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx);
 // Exclusive scan of in[0, n) into out[0, n] (out[n] = total); part holds
 // n / 2048 + 1 ints.
 tpm_status tpm_scan_i32(std::string& err, hipStream_t s, const int32_t* in, int64_t n, int32_t* part, int32_t* out);
+struct tpm_cell_box;
 // Adds W_sr (candidate pairs of the 27-cell stencil) of the context's chain
-// mesh to *dst (device).
-tpm_status tpm_candidate_pairs(tpm_ctx* ctx, unsigned long long* dst);
+// mesh to *dst (device); box: only the sinks of these cells.
+tpm_status tpm_candidate_pairs(tpm_ctx* ctx, unsigned long long* dst, const tpm_cell_box* box = nullptr);
 // Trees of the current chain mesh (no-op when they are current).
 tpm_status tpm_tree_ensure(tpm_ctx* ctx);
+// A box of chain cells [lo, hi) per axis (x, y, z), 0 <= lo < hi <= C.
+struct tpm_cell_box {
+    int lo[3], hi[3];
+};
 // Tree-walk short-range sum + mesh interpolation + kick (TPM_KICK_TREE).
+// box: only the sink buckets of these cells (nullptr: all).
 tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
-                         tpm_stats* stats);
+                         tpm_stats* stats, const tpm_cell_box* box = nullptr);
 tpm_status tpm_pm_solve(tpm_ctx* ctx);
+// The direct kernel over every tile, or (box != nullptr) over the box_tiles
+// tiles of the box's cells (ntiles_box of them).
 tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
-                       tpm_stats* stats);
+                       tpm_stats* stats, const tpm_cell_box* box = nullptr, int64_t ntiles_box = 0);
+// Kick of the sinks in box (nullptr: all) on the direct or tree path,
+// split recursively while its working set exceeds the context's budget.
+tpm_status tpm_kick_any(tpm_ctx* ctx, const tpm_cell_box* box, int min_rung, double scale, double t0,
+                        int first_call, unsigned flags, tpm_stats* stats);
 tpm_status tpm_fft_plan(tpm_ctx* ctx, int M);
 tpm_status tpm_fft_use(std::string& err, tpm_fft& f, int device, hipStream_t s, int M);
 tpm_status tpm_fft_r2c(std::string& err, tpm_fft& f, float* in, float2* out);
@@ -457,11 +502,22 @@ tpm_status tpm_fft_forward(tpm_ctx* ctx, float* in, float2* out);
 tpm_status tpm_fft_inverse(tpm_ctx* ctx, float2* in, float* out);
 tpm_status tpm_ensure_particles(tpm_ctx* ctx, int64_t n);
 tpm_status tpm_ensure_mesh(tpm_ctx* ctx, int M);
-tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, double alpha, double* phi, double* gx,
-                         double* gy, double* gz);
+// Samples are particle indices (samples != nullptr) or fixed32 positions
+// (pos[0..2]).
+tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, const uint32_t* const pos[3],
+                         double alpha, double* phi, double* gx, double* gy, double* gz);
 tpm_status tpm_ic_zeldovich_run(tpm_ctx* ctx, int n_side, uint64_t seed, double rms_disp, double spectral_index);
 tpm_status tpm_ic_uniform_run(tpm_ctx* ctx, int64_t n, uint64_t seed);
 tpm_status tpm_fill_u64(tpm_ctx* ctx, unsigned long long* p, int64_t n);
+
+// FP64 momentum / energy reductions (tpm_moments.hip).  part holds
+// 8 * TPM_MOMENT_BLOCKS doubles, out 8 (tpm_moments layout, on the device).
+constexpr int TPM_MOMENT_BLOCKS = 1024;
+int tpm_moment_chunks(int64_t n);
+tpm_status tpm_moments_enqueue(std::string& err, hipStream_t s, int64_t n, const float* vx, const float* vy,
+                               const float* vz, const float* gx, const float* gy, const float* gz, const float* pot,
+                               double m, double* part, double* out);
+tpm_status tpm_ctx_moments(tpm_ctx* ctx, tpm_moments* out);
 
 // Batched stages (tpm_chainmesh.hip, tpm_pm.hip, tpm_pp.hip).
 tpm_status tpm_batch_chainmesh(tpm_batch* bt, const tpm_batch_layout& L);

@@ -272,8 +272,10 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) 
 // W_sr = sum_c n_c * sum_{27 neighbours} n_c' (exact, 64-bit integer) over
 // the global cells [0, ncell); with a box table, cell c belongs to the last
 // box whose cell0 <= c.
+// A single box's sinks may be restricted to the cells [lo, hi).
 __global__ void candidate_pairs_kernel(int64_t ncell, int nbox, const tpm_box_dev* __restrict__ boxes, int C0,
-                                       const int32_t* __restrict__ cell_begin, unsigned long long* __restrict__ out) {
+                                       const int32_t* __restrict__ cell_begin, unsigned long long* __restrict__ out,
+                                       int3 lo = make_int3(0, 0, 0), int3 hi = make_int3(INT_MAX, INT_MAX, INT_MAX)) {
     unsigned long long acc = 0;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < ncell; c += (int64_t)gridDim.x * blockDim.x) {
         int C = C0;
@@ -285,6 +287,7 @@ __global__ void candidate_pairs_kernel(int64_t ncell, int nbox, const tpm_box_de
         }
         const int64_t l = c - base;
         const int cx = (int)(l % C), cy = (int)((l / C) % C), cz = (int)(l / ((int64_t)C * C));
+        if (cx < lo.x || cx >= hi.x || cy < lo.y || cy >= hi.y || cz < lo.z || cz >= hi.z) continue;
         const int64_t nc = cell_begin[c + 1] - cell_begin[c];
         int64_t s = 0;
         for (int nb = 0; nb < 27; nb++) {
@@ -297,6 +300,32 @@ __global__ void candidate_pairs_kernel(int64_t ncell, int nbox, const tpm_box_de
     }
     atomicAdd(out, acc);
 }
+
+// The tiles whose cell lies in [lo, hi), appended to out (one atomic per
+// wave).  The order of out does not matter: every tile is computed on its
+// own.
+__global__ void box_tiles_kernel(const int4* __restrict__ tiles, const int32_t* __restrict__ ntiles, int C, int3 lo,
+                                 int3 hi, int4* __restrict__ out, int32_t* __restrict__ count) {
+    const int nt = *ntiles;
+    const int lane = (int)lane_id();
+    for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x; t0 < nt; t0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = t0 + threadIdx.x;
+        int4 tile = make_int4(0, 0, 0, 0);
+        bool in = false;
+        if (t < nt) {
+            tile = tiles[t];
+            const int cx = tile.x % C, cy = (tile.x / C) % C, cz = tile.x / (C * C);
+            in = cx >= lo.x && cx < hi.x && cy >= lo.y && cy < hi.y && cz >= lo.z && cz < hi.z;
+        }
+        const unsigned long long m = __ballot(in);
+        int base = 0;
+        if (lane == 0 && m) base = atomicAdd(count, __popcll(m));
+        base = __shfl(base, 0);
+        if (in) out[base + (int)count_below(m)] = tile;
+    }
+}
+
+__global__ void set_i32_kernel(int32_t* p, int32_t v) { *p = v; }
 
 // Drift of every live particle by v * t0 of its box (fixed32, periodic):
 // raw += round(v t0 2^32) mod 2^32.
@@ -379,7 +408,7 @@ void tpm_box_fill(tpm_box_dev* b, const tpm_params& p, int min_rung, int first_c
 }
 
 tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
-                       tpm_stats* stats) {
+                       tpm_stats* stats, const tpm_cell_box* box, int64_t ntiles_box) {
     if (!ctx->chain_valid || ctx->chain_C != ctx->C || ctx->chain_tile != ctx->tile)
         return tpm_fail(ctx, TPM_ERR_STATE, "tpm_kick: chain mesh not built (call tpm_chainmesh_create)");
     const bool use_pm = !(flags & TPM_SKIP_PM);
@@ -410,6 +439,30 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     a.queue = nullptr;
     a.boxes = nullptr;
     a.flags = flags;
+    const int64_t ncell = (int64_t)ctx->C * ctx->C * ctx->C;
+    int64_t tiles_max = ctx->n / ctx->tile + ncell + 1;
+    int3 blo = make_int3(0, 0, 0), bhi = make_int3(INT_MAX, INT_MAX, INT_MAX);
+    if (box) {
+        // the box's tiles into box_tiles (ntiles_box of them, counted by the caller)
+        blo = make_int3(box->lo[0], box->lo[1], box->lo[2]);
+        bhi = make_int3(box->hi[0], box->hi[1], box->hi[2]);
+        // room for every tile of the context, so the append cannot overrun
+        if (tiles_max > ctx->box_tiles_cap) {
+            if (ctx->box_tiles) TPM_HIP(ctx, hipFree(ctx->box_tiles));
+            ctx->box_tiles = nullptr;
+            TPM_HIP(ctx, hipMalloc(&ctx->box_tiles, sizeof(int4) * tiles_max));
+            ctx->box_tiles_cap = tiles_max;
+        }
+        if (ntiles_box > tiles_max) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick_box: tile count");
+        if (!ctx->box_count) TPM_HIP(ctx, hipMalloc(&ctx->box_count, sizeof(int32_t)));
+        set_i32_kernel<<<1, 1, 0, s>>>(ctx->box_count, 0);
+        box_tiles_kernel<<<grid_for(tiles_max, 256), 256, 0, s>>>(ctx->tiles, ctx->ntiles, ctx->C, blo, bhi,
+                                                                  ctx->box_tiles, ctx->box_count);
+        TPM_HIP(ctx, hipGetLastError());
+        a.tiles = ctx->box_tiles;
+        a.ntiles = ctx->box_count;
+        tiles_max = ntiles_box;
+    }
     tpm_box_fill(&a.box0, ctx->p, min_rung, first_call, t0, scale);
     a.box0.cell0 = 0;
     a.box0.mesh0 = 0;
@@ -421,8 +474,6 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     }
     a.flags = flags & (TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK);
 
-    const int64_t ncell = (int64_t)ctx->C * ctx->C * ctx->C;
-    const int64_t tiles_max = ctx->n / ctx->tile + ncell + 1;
     const int64_t blocks = (tiles_max + 7) / 8 * 8;
     if (blocks > INT_MAX) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: too many tiles");
     if (!ctx->ev[0]) {
@@ -436,20 +487,20 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
         ctx->prof_pp.emplace_back(e0, e1);
     }
     TPM_HIP(ctx, hipEventRecord(e0, s));
-    if (ctx->n > 0) {
+    if (ctx->n > 0 && blocks > 0) {
         pp_launch<false>(ctx->tile, false, (unsigned)blocks, s, a, pme);
         TPM_HIP(ctx, hipGetLastError());
     }
     TPM_HIP(ctx, hipEventRecord(e1, s));
     if (stats) {
         candidate_pairs_kernel<<<grid_for(ncell, 256), 256, 0, s>>>(ncell, 0, nullptr, ctx->C, ctx->cell_begin,
-                                                                    ctx->counters + 2);
+                                                                    ctx->counters + 2, blo, bhi);
         TPM_HIP(ctx, hipGetLastError());
         std::vector<unsigned long long> hw(TPM_COUNTER_WORDS);
         int32_t nt = 0;
         TPM_HIP(ctx, hipMemcpyAsync(hw.data(), ctx->counters, sizeof(unsigned long long) * TPM_COUNTER_WORDS,
                                     hipMemcpyDeviceToHost, s));
-        TPM_HIP(ctx, hipMemcpyAsync(&nt, ctx->ntiles, sizeof(nt), hipMemcpyDeviceToHost, s));
+        TPM_HIP(ctx, hipMemcpyAsync(&nt, a.ntiles, sizeof(nt), hipMemcpyDeviceToHost, s));
         TPM_HIP(ctx, hipStreamSynchronize(s));
         float ms = 0.f;
         TPM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
@@ -464,10 +515,15 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     return TPM_OK;
 }
 
-tpm_status tpm_candidate_pairs(tpm_ctx* ctx, unsigned long long* dst) {
+tpm_status tpm_candidate_pairs(tpm_ctx* ctx, unsigned long long* dst, const tpm_cell_box* box) {
     const int64_t ncell = (int64_t)ctx->C * ctx->C * ctx->C;
+    int3 lo = make_int3(0, 0, 0), hi = make_int3(INT_MAX, INT_MAX, INT_MAX);
+    if (box) {
+        lo = make_int3(box->lo[0], box->lo[1], box->lo[2]);
+        hi = make_int3(box->hi[0], box->hi[1], box->hi[2]);
+    }
     candidate_pairs_kernel<<<grid_for(ncell, 256), 256, 0, ctx->stream>>>(ncell, 0, nullptr, ctx->C, ctx->cell_begin,
-                                                                          dst);
+                                                                          dst, lo, hi);
     TPM_HIP(ctx, hipGetLastError());
     return TPM_OK;
 }

@@ -110,25 +110,42 @@ struct PartFrame {
 //     into LDS and the splits move 16-bit local indices; each leaf writes its
 //     records (x, y, z, chain-mesh index) to `pos` once.
 //   PartGlobal: larger cells; int4 records move between pos and tmp.
+//   Both keep, for each split, the positions of the particles the
+//   two-pointer sweep moves (misplaced right ones below the split point in
+//   `ma`, misplaced left ones above it in `mb`), and map[] gives the device
+//   index of the cell's k-th particle in the order the trees start from
+//   (nullptr: the device order).
 struct PartLds {
     const uint32_t *xs, *ys, *zs;
     uint16_t *ia, *ib;
+    uint16_t *ma, *mb;
     int b0;
     int4* pos;
+    const uint32_t* map;
     using rec = int;
     __device__ rec load(int d, int i) const { return ((d & 1) ? ib : ia)[i - b0]; }
     __device__ uint32_t coord(rec k, int dim) const { return dim == 0 ? xs[k] : (dim == 1 ? ys[k] : zs[k]); }
     __device__ void store(int d, int i, rec k) const { ((d & 1) ? ib : ia)[i - b0] = (uint16_t)k; }
+    __device__ void set_a(int k, int i) const { ma[k] = (uint16_t)(i - b0); }
+    __device__ void set_b(int k, int i) const { mb[k] = (uint16_t)(i - b0); }
+    __device__ int get_a(int k) const { return b0 + ma[k]; }
+    __device__ int get_b(int k) const { return b0 + mb[k]; }
     __device__ void leaf(int d, int begin, int end) const {
         const uint16_t* buf = (d & 1) ? ib : ia;
         for (int i = begin + (int)lane_id(); i < end; i += 64) {
             const int k = buf[i - b0];
-            pos[i] = make_int4((int)xs[k], (int)ys[k], (int)zs[k], b0 + k);
+            pos[i] = make_int4((int)xs[k], (int)ys[k], (int)zs[k], map ? (int)map[b0 + k] : b0 + k);
         }
     }
 };
 struct PartGlobal {
     int4 *pos, *tmp;
+    int32_t* moved;  // this cell's [b0, e0) of a device buffer: a in the low half, b in the high half
+    int half;        // (e0 - b0) / 2
+    __device__ void set_a(int k, int i) const { moved[k] = i; }
+    __device__ void set_b(int k, int i) const { moved[half + k] = i; }
+    __device__ int get_a(int k) const { return moved[k]; }
+    __device__ int get_b(int k) const { return moved[half + k]; }
     using rec = int4;
     // (a pointer chosen by d & 1 would be looked up in scratch memory)
     __device__ rec load(int d, int i) const { return (d & 1) ? tmp[i] : pos[i]; }
@@ -192,28 +209,49 @@ __device__ __forceinline__ bool partition_cell(const S s, PartFrame* st, int c, 
             continue;
         }
         const uint32_t thr = tree_fixed(mid);
-        // particles_sort (particles.cpp:414-445): x < xmid goes left; here
-        // each side keeps its previous order (a stable partition)
+        // particles_sort (particles.cpp:414-445): x < xmid goes left.  Its
+        // two-pointer sweep in closed form: with m = begin + (particles
+        // left of the plane), the k-th right particle of [begin, m) counted
+        // upwards trades places with the k-th left particle of [m, end)
+        // counted downwards; the others stay.
         int nl = 0;
         for (int i0 = f.begin; i0 < f.end; i0 += 64) {
             const int i = i0 + lane;
             const bool left = i < f.end && s.coord(s.load(f.depth, i), dim) < thr;
             nl += __popcll(__ballot(left));
         }
-        int l = 0, r = 0;
+        const int m = f.begin + nl;
+        int na = 0, nb = 0;
+        for (int i0 = f.begin; i0 < f.end; i0 += 64) {
+            const int i = i0 + lane;
+            const bool valid = i < f.end;
+            const bool left = valid && s.coord(s.load(f.depth, i), dim) < thr;
+            const bool a_ = valid && i < m && !left, b_ = valid && i >= m && left;
+            const unsigned long long ma = __ballot(a_), mb = __ballot(b_);
+            if (a_) s.set_a(na + (int)count_below(ma), i);
+            if (b_) s.set_b(nb + (int)count_below(mb), i);
+            na += __popcll(ma);
+            nb += __popcll(mb);
+        }
+        __syncthreads();
+        int ka = 0, kb = 0;
         for (int i0 = f.begin; i0 < f.end; i0 += 64) {
             const int i = i0 + lane;
             const bool valid = i < f.end;
             typename S::rec v{};
             if (valid) v = s.load(f.depth, i);
             const bool left = valid && s.coord(v, dim) < thr;
-            const unsigned long long ml = __ballot(left), mr = __ballot(valid && !left);
-            if (valid)
-                s.store(f.depth + 1,
-                        left ? f.begin + l + (int)count_below(ml) : f.begin + nl + r + (int)count_below(mr), v);
-            l += __popcll(ml);
-            r += __popcll(mr);
+            const bool a_ = valid && i < m && !left, b_ = valid && i >= m && left;
+            const unsigned long long ma = __ballot(a_), mb = __ballot(b_);
+            int dst = i;
+            // a of upward rank k <-> b of downward rank k, i.e. upward rank nb - 1 - k
+            if (a_) dst = s.get_b(nb - 1 - (ka + (int)count_below(ma)));
+            if (b_) dst = s.get_a(nb - 1 - (kb + (int)count_below(mb)));
+            if (valid) s.store(f.depth + 1, dst, v);
+            ka += __popcll(ma);
+            kb += __popcll(mb);
         }
+        __syncthreads();
         if (sp + 2 > TREE_STACK) return false;  // unreachable below the depth limit
         if (lane == 0) {
             PartFrame& R = st[sp];
@@ -237,15 +275,22 @@ __device__ __forceinline__ bool partition_cell(const S s, PartFrame* st, int c, 
     return true;
 }
 
+// map (or nullptr): map[i] = device index of the particle the trees'
+// starting order holds at position i of its cell (the caller's order when it
+// differs from the device's), so the two-pointer sweeps start where the
+// reference's tree_create starts.  moved: n ints of scratch for the
+// partition of cells larger than TREE_LDS_CAP.
 __global__ __launch_bounds__(64) void tree_partition_kernel(int C, const int32_t* __restrict__ cell_begin,
                                                             const uint32_t* __restrict__ x,
                                                             const uint32_t* __restrict__ y,
-                                                            const uint32_t* __restrict__ z, int4* __restrict__ pos,
-                                                            int4* __restrict__ tmp, int32_t* __restrict__ cnt,
-                                                            int32_t* __restrict__ flags) {
+                                                            const uint32_t* __restrict__ z,
+                                                            const uint32_t* __restrict__ map, int4* __restrict__ pos,
+                                                            int4* __restrict__ tmp, int32_t* __restrict__ moved,
+                                                            int32_t* __restrict__ cnt, int32_t* __restrict__ flags) {
     __shared__ PartFrame st[TREE_STACK];
     __shared__ uint32_t xs[TREE_LDS_CAP], ys[TREE_LDS_CAP], zs[TREE_LDS_CAP];
     __shared__ uint16_t ia[TREE_LDS_CAP], ib[TREE_LDS_CAP];
+    __shared__ uint16_t ma[TREE_LDS_CAP / 2], mb[TREE_LDS_CAP / 2];
     const int c = blockIdx.x;
     const int ncell = C * C * C;
     const int lane = (int)lane_id();
@@ -254,15 +299,19 @@ __global__ __launch_bounds__(64) void tree_partition_kernel(int C, const int32_t
     bool ok;
     if (e0 - b0 <= TREE_LDS_CAP) {
         for (int k = lane; k < e0 - b0; k += 64) {
-            xs[k] = x[b0 + k];
-            ys[k] = y[b0 + k];
-            zs[k] = z[b0 + k];
+            const int dk = map ? (int)map[b0 + k] : b0 + k;
+            xs[k] = x[dk];
+            ys[k] = y[dk];
+            zs[k] = z[dk];
             ia[k] = (uint16_t)k;
         }
-        ok = partition_cell(PartLds{xs, ys, zs, ia, ib, b0, pos}, st, c, C, b0, e0, nodes, buckets);
+        ok = partition_cell(PartLds{xs, ys, zs, ia, ib, ma, mb, b0, pos, map}, st, c, C, b0, e0, nodes, buckets);
     } else {
-        for (int i = b0 + lane; i < e0; i += 64) pos[i] = make_int4((int)x[i], (int)y[i], (int)z[i], i);
-        ok = partition_cell(PartGlobal{pos, tmp}, st, c, C, b0, e0, nodes, buckets);
+        for (int i = b0 + lane; i < e0; i += 64) {
+            const int di = map ? (int)map[i] : i;
+            pos[i] = make_int4((int)x[di], (int)y[di], (int)z[di], di);
+        }
+        ok = partition_cell(PartGlobal{pos, tmp, moved + b0, (e0 - b0) / 2}, st, c, C, b0, e0, nodes, buckets);
     }
     if (lane == 0) {
         cnt[c] = nodes;
@@ -593,6 +642,7 @@ struct WalkArgs {
     tpm_kick_out o;
     tpm_box_dev B;
     int nbuckets, per_xcd;
+    const int32_t* __restrict__ blist;   // buckets to walk (nullptr: all)
     int n;                               // particles
     float rcut, theta2inv, inv2rs, tsp;  // box units
     float e2;                            // -log2(e) inv2rs^2: exp(-r0^2) = exp2(r^2 e2)
@@ -1054,8 +1104,9 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
     const int wg = blockIdx.x;
     const int k = wg >> 3;
     if (k >= a.per_xcd) return;
-    const int bi = (wg & 7) * a.per_xcd + k;
-    if (bi >= a.nbuckets) return;
+    const int item = (wg & 7) * a.per_xcd + k;
+    if (item >= a.nbuckets) return;
+    const int bi = a.blist ? a.blist[item] : item;
     const tpm_tree_bucket bk = a.buckets[bi];
     const int ns = bk.end - bk.begin;
     const int K = (ns + 31) >> 5;          // sinks per lane
@@ -1073,9 +1124,24 @@ __global__ __launch_bounds__(64) void tree_walk_kernel(WalkArgs a) {
     }
 }
 
-__global__ void tree_order_kernel(int64_t n, const int4* __restrict__ pos, int32_t* __restrict__ order) {
+// order[i] = index of tree position i's particle: its device index, or
+// through d2h its place in the caller's order.
+__global__ void tree_order_kernel(int64_t n, const int4* __restrict__ pos, const int32_t* __restrict__ d2h,
+                                  int32_t* __restrict__ order) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        order[i] = pos[i].w;
+        order[i] = d2h ? d2h[pos[i].w] : pos[i].w;
+}
+
+__global__ void tree_invert_kernel(int64_t n, const uint32_t* __restrict__ h2d, int32_t* __restrict__ d2h) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        d2h[h2d[r]] = (int32_t)r;
+}
+
+// The caller's order becomes the trees' order: h2d[r] = device index of
+// tree position r.
+__global__ void tree_host_order_kernel(int64_t n, const int4* __restrict__ pos, uint32_t* __restrict__ h2d) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        h2d[r] = (uint32_t)pos[r].w;
 }
 
 __global__ void tree_fill_i32_kernel(int32_t* p, int n, int32_t v) {
@@ -1104,14 +1170,17 @@ tpm_status tpm_tree_ensure(tpm_ctx* ctx) {
     const int C = ctx->C;
     const int64_t ncell = (int64_t)C * C * C;
     const int64_t n = ctx->n;
-    if (n > ctx->tree_pos_cap || !ctx->tree_pos || !ctx->tree_tmp) {
-        int64_t c1 = 0, c2 = 0;
+    if (n > ctx->tree_pos_cap || !ctx->tree_pos || !ctx->tree_tmp || !ctx->tree_moved) {
+        int64_t c1 = 0, c2 = 0, c3 = 0;
         if (ctx->tree_pos) TPM_HIP(ctx, hipFree(ctx->tree_pos));
         if (ctx->tree_tmp) TPM_HIP(ctx, hipFree(ctx->tree_tmp));
+        if (ctx->tree_moved) TPM_HIP(ctx, hipFree(ctx->tree_moved));
         ctx->tree_pos = ctx->tree_tmp = nullptr;
+        ctx->tree_moved = nullptr;
         ctx->tree_pos_cap = 0;
         TPM_TRY(grow(ctx, &ctx->tree_pos, &c1, n));
         TPM_TRY(grow(ctx, &ctx->tree_tmp, &c2, n));
+        TPM_TRY(grow(ctx, &ctx->tree_moved, &c3, n));
         ctx->tree_pos_cap = n;
     }
     if (ncell > ctx->tree_cells_cap || !ctx->tree_cnt) {
@@ -1130,8 +1199,12 @@ tpm_status tpm_tree_ensure(tpm_ctx* ctx) {
         ctx->tree_cells_cap = ncell;
     }
     tree_fill_i32_kernel<<<1, 64, 0, s>>>(ctx->tree_flags, 4, 0);
-    tree_partition_kernel<<<(unsigned)ncell, 64, 0, s>>>(C, ctx->cell_begin, ctx->x, ctx->y, ctx->z, ctx->tree_pos,
-                                                          ctx->tree_tmp, ctx->tree_cnt, ctx->tree_flags);
+    // the trees start from the caller's order (tree_create partitions the
+    // caller's arrays in place, tree.cpp:101-107)
+    const uint32_t* map = ctx->host_order == TPM_ORDER_REFERENCE && !ctx->h2d_identity ? ctx->h2d : nullptr;
+    tree_partition_kernel<<<(unsigned)ncell, 64, 0, s>>>(C, ctx->cell_begin, ctx->x, ctx->y, ctx->z, map,
+                                                          ctx->tree_pos, ctx->tree_tmp, ctx->tree_moved, ctx->tree_cnt,
+                                                          ctx->tree_flags);
     TPM_HIP(ctx, hipGetLastError());
     TPM_TRY(tpm_scan_i32(ctx->err, s, ctx->tree_cnt, ncell, ctx->tree_part, ctx->node_begin));
     TPM_TRY(tpm_scan_i32(ctx->err, s, ctx->tree_cnt + ncell, ncell, ctx->tree_part, ctx->bucket_begin));
@@ -1159,7 +1232,7 @@ tpm_status tpm_tree_ensure(tpm_ctx* ctx) {
 }
 
 tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
-                         tpm_stats* stats) {
+                         tpm_stats* stats, const tpm_cell_box* box) {
     const bool use_pm = !(flags & TPM_SKIP_PM);
     if (!ctx->chain_valid || ctx->chain_C != ctx->C)
         return tpm_fail(ctx, TPM_ERR_STATE, "tpm_kick: chain mesh not built (call tpm_chainmesh_create)");
@@ -1183,8 +1256,37 @@ tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, in
     a.B.cell0 = 0;
     a.B.mesh0 = 0;
     a.nbuckets = (int)ctx->n_buckets;
+    a.blist = nullptr;
+    if (box) {
+        // the buckets of the box's cells, from the bucket offsets per cell
+        const int C = ctx->C;
+        const int64_t ncell = (int64_t)C * C * C;
+        std::vector<int32_t> bb((size_t)ncell + 1), list;
+        TPM_HIP(ctx, hipMemcpyAsync(bb.data(), ctx->bucket_begin, sizeof(int32_t) * (ncell + 1),
+                                    hipMemcpyDeviceToHost, s));
+        TPM_HIP(ctx, hipStreamSynchronize(s));
+        for (int z = box->lo[2]; z < box->hi[2]; z++)
+            for (int y = box->lo[1]; y < box->hi[1]; y++)
+                for (int x = box->lo[0]; x < box->hi[0]; x++) {
+                    const int64_t c = ((int64_t)z * C + y) * C + x;
+                    for (int32_t b = bb[c]; b < bb[c + 1]; b++) list.push_back(b);
+                }
+        if ((int64_t)list.size() > ctx->box_buckets_cap || !ctx->box_buckets) {
+            if (ctx->box_buckets) TPM_HIP(ctx, hipFree(ctx->box_buckets));
+            ctx->box_buckets = nullptr;
+            TPM_HIP(ctx, hipMalloc(&ctx->box_buckets, sizeof(int32_t) * std::max<size_t>(list.size(), 1)));
+            ctx->box_buckets_cap = (int64_t)std::max<size_t>(list.size(), 1);
+        }
+        if (!list.empty())
+            TPM_HIP(ctx, hipMemcpyAsync(ctx->box_buckets, list.data(), sizeof(int32_t) * list.size(),
+                                        hipMemcpyHostToDevice, s));
+        // the host list must outlive the copy
+        TPM_HIP(ctx, hipStreamSynchronize(s));
+        a.blist = ctx->box_buckets;
+        a.nbuckets = (int)list.size();
+    }
     a.n = (int)ctx->n;
-    a.per_xcd = (int)((ctx->n_buckets + 7) / 8);
+    a.per_xcd = (a.nbuckets + 7) / 8;
     // treepm_params (kick_treepm.cu:826-853): rcut, theta = 0.4, 1 / (2 rs)
     a.rcut = (float)ctx->p.rcut;
     const float theta = 0.4f;
@@ -1214,7 +1316,7 @@ tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, in
     TPM_HIP(ctx, hipMemcpyAsync(fl, ctx->tree_flags, sizeof(fl), hipMemcpyDeviceToHost, s));
     std::vector<unsigned long long> hw(TPM_COUNTER_WORDS, 0ull);
     if (stats) {
-        TPM_TRY(tpm_candidate_pairs(ctx, ctx->counters + 2));
+        TPM_TRY(tpm_candidate_pairs(ctx, ctx->counters + 2, box));
         TPM_HIP(ctx, hipMemcpyAsync(hw.data(), ctx->counters, sizeof(unsigned long long) * TPM_COUNTER_WORDS,
                                     hipMemcpyDeviceToHost, s));
     }
@@ -1231,7 +1333,7 @@ tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, in
         stats->candidate_pairs = (double)hc[2];
         stats->pc_interactions = (double)hc[3];
         stats->ms_pp = ms;
-        stats->n_tiles = ctx->n_buckets;
+        stats->n_tiles = a.nbuckets;
     }
     return TPM_OK;
 }
@@ -1268,18 +1370,42 @@ extern "C" tpm_status tpm_tree_download(tpm_ctx* ctx, int64_t* node_begin, int64
     if (buckets && ctx->n_buckets)
         TPM_HIP(ctx, hipMemcpyAsync(buckets, ctx->buckets, sizeof(tpm_tree_bucket) * ctx->n_buckets,
                                     hipMemcpyDeviceToHost, s));
-    int32_t* dorder = nullptr;
+    int32_t *dorder = nullptr, *d2h = nullptr;
     if (order && ctx->n) {
+        const bool mapped = ctx->host_order == TPM_ORDER_REFERENCE && !ctx->h2d_identity;
         TPM_HIP(ctx, hipMallocAsync((void**)&dorder, sizeof(int32_t) * ctx->n, s));
-        tree_order_kernel<<<grid_for(ctx->n, 256), 256, 0, s>>>(ctx->n, ctx->tree_pos, dorder);
+        if (mapped) {
+            TPM_HIP(ctx, hipMallocAsync((void**)&d2h, sizeof(int32_t) * ctx->n, s));
+            tree_invert_kernel<<<grid_for(ctx->n, 256), 256, 0, s>>>(ctx->n, ctx->h2d, d2h);
+        }
+        tree_order_kernel<<<grid_for(ctx->n, 256), 256, 0, s>>>(ctx->n, ctx->tree_pos, d2h, dorder);
         TPM_HIP(ctx, hipGetLastError());
         TPM_HIP(ctx, hipMemcpyAsync(order, dorder, sizeof(int32_t) * ctx->n, hipMemcpyDeviceToHost, s));
         TPM_HIP(ctx, hipFreeAsync(dorder, s));
+        if (d2h) TPM_HIP(ctx, hipFreeAsync(d2h, s));
     }
     TPM_HIP(ctx, hipStreamSynchronize(s));
     for (int64_t c = 0; c <= ncell; c++) {
         if (node_begin) node_begin[c] = nb[c];
         if (bucket_begin) bucket_begin[c] = bb[c];
     }
+    return TPM_OK;
+}
+
+extern "C" tpm_status tpm_tree_sort_particles(tpm_ctx* ctx) {
+    if (!ctx) return TPM_ERR_INVALID;
+    if (ctx->host_order != TPM_ORDER_REFERENCE)
+        return tpm_fail(ctx, TPM_ERR_STATE, "tpm_tree_sort_particles: needs TPM_ORDER_REFERENCE");
+    (void)hipSetDevice(ctx->device);
+    TPM_TRY(tpm_tree_ensure(ctx));
+    const int64_t n = ctx->n;
+    if (n > 0) {
+        if (!ctx->h2d || ctx->ref_cap < n)
+            return tpm_fail(ctx, TPM_ERR_STATE, "tpm_tree_sort_particles: build the chain mesh first");
+        tree_host_order_kernel<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, ctx->tree_pos, ctx->h2d);
+        TPM_HIP(ctx, hipGetLastError());
+    }
+    ctx->h2d_identity = false;
+    ctx->ref_valid = false;
     return TPM_OK;
 }

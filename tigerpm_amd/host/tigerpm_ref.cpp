@@ -11,7 +11,6 @@
 #include <cstring>
 #include <random>
 #include <stdexcept>
-#include <unordered_map>
 
 #include "tigerpm_hip.h"
 #include "tigerpm_ref.hpp"
@@ -33,6 +32,21 @@ bool g_phi_valid = false;
 int g_long_type = GRAVITY_LONG_PME;
 bool g_pm_computed = false;
 kick_report g_report{};
+// Trees of the current chain mesh on the host (tree_create): the device's
+// forest, downloaded once per chain-mesh build.
+struct host_forest {
+    bool valid = false;
+    std::vector<int64_t> node_begin, bucket_begin;
+    std::vector<tpm_tree_node> nodes;
+    std::vector<tpm_tree_bucket> buckets;
+    std::vector<int32_t> order;     // tree position -> host index when built
+    std::vector<char> cell_sorted;  // the host arrays of this cell are in tree order
+};
+host_forest g_forest;
+// tree_create reordered host cells: the device's view of the host order
+// must follow (tpm_tree_sort_particles) before the next download
+bool g_tree_pending = false;
+bool g_chain_valid = false;  // the device's chain mesh is of the current host particles
 
 void check(tpm_status s, const char* what) {
     if (s != TPM_OK) {
@@ -43,7 +57,12 @@ void check(tpm_status s, const char* what) {
 }
 
 tpm_ctx* ctx() {
-    if (!g_ctx) check(tpm_ctx_create(0, &g_ctx), "tpm_ctx_create");
+    if (!g_ctx) {
+        check(tpm_ctx_create(0, &g_ctx), "tpm_ctx_create");
+        // the host arrays follow the reference partition's order: the device
+        // keeps its own order and maps on download
+        check(tpm_set_host_order(g_ctx, TPM_ORDER_REFERENCE), "tpm_set_host_order");
+    }
     return g_ctx;
 }
 
@@ -103,10 +122,18 @@ void upload() {
                                reinterpret_cast<const int8_t*>(particles_R.data())),
           "tpm_particles_upload");
     g_pm_computed = false;
+    g_tree_pending = false;
+    g_forest.valid = false;
+    g_chain_valid = false;
 }
 
 // Positions, velocities and rungs back from the device (after a sort or IC).
 void download_state(bool forces) {
+    if (g_tree_pending) {
+        // every cell into tree order, as tree_create leaves the ones it met
+        check(tpm_tree_sort_particles(ctx()), "tpm_tree_sort_particles");
+        g_tree_pending = false;
+    }
     check(tpm_particles_download(ctx(), reinterpret_cast<uint32_t*>(particles_X[0].data()),
                                  reinterpret_cast<uint32_t*>(particles_X[1].data()),
                                  reinterpret_cast<uint32_t*>(particles_X[2].data()), particles_U[0].data(),
@@ -205,6 +232,8 @@ void particles_random_init() {
     check(tpm_ic_uniform(ctx(), n, o.seed), "tpm_ic_uniform");
     download_state(false);
     g_pm_computed = false;
+    g_chain_valid = false;
+    g_forest.valid = false;
 }
 
 void particles_domain_sort() {}
@@ -242,6 +271,8 @@ void initialize() {
     check(tpm_ic_zeldovich(ctx(), o.parts_dim, o.seed, o.rms_disp, -2.0), "tpm_ic_zeldovich");
     download_state(false);
     g_pm_computed = false;
+    g_chain_valid = false;
+    g_forest.valid = false;
 }
 
 // ---------------------------------------------------------------- chainmesh
@@ -249,6 +280,8 @@ void chainmesh_create() {
     push_params(g_long_type);
     upload();
     check(tpm_chainmesh_create(ctx()), "tpm_chainmesh_create");
+    g_forest.valid = false;
+    g_chain_valid = true;
     download_state(false);  // the reference sorts the host arrays in place
     const int C = get_options().chain_dim;
     g_cells.assign((size_t)C * C * C + 1, 0);
@@ -363,43 +396,176 @@ std::pair<float, std::array<float, NDIM>> gravity_long_force_at(const std::array
 }
 
 // ------------------------------------------------------------------- kicks
-static void device_kick(int min_rung, double scale, double t0, bool first_call, unsigned flags) {
+// The device holds the current chain mesh and potential; rebuilds them when
+// the particles were uploaded since (tpm_kick reports TPM_ERR_STATE).
+static void device_ready() {
     push_params(g_long_type);
-    if (g_cells.empty()) chainmesh_create();
+    if (!g_chain_valid) chainmesh_create();
     if (!g_pm_computed) {
         check(tpm_gravity_long_compute(ctx()), "tpm_gravity_long_compute");
         g_pm_computed = true;
     }
-    tpm_stats st{};
-    tpm_status s = tpm_kick(ctx(), min_rung, scale, t0, first_call ? 1 : 0, flags, &st);
-    if (s == TPM_ERR_STATE) {  // particles changed on the host since the sort
-        chainmesh_create();
-        check(tpm_gravity_long_compute(ctx()), "tpm_gravity_long_compute");
-        g_pm_computed = true;
-        s = tpm_kick(ctx(), min_rung, scale, t0, first_call ? 1 : 0, flags, &st);
-    }
-    check(s, "tpm_kick");
+}
+
+static void record(const tpm_stats& st) {
     g_report.ms_pp = st.ms_pp;
     g_report.candidate_pairs = st.candidate_pairs;
     g_report.executed_pairs = st.executed_pairs;
     g_report.pc_interactions = st.pc_interactions;
+}
+
+// The kick of the sinks in box (nullptr: all cells) with the given flags.
+static void device_kick(const range<int>* box, int min_rung, double scale, double t0, bool first_call,
+                        unsigned flags) {
+    device_ready();
+    tpm_stats st{};
+    int32_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (box)
+        for (int d = 0; d < NDIM; d++) {
+            lo[d] = box->begin[d];
+            hi[d] = box->end[d];
+        }
+    auto run = [&]() {
+        return box ? tpm_kick_box(ctx(), lo, hi, min_rung, scale, t0, first_call ? 1 : 0, flags, &st)
+                   : tpm_kick(ctx(), min_rung, scale, t0, first_call ? 1 : 0, flags, &st);
+    };
+    tpm_status s = run();
+    if (s == TPM_ERR_STATE) {  // particles changed on the host since the sort
+        chainmesh_create();
+        check(tpm_gravity_long_compute(ctx()), "tpm_gravity_long_compute");
+        g_pm_computed = true;
+        s = run();
+    }
+    check(s, box ? "tpm_kick_box" : "tpm_kick");
+    record(st);
     download_state(true);
 }
 
-// kick_treepm_begin (kick_treepm.cpp:10-54 + kick_treepm.cu:761-1089).
-// --short=tree: the reference's algorithm -- per-cell trees built and walked
-// on the device (multipoles for far nodes).  --short=direct (default): the
-// erfc-split pair force over every pair of the 27 neighbour cells.
-void kick_treepm_begin(int min_rung, double scale, double t0, bool first_call) {
-    device_kick(min_rung, scale, t0, first_call, get_options().short_range == "tree" ? TPM_KICK_TREE : 0u);
+static unsigned short_range_flags() { return get_options().short_range == "tree" ? TPM_KICK_TREE : 0u; }
+
+// The device's forest on the host, once per chain-mesh build.
+static host_forest& forest() {
+    if (g_forest.valid) return g_forest;
+    device_ready();
+    check(tpm_tree_build(ctx()), "tpm_tree_build");
+    int64_t nn = 0, nb = 0;
+    check(tpm_tree_info(ctx(), &nn, &nb), "tpm_tree_info");
+    const int C = get_options().chain_dim;
+    const size_t ncell = (size_t)C * C * C;
+    host_forest& f = g_forest;
+    f.node_begin.assign(ncell + 1, 0);
+    f.bucket_begin.assign(ncell + 1, 0);
+    f.nodes.resize((size_t)nn);
+    f.buckets.resize((size_t)nb);
+    f.order.resize((size_t)particles_size());
+    check(tpm_tree_download(ctx(), f.node_begin.data(), f.bucket_begin.data(), f.nodes.data(), f.buckets.data(),
+                            f.order.data()),
+          "tpm_tree_download");
+    f.cell_sorted.assign(ncell, 0);
+    f.valid = true;
+    return f;
 }
+
+static int64_t wrap_cell(const std::array<int, NDIM>& i) {
+    const int C = get_options().chain_dim;
+    const int ix = ((i[0] % C) + C) % C, iy = ((i[1] % C) + C) % C, iz = ((i[2] % C) + C) % C;
+    return ((int64_t)iz * C + iy) * C + ix;
+}
+
+// tree_create (tree.cpp:216-245)
+std::pair<tree, std::vector<sink_bucket>> tree_create(const std::array<int, NDIM>& cell_index, chaincell cell) {
+    host_forest& f = forest();
+    const int64_t c = wrap_cell(cell_index);
+    const chaincell mine = chainmesh_get(cell_index);
+    if (mine.pbegin != cell.pbegin || mine.pend != cell.pend)
+        throw std::runtime_error("tree_create: the cell's range is not the current chain mesh's");
+    std::pair<tree, std::vector<sink_bucket>> rc;
+    for (int64_t k = f.node_begin[c]; k < f.node_begin[c + 1]; k++) {
+        const tpm_tree_node& d = f.nodes[(size_t)k];
+        tree_node t;
+        for (int q = 0; q < NDIM; q++) t.x[q] = fixed32::from_raw(d.x[q]);
+        t.radius = d.radius;
+        t.children = {d.child[0], d.child[1]};
+        t.src_begin = t.snk_begin = d.begin;
+        t.src_end = t.snk_end = d.end;
+        for (int q = 0; q < MULTIPOLE_SIZE; q++) t.multi[q] = d.m[q];
+        rc.first.nodes.push_back(t);
+    }
+    for (int64_t k = f.bucket_begin[c]; k < f.bucket_begin[c + 1]; k++) {
+        const tpm_tree_bucket& d = f.buckets[(size_t)k];
+        sink_bucket b;
+        b.src_begin = b.snk_begin = d.begin;
+        b.src_end = b.snk_end = d.end;
+        b.radius = d.radius;
+        for (int q = 0; q < NDIM; q++) b.x[q] = fixed32::from_raw(d.x[q]);
+        rc.second.push_back(b);
+    }
+    if (!f.cell_sorted[(size_t)c]) {
+        // the cell's particles into tree order: what particles_sort leaves
+        // in X, U and R (particles.cpp:426-434)
+        const int b = cell.pbegin, n = cell.pend - cell.pbegin;
+        std::vector<fixed32> x(n);
+        std::vector<float> u(n);
+        std::vector<char> r(n);
+        for (int d = 0; d < NDIM; d++) {
+            for (int k = 0; k < n; k++) x[k] = particles_X[d][f.order[(size_t)(b + k)]];
+            std::copy(x.begin(), x.end(), particles_X[d].begin() + b);
+            for (int k = 0; k < n; k++) u[k] = particles_U[d][f.order[(size_t)(b + k)]];
+            std::copy(u.begin(), u.end(), particles_U[d].begin() + b);
+        }
+        for (int k = 0; k < n; k++) r[k] = particles_R[f.order[(size_t)(b + k)]];
+        std::copy(r.begin(), r.end(), particles_R.begin() + b);
+        f.cell_sorted[(size_t)c] = 1;
+        g_tree_pending = true;
+    }
+    return rc;
+}
+
+// kick_treepm_begin (kick_treepm.cpp:10-54 + kick_treepm.cu:761-1089): the
+// trees of every cell -- on the device, the whole forest at once, with the
+// particles left in tree order as tree_create leaves them -- then the kick
+// of the interior box.
+void kick_treepm_begin(int min_rung, double scale, double t0, bool first_call) {
+    device_ready();
+    check(tpm_tree_build(ctx()), "tpm_tree_build");
+    g_tree_pending = true;
+    device_kick(nullptr, min_rung, scale, t0, first_call, short_range_flags());
+}
+
+// kick_treepm (kick_treepm.hpp:14)
+void kick_treepm(std::vector<tree> trees, std::vector<std::vector<sink_bucket>> buckets, range<int> box, int min_rung,
+                 double scale, double t0, bool first_call) {
+    const range<int> bigbox = box.pad(1);
+    if ((int)trees.size() != bigbox.volume() || (int)buckets.size() != box.volume())
+        throw std::runtime_error("kick_treepm: need the trees of box.pad(1) and the sink buckets of box");
+    host_forest& f = forest();
+    std::array<int, NDIM> i;
+    for (i[0] = bigbox.begin[0]; i[0] < bigbox.end[0]; i[0]++)
+        for (i[1] = bigbox.begin[1]; i[1] < bigbox.end[1]; i[1]++)
+            for (i[2] = bigbox.begin[2]; i[2] < bigbox.end[2]; i[2]++) {
+                const int64_t c = wrap_cell(i);
+                if ((int64_t)trees[bigbox.index(i)].size() != f.node_begin[c + 1] - f.node_begin[c] ||
+                    (box.contains(i) &&
+                     (int64_t)buckets[box.index(i)].size() != f.bucket_begin[c + 1] - f.bucket_begin[c]))
+                    throw std::runtime_error("kick_treepm: the trees are not tree_create's of the current chain mesh");
+            }
+    device_kick(&box, min_rung, scale, t0, first_call, short_range_flags());
+}
+
 void kick_treepm_end() {}  // no ghost particles to drop on one device
-// kick_pme_begin (kick_pme.cpp, kick_pme.cu:128-357): the same kick with
-// kick_pme's pair law (no cutoff, plain softening inside h, no phi0).
+
+// kick_pme (kick_pme.hpp:15, kick_pme.cu:390-621): kick_pme's pair law (no
+// cutoff, plain softening inside h, no phi0) over the sinks of box.
+void kick_pme(range<int> box, int min_rung, double scale, double t0, bool first_call) {
+    device_kick(&box, min_rung, scale, t0, first_call, TPM_KICK_PME);
+}
+// kick_pme_begin (kick_pme.cpp): kick_pme of the interior box.
 void kick_pme_begin(int min_rung, double scale, double t0, bool first_call) {
-    device_kick(min_rung, scale, t0, first_call, TPM_KICK_PME);
+    device_kick(nullptr, min_rung, scale, t0, first_call, TPM_KICK_PME);
 }
 void kick_pme_end() {}
+
+void set_device_budget(int64_t bytes) { check(tpm_set_budget(ctx(), bytes), "tpm_set_budget"); }
 
 // kick_pm (kick_pm.cpp:9-39): PM force at every particle on the CPU.
 void kick_pm() {
@@ -432,31 +598,26 @@ static std::vector<int64_t> sample_indices(int nsamples) {
     return s;
 }
 
+// gravity_short_ewald_call_kernel (gravity_short.cu:10-91): the periodic
+// potential and force of every resident particle at each sink position
+// (gravity_ewald_kernel, :93-201: every source counts, softened inside
+// hsoft, so a sink on a particle gets its -35/16 h^-1).  FP64 Ewald sum
+// (tpm_ewald_at); unit masses, as the reference's kernel.
 std::pair<std::vector<double>, std::array<std::vector<double>, NDIM>> gravity_short_ewald_call_kernel(
     const std::vector<fixed32>& sinkx, const std::vector<fixed32>& sinky, const std::vector<fixed32>& sinkz) {
-    // The Ewald check evaluates at particles (excluding the particle itself):
-    // find each sink among the resident particles.
-    std::unordered_multimap<uint64_t, int64_t> where;
-    for (int i = 0; i < particles_size(); i++)
-        where.emplace(((uint64_t)particles_X[0][i].raw() << 32) | particles_X[1][i].raw(), i);
-    std::vector<int64_t> idx;
-    for (size_t s = 0; s < sinkx.size(); s++) {
-        auto r = where.equal_range(((uint64_t)sinkx[s].raw() << 32) | sinky[s].raw());
-        int64_t found = -1;
-        for (auto it = r.first; it != r.second; ++it)
-            if (particles_X[2][it->second] == sinkz[s]) found = it->second;
-        if (found < 0) throw std::runtime_error("gravity_short_ewald_call_kernel: sink is not a particle");
-        idx.push_back(found);
-    }
+    const size_t ns = sinkx.size();
+    if (sinky.size() != ns || sinkz.size() != ns)
+        throw std::runtime_error("gravity_short_ewald_call_kernel: sink arrays of different lengths");
     push_params(g_long_type);
     upload();
     std::pair<std::vector<double>, std::array<std::vector<double>, NDIM>> rc;
-    const size_t ns = idx.size();
     rc.first.resize(ns);
     for (int d = 0; d < NDIM; d++) rc.second[d].resize(ns);
-    check(tpm_ewald(ctx(), (int64_t)ns, idx.data(), 0.0, rc.first.data(), rc.second[0].data(), rc.second[1].data(),
-                    rc.second[2].data()),
-          "tpm_ewald");
+    static_assert(sizeof(fixed32) == sizeof(uint32_t), "layout");
+    check(tpm_ewald_at(ctx(), (int64_t)ns, reinterpret_cast<const uint32_t*>(sinkx.data()),
+                       reinterpret_cast<const uint32_t*>(sinky.data()), reinterpret_cast<const uint32_t*>(sinkz.data()),
+                       0.0, rc.first.data(), rc.second[0].data(), rc.second[1].data(), rc.second[2].data()),
+          "tpm_ewald_at");
     return rc;
 }
 

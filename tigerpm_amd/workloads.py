@@ -17,9 +17,13 @@ from ._lib import TpmParams
 WORKLOADS = {
     # n_side^3 particles, mesh^3 PM mesh, assignment order, initial conditions,
     # rms displacement [cells], Ewald samples, seed, interlaced deposit
-    "uniform64_cic": dict(n_side=64, mesh=64, order=2, ic="uniform", rms=0.0, samples=1024, seed=1, interlace=1),
-    "zeldovich256": dict(n_side=256, mesh=256, order=3, ic="zeldovich", rms=0.5, samples=4096, seed=2,
-                         interlace=1),
+    "uniform64_cic": dict(n_side=64, mesh=64, order=2, ic="uniform", rms=0.0, samples=1024, seed=1, interlace=0),
+    "uniform64_cic_interlaced": dict(n_side=64, mesh=64, order=2, ic="uniform", rms=0.0, samples=1024, seed=1,
+                                     interlace=1),
+    "zeldovich256_tsc": dict(n_side=256, mesh=256, order=3, ic="zeldovich", rms=0.5, samples=4096, seed=2,
+                             interlace=0),
+    "zeldovich256_tsc_interlaced": dict(n_side=256, mesh=256, order=3, ic="zeldovich", rms=0.5, samples=4096,
+                                        seed=2, interlace=1),
     "zeldovich512_tsc": dict(n_side=512, mesh=512, order=3, ic="zeldovich", rms=2.0, samples=8192, seed=3,
                              interlace=0),
 }
@@ -43,3 +47,10 @@ def reference_params(parts_dim, parts_o_four=3, four_o_chain=5):
     return TpmParams(mesh_dim=four_dim, chain_dim=chain_dim, order=4, do_phi=1, long_type=1, rs=1.0 / four_dim,
                      rcut=1.0 / chain_dim, hsoft=2.0 / 25.0 / parts_dim, GM=1.0, eta=0.2 / math.sqrt(2.0),
                      n_total=float(parts_dim) ** 3)
+
+
+def workload_params(name):
+    """The parameters of a named workload (baseline_params with its mesh,
+    order and interlacing)."""
+    w = WORKLOADS[name]
+    return baseline_params(w["n_side"], w["mesh"], w["order"], interlace=w["interlace"])
