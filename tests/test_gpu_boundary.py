@@ -208,7 +208,7 @@ def test_batch_moments_equal_the_box_alone(gpu_ctx):
 def test_ewald_at_positions(gpu_ctx):
     """tpm_ewald_at against the oracle's structure-factor sum at arbitrary
     positions, including positions on particles and inside the softening."""
-    n_side = 16
+    n_side = 24
     n = n_side ** 3
     x, y, z = uniform(n, 81)
     p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, 3))
@@ -224,7 +224,8 @@ def test_ewald_at_positions(gpu_ctx):
         assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()
     # on a particle: the per-particle sum plus its softened self potential
     own = gpu_ctx.ewald(np.arange(4))
-    assert np.allclose(got[0][:4], own[0] - 35.0 / 16.0 / p.hsoft, rtol=1e-12, atol=0)
+    # (FP64 sums accumulated by atomics: their order varies, so 1e-10)
+    assert np.allclose(got[0][:4], own[0] - 35.0 / 16.0 / p.hsoft, rtol=1e-10, atol=0)
 
 
 def _kick_state(n, seed):
@@ -305,7 +306,7 @@ def test_budget_split_is_bit_identical(gpu_ctx, flags):
 
 
 def test_budget_below_one_cell_fails(gpu_ctx):
-    n_side = 16
+    n_side = 24
     pos, v, rung = _kick_state(n_side ** 3, 93)
     _fresh(gpu_ctx, T.baseline_params(n_side, n_side, 3), pos, v, rung)
     gpu_ctx.set_budget(1000)
