@@ -316,6 +316,21 @@ TPM_API tpm_status tpm_ewald(tpm_ctx* ctx, int64_t nsamples, const int64_t* samp
  * Host arrays of nsamples raw positions. */
 TPM_API tpm_status tpm_ewald_at(tpm_ctx* ctx, int64_t nsamples, const uint32_t* sx, const uint32_t* sy,
                                 const uint32_t* sz, double alpha, double* phi, double* gx, double* gy, double* gz);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): tpm_ewald_at with the sample positions and the four outputs
+ * in device memory, enqueued on the context's stream without waiting
+ * (tpm_sync waits) and without allocating.  Its work space (wave vectors,
+ * coefficients and the structure factor) is one fixed 640 KiB device array
+ * per GPU, kept until the process ends: it holds up to 16384 wave vectors,
+ * i.e. alpha below 12.08 (alpha <= 0 picks 9.6); a larger alpha, or more than
+ * TPM_EWALD_NOALLOC_MAX_SAMPLES samples, is rejected with TPM_ERR_INVALID.
+ * Every call on one GPU shares that array, so calls that overlap on one GPU
+ * (from different contexts or threads) race. */
+#define TPM_EWALD_NOALLOC_MAX_SAMPLES (1 << 20)
+TPM_API tpm_status tpm_ewald_at_noalloc(tpm_ctx* ctx, int64_t nsamples, const uint32_t* d_sx, const uint32_t* d_sy,
+                                        const uint32_t* d_sz, double alpha, double* d_phi, double* d_gx,
+                                        double* d_gy, double* d_gz);
 
 /* Zel'dovich initial conditions on the device: n_side^3 particles on the
  * lattice (i+1/2)/n_side displaced by a Gaussian field with power

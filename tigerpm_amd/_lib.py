@@ -171,6 +171,7 @@ def lib():
         "tpm_tree_download": (_int, [vp, vp, vp, vp, vp, vp]),
         "tpm_ewald": (_int, [vp, _i64, _i64p, _dbl, _f64p, _f64p, _f64p, _f64p]),
         "tpm_ewald_at": (_int, [vp, _i64, _u32p, _u32p, _u32p, _dbl, _f64p, _f64p, _f64p, _f64p]),
+        "tpm_ewald_at_noalloc": (_int, [vp, _i64, vp, vp, vp, _dbl, vp, vp, vp, vp]),
         "tpm_set_host_order": (_int, [vp, _int]),
         "tpm_kick_box": (_int, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), _int, _dbl, _dbl,
                                 _int, ctypes.c_uint, ctypes.POINTER(TpmStats)]),
@@ -386,6 +387,13 @@ class Context:
         out = [np.empty(len(sx), np.float64) for _ in range(4)]
         self._chk(self._L.tpm_ewald_at(self._h, len(sx), sx, sy, sz, float(alpha), *out))
         return out
+
+    def ewald_at_noalloc(self, n, d_sx, d_sy, d_sz, d_phi, d_gx, d_gy, d_gz, alpha=0.0):
+        """tpm_ewald_at_noalloc on device addresses (ints): n positions in,
+        four FP64 arrays of n out, enqueued on the context's stream (sync()
+        waits)."""
+        self._chk(self._L.tpm_ewald_at_noalloc(self._h, int(n), d_sx, d_sy, d_sz, float(alpha), d_phi, d_gx, d_gy,
+                                               d_gz))
 
     def set_host_order(self, order):
         """TPM_ORDER_DEVICE or TPM_ORDER_REFERENCE (the reference partition's

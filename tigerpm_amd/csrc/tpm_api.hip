@@ -636,6 +636,16 @@ extern "C" tpm_status tpm_ewald_at(tpm_ctx* ctx, int64_t nsamples, const uint32_
     return tpm_ewald_run(ctx, nsamples, nullptr, pos, alpha, phi, gx, gy, gz);
 }
 
+extern "C" tpm_status tpm_ewald_at_noalloc(tpm_ctx* ctx, int64_t nsamples, const uint32_t* d_sx, const uint32_t* d_sy,
+                                           const uint32_t* d_sz, double alpha, double* d_phi, double* d_gx,
+                                           double* d_gy, double* d_gz) {
+    if (!ctx || nsamples < 0 || (nsamples > 0 && (!d_sx || !d_sy || !d_sz || !d_phi || !d_gx || !d_gy || !d_gz)))
+        return TPM_ERR_INVALID;
+    if (!ctx->params_set) return tpm_fail(ctx, TPM_ERR_STATE, "tpm_ewald_at_noalloc: parameters not set (hsoft)");
+    TPM_HIP(ctx, hipSetDevice(ctx->device));
+    return tpm_ewald_noalloc_run(ctx, nsamples, d_sx, d_sy, d_sz, alpha, d_phi, d_gx, d_gy, d_gz);
+}
+
 extern "C" tpm_status tpm_ic_zeldovich(tpm_ctx* ctx, int n_side, uint64_t seed, double rms_disp,
                                        double spectral_index) {
     if (!ctx) return TPM_ERR_INVALID;

@@ -13,6 +13,8 @@
 // singular term but includes its image/background term (the convention of
 // the CPU restatement); at an arbitrary position every particle counts,
 // softened inside hsoft.
+#include <mutex>
+
 #include "tpm_internal.hpp"
 
 namespace {
@@ -202,7 +204,148 @@ __global__ void fill_f64_kernel(double* p, int64_t n) {
         p[i] = 0.0;
 }
 
+// ---- Work space of tpm_ewald_at_noalloc ----------------------------------
+// The wave vectors, their coefficients and S(h) for up to EW_WS_MAX_H
+// vectors live in one fixed device array (one copy per GPU, held until the
+// process ends) instead of per-call allocations.
+#define EW_WS_MAX_H 16384
+#define EW_WS_HL 0                                       // int4[EW_WS_MAX_H]
+#define EW_WS_COEF (EW_WS_HL + 16 * EW_WS_MAX_H)         // double[EW_WS_MAX_H]
+#define EW_WS_S (EW_WS_COEF + 8 * EW_WS_MAX_H)           // double[2 * EW_WS_MAX_H]
+#define EW_WS_BYTES (EW_WS_S + 16 * EW_WS_MAX_H)         // 640 KiB
+static __device__ __align__(256) unsigned char scratch[EW_WS_BYTES];
+
+// Number of wave vectors 0 < |h| <= hmax in the half space the host loop of
+// tpm_ewald_run enumerates.
+int ew_count_h(int hmax) {
+    int nh = 0;
+    for (int hx = 0; hx <= hmax; hx++)
+        for (int hy = -hmax; hy <= hmax; hy++)
+            for (int hz = -hmax; hz <= hmax; hz++) {
+                const int h2 = hx * hx + hy * hy + hz * hz;
+                if (h2 == 0 || h2 > hmax * hmax) continue;
+                if (hx == 0 && (hy < 0 || (hy == 0 && hz < 0))) continue;
+                nh++;
+            }
+    return nh;
+}
+
+// The same vectors in the same order as that host loop, made on the device
+// by one workgroup: candidates of the (hmax+1) x (2 hmax+1)^2 box in loop
+// order, 256 at a time, compacted with a block-wide scan of ballots.
+__global__ __launch_bounds__(256) void ew_hlist_kernel(int hmax, double alpha, int4* __restrict__ hl,
+                                                        double* __restrict__ coef) {
+    __shared__ int wcount[4];
+    const int side = 2 * hmax + 1;
+    const int64_t total = (int64_t)(hmax + 1) * side * side;
+    const int wave = (int)threadIdx.x >> 6;
+    const int lane = (int)threadIdx.x & 63;
+    int base = 0;
+    for (int64_t c0 = 0; c0 < total; c0 += 256) {
+        const int64_t c = c0 + threadIdx.x;
+        bool keep = false;
+        int hx = 0, hy = 0, hz = 0, h2 = 0;
+        if (c < total) {
+            hx = (int)(c / ((int64_t)side * side));
+            hy = (int)((c / side) % side) - hmax;
+            hz = (int)(c % side) - hmax;
+            h2 = hx * hx + hy * hy + hz * hz;
+            keep = h2 != 0 && h2 <= hmax * hmax && !(hx == 0 && (hy < 0 || (hy == 0 && hz < 0)));
+        }
+        const unsigned long long m = __ballot(keep);
+        if (lane == 0) wcount[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; w++) off += wcount[w];
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (keep) {
+            hl[off + below] = make_int4(hx, hy, hz, 0);
+            coef[off + below] = exp(-M_PI * M_PI * h2 / (alpha * alpha)) / h2;
+        }
+        base += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+// Device address of this GPU's work space, looked up once per device (with
+// that device current) and kept; nullptr (and the message) on failure.
+#define EW_MAX_DEVICES 64
+static void* ew_scratch_ptr(tpm_ctx* ctx) {
+    static std::mutex mu;
+    static void* ptr[EW_MAX_DEVICES] = {};
+    if (ctx->device < 0 || ctx->device >= EW_MAX_DEVICES) {
+        ctx->err = "tpm_ewald_at_noalloc: device index out of range";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[ctx->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        ptr[ctx->device] = p;
+    }
+    return ptr[ctx->device];
+}
+
+tpm_status tpm_ewald_noalloc_run(tpm_ctx* ctx, int64_t ns, const uint32_t* spx, const uint32_t* spy,
+                                 const uint32_t* spz, double alpha, double* phi, double* gx, double* gy, double* gz) {
+    if (alpha <= 0.0) alpha = 9.6;
+    if (alpha < 9.6)
+        return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_ewald_at_noalloc: alpha must be >= 9.6 (minimum image only)");
+    const int hmax = (int)floor(5.2 * alpha / M_PI);
+    const int nh = ew_count_h(hmax);
+    if (nh > EW_WS_MAX_H)
+        return tpm_fail(ctx, TPM_ERR_INVALID,
+                        "tpm_ewald_at_noalloc: alpha = " + std::to_string(alpha) + " needs " + std::to_string(nh) +
+                            " wave vectors, more than the work space's " + std::to_string(EW_WS_MAX_H) +
+                            " (alpha below 12.08)");
+    if (ns > (int64_t)TPM_EWALD_NOALLOC_MAX_SAMPLES)
+        return tpm_fail(ctx, TPM_ERR_INVALID,
+                        "tpm_ewald_at_noalloc: more than " + std::to_string(TPM_EWALD_NOALLOC_MAX_SAMPLES) + " samples");
+    if (ns <= 0) return TPM_OK;
+    unsigned char* ws = static_cast<unsigned char*>(ew_scratch_ptr(ctx));
+    if (!ws) return TPM_ERR_HIP;
+    int4* hl = reinterpret_cast<int4*>(ws + EW_WS_HL);
+    double* coef = reinterpret_cast<double*>(ws + EW_WS_COEF);
+    double* S = reinterpret_cast<double*>(ws + EW_WS_S);
+    hipStream_t st = ctx->stream;
+    ew_hlist_kernel<<<1, 256, 0, st>>>(hmax, alpha, hl, coef);
+    // S(h) accumulates by atomics: cleared on every call (by a kernel, so the
+    // call can be captured in a graph)
+    fill_f64_kernel<<<grid_for(2 * nh, 256), 256, 0, st>>>(S, 2 * nh);
+    ewald_init_kernel<<<grid_for(ns, 256), 256, 0, st>>>(ns, (double)ctx->n * M_PI / (alpha * alpha), phi, gx, gy,
+                                                          gz);
+    TPM_HIP(ctx, hipGetLastError());
+    const int64_t n = ctx->n;
+    if (n > 0) {
+        int64_t sf_chunks = std::max<int64_t>(1, std::min<int64_t>(4096, (n + 32767) / 32768));
+        const int64_t sf_per = (n + sf_chunks - 1) / sf_chunks;
+        sf_chunks = (n + sf_per - 1) / sf_per;
+        dim3 g1((unsigned)sf_chunks, (unsigned)((nh + 255) / 256));
+        structure_factor_kernel<<<g1, 256, 0, st>>>(n, ctx->x, ctx->y, ctx->z, nh, hl, sf_per, S, S + nh);
+        TPM_HIP(ctx, hipGetLastError());
+    }
+    kspace_kernel<<<grid_for(ns, 64), 64, 0, st>>>(ns, spx, spy, spz, nh, hl, coef, S, S + nh, phi, gx, gy, gz);
+    TPM_HIP(ctx, hipGetLastError());
+    if (n > 0) {
+        const int64_t sblocks = (ns + 63) / 64;
+        int64_t rs_chunks =
+            std::max<int64_t>(1, std::min<int64_t>((n + 4095) / 4096, (8192 + sblocks - 1) / sblocks));
+        const int64_t rs_per = (n + rs_chunks - 1) / rs_chunks;
+        rs_chunks = (n + rs_per - 1) / rs_per;
+        dim3 g2((unsigned)rs_chunks, (unsigned)sblocks);
+        realspace_kernel<<<g2, 64, 0, st>>>(n, ctx->x, ctx->y, ctx->z, ns, spx, spy, spz, nullptr, rs_per, alpha,
+                                            ctx->p.hsoft, phi, gx, gy, gz);
+        TPM_HIP(ctx, hipGetLastError());
+    }
+    return TPM_OK;
+}
 
 tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, const uint32_t* const pos[3],
                          double alpha, double* phi, double* gx, double* gy, double* gz) {

@@ -506,6 +506,8 @@ tpm_status tpm_ensure_mesh(tpm_ctx* ctx, int M);
 // (pos[0..2]).
 tpm_status tpm_ewald_run(tpm_ctx* ctx, int64_t ns, const int64_t* samples, const uint32_t* const pos[3],
                          double alpha, double* phi, double* gx, double* gy, double* gz);
+tpm_status tpm_ewald_noalloc_run(tpm_ctx* ctx, int64_t ns, const uint32_t* spx, const uint32_t* spy,
+                                 const uint32_t* spz, double alpha, double* phi, double* gx, double* gy, double* gz);
 tpm_status tpm_ic_zeldovich_run(tpm_ctx* ctx, int n_side, uint64_t seed, double rms_disp, double spectral_index);
 tpm_status tpm_ic_uniform_run(tpm_ctx* ctx, int64_t n, uint64_t seed);
 tpm_status tpm_fill_u64(tpm_ctx* ctx, unsigned long long* p, int64_t n);
