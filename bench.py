@@ -323,21 +323,32 @@ def serving_leg(T, rates, count, budget_gib, seed=77):
     kick + drift steps per box with the state resident, admission by the
     arena budget; latency = arrival to the end of the box's last step."""
     from tigerpm_amd import serving as S
-    out = {"budget_gib": budget_gib, "kicks_per_request": 10, "rates": []}
+    out = {"budget_gib": budget_gib, "kicks_per_request": 10, "requests_per_rate": count, "rates": []}
     lim = S.budget_limits(budget_gib * (1 << 30))
-    for rate in rates:
-        # the same boxes at every rate; only the arrival times scale
-        reqs = S.poisson_requests(count, rate, seed)
-        with T.Batch(**lim) as bt:
-            srv = S.Server(bt, kicks=10)
-            t_prep = srv.prepare(reqs)
+    with T.Batch(**lim) as bt:
+        srv = S.Server(bt, kicks=10)
+        # FFT plans of every size the requests use, made once (not part of
+        # any latency)
+        out["prepare_s"] = srv.prepare(S.poisson_requests(count, 1.0, seed))
+        # capacity: every request waiting at t = 0 (closed loop, the batch
+        # always as full as the budget allows)
+        cap = srv.run([r._replace(arrival=0.0) for r in S.poisson_requests(count, 1.0, seed)])
+        cap.pop("records")
+        out["capacity"] = cap
+        log("serving capacity (all requests at t = 0): %.4g kicks/s, p50 %.3f s, p99 %.3f s, mean resident %.1f"
+            % (cap["kicks_per_s"], cap["latency_p50_s"], cap["latency_p99_s"], cap["mean_resident_boxes"]))
+        for rate in rates:
+            # the same boxes at every rate; only the arrival times scale
+            reqs = S.poisson_requests(count, rate, seed)
             r = srv.run(reqs)
-        r.pop("records")
-        r["rate_per_s"] = rate
-        r["prepare_s"] = t_prep
-        log("serving rate %g/s: p50 %.3f s, p99 %.3f s, %.4g kicks/s, mean resident %.1f"
-            % (rate, r["latency_p50_s"], r["latency_p99_s"], r["kicks_per_s"], r["mean_resident_boxes"]))
-        out["rates"].append(r)
+            r.pop("records")
+            r["rate_per_s"] = rate
+            r["offered_kicks_per_s"] = sum(q.n_side ** 3 for q in reqs) * 10 / max(reqs[-1].arrival, 1e-9)
+            r["fraction_of_capacity"] = r["kicks_per_s"] / cap["kicks_per_s"]
+            log("serving rate %g/s: p50 %.3f s, p99 %.3f s, %.4g kicks/s (%.2f of capacity), mean resident %.1f"
+                % (rate, r["latency_p50_s"], r["latency_p99_s"], r["kicks_per_s"], r["fraction_of_capacity"],
+                   r["mean_resident_boxes"]))
+            out["rates"].append(r)
     # accuracy after 10 steps for three requests served alone: forces at the
     # final positions (one more step with the kick and drift off) vs Ewald
     reqs = S.poisson_requests(count, rates[0], seed)
@@ -378,9 +389,15 @@ def main():
                     help="replay the ensemble step as a captured graph (0: plain launches, e.g. for tracing)")
     ap.add_argument("--only", choices=["ensemble", "serving", "tree"],
                     help="run only this leg and print its result (for profiling; not the benchmark line)")
-    ap.add_argument("--serving-rates", default="4,8,12,16", help="Poisson arrival rates (requests/s)")
-    ap.add_argument("--serving-requests", type=int, default=40)
+    ap.add_argument("--serving-rates", default="4,8,12,16,20,24,32",
+                    help="Poisson arrival rates (requests/s); the leg also measures the capacity")
+    ap.add_argument("--serving-requests", type=int, default=60)
     ap.add_argument("--serving-budget-gib", type=float, default=16.0)
+    ap.add_argument("--deposit-block", type=int, default=0,
+                    help="deposit block edge in chain cells (0: automatic, see tpm_tuning)")
+    ap.add_argument("--tile-sinks", type=int, default=16, help="sinks per PP tile (8, 16, 32, 64)")
+    ap.add_argument("--pp-waves", type=int, default=0, help="PP tiles per workgroup (1, 4; 0: default)")
+    ap.add_argument("--pp-lists", type=int, default=-1, help="per-sink source lists (0, 1; -1: default)")
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -400,6 +417,8 @@ def main():
     w = WORKLOADS[args.workload]
     ctx = T.Context(0)
     p = ctx.set_params(T.baseline_params(w["n_side"], w["mesh"], w["order"], interlace=w["interlace"]))
+    ctx.set_tuning(tile_sinks=args.tile_sinks, deposit_block=args.deposit_block, pp_waves=args.pp_waves,
+                   pp_lists=args.pp_lists)
     if args.only == "tree":
         if w["ic"] == "zeldovich":
             ctx.ic_zeldovich(w["n_side"], w["seed"], w["rms"], -2.0)
@@ -454,16 +473,19 @@ def main():
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "n_particles": n, "mesh": p.mesh_dim, "chain_dim": p.chain_dim,
                    "assignment": {2: "CIC", 3: "TSC", 4: "cloud4"}[p.order], "rs_cells": p.rs * p.mesh_dim,
-                   "rcut_over_rs": p.rcut / p.rs, "ic": w["ic"], "rms_disp_cells": w["rms"], "seed": w["seed"]},
+                   "rcut_over_rs": p.rcut / p.rs, "ic": w["ic"], "rms_disp_cells": w["rms"], "seed": w["seed"],
+                   "tuning": ctx.get_tuning()},
         # Bound vocabulary is hbm|mfma: the PP kernel is FP32 compute bound and
         # its peak is MI355X's FP32 figure (157.3 TFLOP/s, vector and matrix).
-        "roofline": {"bound": "mfma", "bound_note": "FP32 vector ALU peak; the kernel issues no MFMA",
+        # unit_kind says what actually bounds it: FP32 on the vector ALUs.
+        "roofline": {"bound": "mfma", "unit_kind": "valu_fp32",
+                     "bound_note": "FP32 vector ALU peak; the kernel issues no MFMA instruction",
                      "kernel": "pp_kick_kernel",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "flops_per_launch": FLOPS_PER_PAIR * W, "kernel_ms": pp_ms,
                      "candidate_pairs": W, "executed_pairs": st.executed_pairs,
-                     "executed_tflops": executed},
+                     "executed_tflops": executed, "executed_frac": executed / PEAK_FP32_TFLOPS},
         "stages_ms": {"chainmesh": prof["chainmesh_ms"] / max(prof["steps"], 1),
                       "pm": prof["pm_ms"] / max(prof["steps"], 1), "pp_kick": pp_ms,
                       "step": prof["step_ms"] / max(prof["steps"], 1)},
@@ -489,13 +511,16 @@ def main():
         t = json.load(open(tfile))
         out["roofline"]["traffic"] = t["traffic_bytes_per_launch"]
         out["roofline"]["traffic_source"] = t["source"]
-    # Ceiling of the pair loop at the kernel's occupancy: the same pair body
-    # timed alone over sources held in LDS (tpm_probe_pair_rate).
+    # The pair body timed alone at the kernel's occupancy over sources held
+    # in LDS (tpm_probe_pair_rate).  A reference rate, not a ceiling: its
+    # uniform sources mix in-cutoff and softened pairs differently from the
+    # kernel's, and the kernel has run faster than it.
     body = ctx.probe_pair_rate(8)
     kernel_rate = st.executed_pairs / (pp_ms * 1e-3)
-    out["roofline"]["pair_body_ceiling"] = {
-        "pairs_per_s": body, "kernel_executed_pairs_per_s": kernel_rate, "frac": kernel_rate / body,
-        "waves_per_simd": 8, "note": "pp_kick_kernel's executed pairs/s over its pair body's rate alone"}
+    out["roofline"]["pair_body_rate"] = {
+        "pairs_per_s": body, "kernel_executed_pairs_per_s": kernel_rate, "ratio": kernel_rate / body,
+        "waves_per_simd": 8,
+        "note": "pp_kick_kernel's executed pairs/s over the pair body's rate alone (a reference, not a ceiling)"}
     log("pair body alone %.4g pairs/s; kernel %.4g executed pairs/s (%.3f)" % (body, kernel_rate, kernel_rate / body))
     up = useful_pairs(ctx, p, 2048, w["seed"])
     up["candidate_per_particle"] = W / n

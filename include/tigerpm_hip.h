@@ -121,11 +121,20 @@ typedef struct {
  *   tile_sinks:    sinks per PP work item, 8, 16, 32 or 64 (one wave64 walks
  *                  64/tile_sinks source streams of one culled source list);
  *   deposit_block: chain cells per edge of an LDS-privatised deposit block,
- *                  0 = largest that fits 64 KiB, -1 = per-particle global
- *                  atomics. */
+ *                  0 = the largest edge <= 3 whose int64 footprint fits
+ *                  64 KiB, k > 0 = the largest edge <= k that fits a
+ *                  workgroup's 160 KiB, -1 = per-particle global atomics.
+ *                  The integer grid is the same bits for every choice. */
 typedef struct {
     int tile_sinks;
     int deposit_block;
+    int pp_waves;  /* PP tiles per workgroup, one wave each: 1 or 4 (0 =
+                      default); the waves of one workgroup share a CU and its
+                      L1 cache, so neighbouring tiles' source loads meet there */
+    int pp_lists;  /* 16-sink tiles: 1 = per-sink source lists (each sink
+                      evaluates only the staged sources within the cull
+                      distance of itself, not of the tile's box), 0 = one
+                      list per tile, -1 = default */
 } tpm_tuning;
 
 /* Context lifetime.  device = HIP device ordinal. */
@@ -380,7 +389,8 @@ typedef struct {
 /* Launch shape of the batched step (results do not depend on it beyond FP32
  * summation order):
  *   tile_sinks:    8, 16, 32 or 64 sinks per PP work item;
- *   deposit_block: chain cells per deposit block edge, 0 = auto;
+ *   deposit_block: chain cells per deposit block edge, 0 = auto (as in
+ *                  tpm_tuning);
  *   sched:         0 = one workgroup per tile (XCD-aware order),
  *                  1 = persistent workgroups taking tiles from per-XCD queues;
  *   graph:         1 = capture the step in a HIP graph and replay it while
@@ -390,6 +400,7 @@ typedef struct {
     int deposit_block;
     int sched;
     int graph;
+    int pp_lists;  /* per-sink source lists in the PP kernel (as tpm_tuning) */
 } tpm_batch_tuning;
 
 typedef struct {

@@ -28,16 +28,20 @@ def assert_same(a, b, what):
             what, k, np.max(np.abs(a[k].astype(np.float64) - b[k].astype(np.float64))))
 
 
-def test_batch_equals_plain_kick():
+@pytest.mark.parametrize("lists", [0, 1])
+def test_batch_equals_plain_kick(lists):
     """A batch of boxes of different sizes and chain meshes: each box's two
-    steps equal tpm_kick_treepm on the same box, every array bit for bit."""
+    steps equal tpm_kick_treepm on the same box, every array bit for bit
+    (one list per tile, and per-sink lists)."""
     boxes = [(20, 11, 1.5), (24, 12, 0.4), (28, 13, 1.0), (32, 14, 2.0)]
     with new_batch() as bt:
+        bt.set_tuning(tile_sinks=16, pp_lists=lists)
         ids = [bt.add_zeldovich(spec(ns), ns, seed, rms) for ns, seed, rms in boxes]
         ctxs = []
         for i in ids:
             c = T.Context(0)
             bt.to_ctx(i, c)
+            c.set_tuning(tile_sinks=16, pp_lists=lists)
             ctxs.append(c)
         for k in range(2):
             st = bt.step(0)
@@ -197,3 +201,43 @@ def test_admission_by_budget():
         with pytest.raises(T.TpmError) as e:
             bt.add(spec(18, order=2), u, u, u)  # order differs from the batch's
         assert e.value.status == -2
+
+
+def test_ensemble_shape_vs_oracle():
+    """The ensemble configuration's shape: eight boxes with n_side
+    log-spaced over [32, 64], rms displacements over [0.3, 2] cells, mesh =
+    n_side, cloud4, one batched step.  Every box's forces, potentials,
+    velocities and rungs against the oracle's whole CPU kick of the same
+    particles."""
+    boxes = [(32, 101, 0.3), (35, 102, 1.7), (39, 103, 0.9), (43, 104, 2.0), (47, 105, 0.5), (52, 106, 1.2),
+             (58, 107, 0.7), (64, 108, 1.5)]
+    total = sum(ns ** 3 for ns, _, _ in boxes)
+    with T.Batch(total + 1000, total + 1000, len(boxes), 4, 0) as bt:
+        ids = [bt.add_zeldovich(spec(ns, order=4, interlace=0), ns, seed, rms) for ns, seed, rms in boxes]
+        print("ensemble shape: %d boxes, %d particles" % (len(ids), total), flush=True)
+        st = bt.step(0)
+        assert st.n_boxes == len(boxes) and st.n_particles == total
+        for (ns, seed, rms), i in zip(boxes, ids):
+            d = bt.download(i)
+            n = ns ** 3
+            p = T.baseline_params(ns, ns, 4)
+            op = O.make_params(p.mesh_dim, p.chain_dim, p.order, p.rs, p.rcut, p.hsoft, p.GM, p.eta, p.n_total,
+                               p.do_phi, p.interlace)
+            zf = np.zeros(n, np.float32)
+            print("oracle box n_side %d" % ns, flush=True)
+            o = O.kick_full(op, d["x"], d["y"], d["z"], zf, zf, zf, np.zeros(n, np.int8), t0=T0)
+            for k in ("x", "y", "z"):
+                assert np.array_equal(d[k], o[k]), (ns, k)
+            G = np.stack([d["gx"], d["gy"], d["gz"]], 1).astype(np.float64)
+            OG = np.stack([o["gx"], o["gy"], o["gz"]], 1).astype(np.float64)
+            err = np.sqrt(((G - OG) ** 2).sum() / (OG ** 2).sum())
+            assert err < 1e-5, (ns, err)
+            ep = np.sqrt(((d["pot"] - o["pot"]).astype(np.float64) ** 2).sum() / (o["pot"].astype(np.float64) ** 2).sum())
+            assert ep < 1e-5, (ns, ep)
+            V = np.stack([d["vx"], d["vy"], d["vz"]], 1).astype(np.float64)
+            OV = np.stack([o["vx"], o["vy"], o["vz"]], 1).astype(np.float64)
+            same = d["rung"] == o["rung"]
+            # rungs equal except where log2 of the step sits on an integer
+            assert same.mean() > 0.999, (ns, same.mean())
+            ev = np.sqrt(((V[same] - OV[same]) ** 2).sum() / (OV[same] ** 2).sum())
+            assert ev < 1e-5, (ns, ev)

@@ -66,7 +66,8 @@ class TpmStats(ctypes.Structure):
 
 
 class TpmTuning(ctypes.Structure):
-    _fields_ = [("tile_sinks", ctypes.c_int), ("deposit_block", ctypes.c_int)]
+    _fields_ = [("tile_sinks", ctypes.c_int), ("deposit_block", ctypes.c_int), ("pp_waves", ctypes.c_int),
+                ("pp_lists", ctypes.c_int)]
 
 
 TPM_BATCH_DRIFT = 8
@@ -85,7 +86,7 @@ class TpmBoxSpec(ctypes.Structure):
 
 class TpmBatchTuning(ctypes.Structure):
     _fields_ = [("tile_sinks", ctypes.c_int), ("deposit_block", ctypes.c_int), ("sched", ctypes.c_int),
-                ("graph", ctypes.c_int)]
+                ("graph", ctypes.c_int), ("pp_lists", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -259,13 +260,15 @@ class Context:
         self.params = q
         return q
 
-    def set_tuning(self, tile_sinks=16, deposit_block=0):
-        self._chk(self._L.tpm_set_tuning(self._h, ctypes.byref(TpmTuning(int(tile_sinks), int(deposit_block)))))
+    def set_tuning(self, tile_sinks=16, deposit_block=0, pp_waves=0, pp_lists=-1):
+        self._chk(self._L.tpm_set_tuning(self._h, ctypes.byref(TpmTuning(int(tile_sinks), int(deposit_block),
+                                                                         int(pp_waves), int(pp_lists)))))
 
     def get_tuning(self):
         t = TpmTuning()
         self._chk(self._L.tpm_get_tuning(self._h, ctypes.byref(t)))
-        return dict(tile_sinks=t.tile_sinks, deposit_block=t.deposit_block)
+        return dict(tile_sinks=t.tile_sinks, deposit_block=t.deposit_block, pp_waves=t.pp_waves,
+                    pp_lists=t.pp_lists)
 
     def upload(self, x, y, z, vx=None, vy=None, vz=None, rung=None):
         x, y, z = [np.ascontiguousarray(a, np.uint32) for a in (x, y, z)]
@@ -543,8 +546,8 @@ class Batch:
         ctx._chk(ctx._L.tpm_get_params(ctx._h, ctypes.byref(q)))
         ctx.params = q
 
-    def set_tuning(self, tile_sinks=16, deposit_block=0, sched=0, graph=0):
-        t = TpmBatchTuning(int(tile_sinks), int(deposit_block), int(sched), int(graph))
+    def set_tuning(self, tile_sinks=16, deposit_block=0, sched=0, graph=0, pp_lists=-1):
+        t = TpmBatchTuning(int(tile_sinks), int(deposit_block), int(sched), int(graph), int(pp_lists))
         self._chk(self._L.tpm_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def get_tuning(self):

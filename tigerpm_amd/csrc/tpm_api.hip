@@ -126,8 +126,13 @@ extern "C" tpm_status tpm_set_tuning(tpm_ctx* ctx, const tpm_tuning* t) {
         return tpm_fail(ctx, TPM_ERR_INVALID, "tile_sinks must be 8, 16, 32 or 64");
     if (t->deposit_block < -1 || t->deposit_block > 8)
         return tpm_fail(ctx, TPM_ERR_INVALID, "deposit_block must be in [-1, 8]");
+    if (t->pp_waves != 0 && t->pp_waves != 1 && t->pp_waves != 4)
+        return tpm_fail(ctx, TPM_ERR_INVALID, "pp_waves must be 0, 1 or 4");
+    if (t->pp_lists < -1 || t->pp_lists > 1) return tpm_fail(ctx, TPM_ERR_INVALID, "pp_lists must be -1, 0 or 1");
     ctx->tile = t->tile_sinks;
     ctx->deposit_block = t->deposit_block;
+    ctx->pp_waves = t->pp_waves == 0 ? TPM_PP_WAVES_DEFAULT : t->pp_waves;
+    ctx->pp_lists = t->pp_lists < 0 ? TPM_PP_LISTS_DEFAULT : t->pp_lists;
     return TPM_OK;
 }
 
@@ -135,6 +140,8 @@ extern "C" tpm_status tpm_get_tuning(const tpm_ctx* ctx, tpm_tuning* t) {
     if (!ctx || !t) return TPM_ERR_INVALID;
     t->tile_sinks = ctx->tile;
     t->deposit_block = ctx->deposit_block;
+    t->pp_waves = ctx->pp_waves;
+    t->pp_lists = ctx->pp_lists;
     return TPM_OK;
 }
 

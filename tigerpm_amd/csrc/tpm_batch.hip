@@ -456,7 +456,7 @@ extern "C" tpm_status tpm_batch_create(int device, const tpm_batch_limits* l, tp
         return fail(TPM_ERR_HIP);
     bt->mesh_alloc.reset(c.m);
     bt->k_alloc.reset(c.k);
-    bt->persist_grid = tpm_pp_persistent_grid(device, bt->tune.tile_sinks);
+    bt->persist_grid = tpm_pp_persistent_grid(device, bt->tune.tile_sinks, bt->tune.pp_lists != 0);
     *out = bt;
     return TPM_OK;
 }
@@ -678,6 +678,7 @@ extern "C" tpm_status tpm_batch_set_tuning(tpm_batch* bt, const tpm_batch_tuning
         return tpm_fail(bt, TPM_ERR_INVALID, "deposit_block must be in [0, 8]");
     if (t->sched != 0 && t->sched != 1) return tpm_fail(bt, TPM_ERR_INVALID, "sched must be 0 or 1");
     if (t->graph != 0 && t->graph != 1) return tpm_fail(bt, TPM_ERR_INVALID, "graph must be 0 or 1");
+    if (t->pp_lists < -1 || t->pp_lists > 1) return tpm_fail(bt, TPM_ERR_INVALID, "pp_lists must be -1, 0 or 1");
     (void)hipSetDevice(bt->device);
     if (t->deposit_block != bt->tune.deposit_block) {
         // re-block every resident box; keep the old blocking if one does not fit
@@ -694,8 +695,11 @@ extern "C" tpm_status tpm_batch_set_tuning(tpm_batch* bt, const tpm_batch_tuning
             b.nb = (b.C + b.B - 1) / b.B;
         }
     }
-    if (t->tile_sinks != bt->tune.tile_sinks) bt->persist_grid = tpm_pp_persistent_grid(bt->device, t->tile_sinks);
+    const int lists = t->pp_lists < 0 ? TPM_PP_LISTS_DEFAULT : t->pp_lists;
+    if (t->tile_sinks != bt->tune.tile_sinks || lists != bt->tune.pp_lists)
+        bt->persist_grid = tpm_pp_persistent_grid(bt->device, t->tile_sinks, lists != 0);
     bt->tune = *t;
+    bt->tune.pp_lists = lists;
     drop_graphs(bt);
     bt->layout_version++;
     return TPM_OK;
@@ -728,11 +732,13 @@ extern "C" tpm_status tpm_batch_autotune(tpm_batch* bt, int reps, tpm_batch_tuni
     };
     tpm_batch_tuning bestt = keep;
     double bms = 1e300;
-    const int tiles[4] = {8, 16, 32, 64};
-    for (int ti = 0; ti < 4; ti++)
+    // (tile size, per-sink lists): lists only with 16-sink tiles
+    const int tiles[5][2] = {{8, 0}, {16, 0}, {16, 1}, {32, 0}, {64, 0}};
+    for (int ti = 0; ti < 5; ti++)
         for (int sc = 0; sc <= 1; sc++) {
             tpm_batch_tuning t = keep;
-            t.tile_sinks = tiles[ti];
+            t.tile_sinks = tiles[ti][0];
+            t.pp_lists = tiles[ti][1];
             t.sched = sc;
             double ms = 0.0;
             TPM_TRY(time_it(t, &ms));

@@ -91,6 +91,12 @@ struct tpm_box_dev {
 // parameters (compute_pp_interaction constants, kick_treepm.cu:833-853).
 void tpm_box_fill(tpm_box_dev* b, const tpm_params& p, int min_rung, int first_call, double t0, double scale);
 
+// PP tiles per workgroup when the tuning asks for the default.
+#define TPM_PP_WAVES_DEFAULT 1
+// Per-sink source lists in the PP kernel (16-sink tiles) unless the tuning
+// says otherwise, for single boxes and batches alike.
+#define TPM_PP_LISTS_DEFAULT 0
+
 struct tpm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -131,6 +137,8 @@ struct tpm_ctx {
     int tile = 16;        // sinks per PP tile: 16, 32 or 64 (tuning)
     int chain_tile = 0;   // tile size the current tile list was built with
     int deposit_block = 0;  // chain cells per deposit block edge; 0 auto, -1 per-particle atomics
+    int pp_waves = TPM_PP_WAVES_DEFAULT;  // PP tiles per workgroup (1 or 4)
+    int pp_lists = TPM_PP_LISTS_DEFAULT;  // per-sink source lists (16-sink tiles)
     int64_t chain_epoch = 0;  // counts chain-mesh builds (trees remember the one they follow)
 
     // The order the caller sees (tpm_set_host_order).  With
@@ -435,7 +443,7 @@ struct tpm_batch {
     // slot 0: the persistent PP launch's tile queues
     unsigned long long* counters = nullptr;
     double* moment_buf = nullptr;  // tpm_batch_moments: partials + results
-    tpm_batch_tuning tune{16, 0, 0, 0};
+    tpm_batch_tuning tune{16, 0, 0, 0, TPM_PP_LISTS_DEFAULT};
     int persist_grid = 0;  // workgroups of the persistent PP launch
 
     // captured steps, one per buffer parity (which of the two particle
@@ -528,8 +536,11 @@ tpm_status tpm_batch_pp(tpm_batch* bt, const tpm_batch_layout& L, unsigned flags
 tpm_status tpm_batch_drift(tpm_batch* bt, const tpm_batch_layout& L);
 tpm_status tpm_batch_candidates(tpm_batch* bt, const tpm_batch_layout& L);
 tpm_status tpm_batch_fill_u64(tpm_batch* bt, unsigned long long* p, int64_t n);
-int tpm_pp_persistent_grid(int device, int ts);
+int tpm_pp_persistent_grid(int device, int ts, bool lists);
 tpm_status tpm_green_table(std::string& err, hipStream_t s, int M, double rs, int order, float* table);
 // Deposit blocking of one box: block edge B (chain cells) and the widest
 // footprint in mesh nodes, so that w^3 int64 nodes fit 64 KiB of LDS.
+// LDS of one workgroup on gfx950 (160 KiB), the cap of a requested
+// deposit block's footprint.
+#define TPM_DEPOSIT_LDS_MAX (160 * 1024)
 void tpm_deposit_blocking(int C, int M, int interlace, int want, int* B, int* wmax);

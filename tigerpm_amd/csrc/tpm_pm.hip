@@ -303,6 +303,24 @@ tpm_status tpm_batch_fill_u64(tpm_batch* bt, unsigned long long* p, int64_t n) {
     return TPM_OK;
 }
 
+// Dynamic LDS above 64 KiB must be allowed per kernel: the deposit
+// kernels may take up to TPM_DEPOSIT_LDS_MAX (once per process and device).
+static tpm_status deposit_lds_limit(std::string& err) {
+    static std::mutex mu;
+    static bool done[64] = {};
+    int dev = 0;
+    TPM_HIPE(err, hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if (dev >= 0 && dev < 64 && done[dev]) return TPM_OK;
+    const void* k[6] = {(const void*)deposit_blocked_kernel<2>, (const void*)deposit_blocked_kernel<3>,
+                        (const void*)deposit_blocked_kernel<4>, (const void*)bdeposit_kernel<2>,
+                        (const void*)bdeposit_kernel<3>,        (const void*)bdeposit_kernel<4>};
+    for (const void* f : k)
+        TPM_HIPE(err, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, TPM_DEPOSIT_LDS_MAX));
+    if (dev >= 0 && dev < 64) done[dev] = true;
+    return TPM_OK;
+}
+
 // Deposit (grid shift) of every live box into its region of rho_q, then the
 // float image of the used part of the arena into out.
 static tpm_status batch_deposit(tpm_batch* bt, const tpm_batch_layout& L, uint64_t shift, float* out) {
@@ -310,6 +328,7 @@ static tpm_status batch_deposit(tpm_batch* bt, const tpm_batch_layout& L, uint64
     TPM_TRY(tpm_batch_fill_u64(bt, bt->rho_q, L.mesh_hw));
     if (L.ndep > 0) {
         const size_t lds = (size_t)L.lds;
+        if (lds > 65536) TPM_TRY(deposit_lds_limit(bt->err));
         switch (L.order) {
             case 2:
                 bdeposit_kernel<2><<<(unsigned)L.ndep, 256, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
@@ -513,7 +532,10 @@ tpm_status tpm_ensure_mesh(tpm_ctx* ctx, int M) {
 
 void tpm_deposit_blocking(int C, int M, int interlace, int want, int* Bout, int* wout) {
     // the largest block edge (cells, <= want or 3) whose footprint of int64
-    // nodes fits 64 KiB of LDS, for both grid shifts in use
+    // nodes fits the LDS of one workgroup: 64 KiB when chosen automatically
+    // (want = 0: several workgroups per CU), up to gfx950's 160 KiB when the
+    // caller asks for a block edge (fewer, larger blocks: less halo traffic)
+    const int64_t lds_cap = want > 0 ? TPM_DEPOSIT_LDS_MAX : 65536;
     const double ninv = 1.0 / C;
     std::vector<uint64_t> b(C + 1);
     for (int k = 0; k < C; k++) b[k] = tpm_split_plane(k, ninv);
@@ -529,7 +551,7 @@ void tpm_deposit_blocking(int C, int M, int interlace, int want, int* Bout, int*
                 block_nodes(b[c0], b[c1 + 1] - 1, M, sh ? (1ull << 31) : 0ull, &lo, &w);
                 wm = std::max(wm, w);
             }
-        if ((int64_t)wm * wm * wm * 8 <= 65536) {
+        if ((int64_t)wm * wm * wm * 8 <= lds_cap) {
             *Bout = cand;
             *wout = wm;
             return;
@@ -563,6 +585,7 @@ static tpm_status deposit_density(tpm_ctx* ctx, uint64_t shift) {
         da.nb = (ctx->C + B - 1) / B;
         const unsigned nblk = (unsigned)((int64_t)da.nb * da.nb * da.nb);
         const size_t lds = sizeof(unsigned long long) * (size_t)wmax * wmax * wmax;
+        if (lds > 65536) TPM_TRY(deposit_lds_limit(ctx->err));
         switch (ctx->p.order) {
             case 2: deposit_blocked_kernel<2><<<nblk, 256, lds, s>>>(da); break;
             case 3: deposit_blocked_kernel<3><<<nblk, 256, lds, s>>>(da); break;

@@ -22,6 +22,7 @@
 // is the reference's up to exact power-of-two factors.  The self pair is
 // evaluated like any other source, as in the reference: inside the
 // softening branch it contributes -35/16 h^-1 to phi and nothing to g.
+#include <type_traits>
 #include <vector>
 
 #include "tpm_pp_device.hpp"
@@ -29,6 +30,12 @@
 #include <climits>
 
 #define SBUF 256  // staged sources per flush (4 KiB LDS)
+// Per-sink source lists (LISTS, 16-sink tiles): a flush of <= LCAP staged
+// sources (staging flushes above LFLUSH), the far dummy at LFAR.
+#define LCAP 144
+#define LFLUSH (LCAP - 64)
+#define LSBUF 160
+#define LFAR (LSBUF - 1)
 
 // Every fused multiply-add in this file is written out (fmaf) and the
 // compiler may fuse nothing else, so the results come out the same bits in
@@ -94,12 +101,60 @@ __device__ __forceinline__ tpm_box_dev tile_box(const PPArgs& a, int w) {
     }
 }
 
-template <int TS, bool BATCH, bool PERSIST, bool PME = false>
+// f(integral_constant<int, k>) for k = 0 .. N-1, k a constant expression
+// in each call.
+template <int K, int N>
+struct static_for_t {
+    template <class F>
+    static __device__ __forceinline__ void run(F& f) {
+        f(std::integral_constant<int, K>{});
+        static_for_t<K + 1, N>::run(f);
+    }
+};
+template <int N>
+struct static_for_t<N, N> {
+    template <class F>
+    static __device__ __forceinline__ void run(F&) {}
+};
+template <int N, class F>
+__device__ __forceinline__ void static_for(F& f) {
+    static_for_t<0, N>::run(f);
+}
+
+// WPG = tiles per workgroup, one wave each (1 or 4).  The waves of a
+// workgroup run on one CU and take consecutive tiles of their XCD's run,
+// mostly of one chain cell, so their source loads meet in that CU's L1;
+// they share nothing else and never wait for each other.
+// LISTS = true (16-sink tiles, culling on): each flush of staged sources is
+// sorted into one list per sink of the sources within the cull distance of
+// that sink (not of the tile's box), and each sink's 4 source streams walk
+// its own list; a flush takes as many steps as its longest list.
+template <int TS, bool BATCH, bool PERSIST, bool PME = false, int WPG = 1, bool LISTS = false>
 // Occupancy: 8 waves per SIMD (64 VGPRs, no spills) for one box and for a
-// batch (whose box record tile_box keeps in SGPRs).
-__global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) {
+// batch (whose box record tile_box keeps in SGPRs); 7 with per-sink lists
+// (72 VGPRs: at 64 the list build spills).
+__global__ __launch_bounds__(64 * WPG, PERSIST ? 1 : (LISTS ? 7 : 8)) void pp_kick_kernel(PPArgs a) {
+    static_assert(WPG == 1 || (WPG == 4 && !PERSIST), "several tiles per workgroup: one-shot launch only");
+    static_assert(!LISTS || (TS == 16 && !PME), "per-sink lists: 16-sink tiles, kick_treepm's law");
     constexpr int SPLIT = 64 / TS;  // source streams per wave
-    __shared__ int4 sbuf[SBUF];
+    constexpr int NSB = LISTS ? LSBUF : SBUF;
+    constexpr int FLUSH_AT = LISTS ? LFLUSH : SBUF - 2 * 64;
+    __shared__ int4 sbufs[WPG][NSB];
+    __shared__ uint8_t lsts[WPG][LISTS ? 16 * LCAP + 16 : 4];
+    __shared__ float4 sinkfs[WPG][LISTS ? 16 : 1];
+    const int wslot = WPG == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    int4* const sbuf = sbufs[wslot];
+    uint8_t* const lst = lsts[wslot];
+    float4* const sinkf = sinkfs[wslot];
+    // the staged list is written and read by the lanes of one wave
+    auto lds_sync = [&]() {
+        if constexpr (WPG == 1) {
+            __syncthreads();
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+        }
+    };
     const int nt = *a.ntiles;
     const int per_xcd = (nt + 7) >> 3;
     const int xcd = (int)(blockIdx.x & 7);
@@ -108,7 +163,7 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) 
     if constexpr (PERSIST) {
         t = INT_MAX;
     } else {
-        const int k = blockIdx.x >> 3;
+        const int k = (int)(blockIdx.x >> 3) * WPG + (WPG == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6));
         if (k >= per_xcd) return;
         t = xcd * per_xcd + k;
         if (t >= nt) return;
@@ -169,26 +224,106 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) 
                 const bool cull = B.cull != 0;
                 const float cull2s = B.cull2s;
                 int nbuf = 0;
-                unsigned long long staged = 0ull;
-                // Far dummy that pads the list to a multiple of SPLIT: r >= rcut.
-                const int4 far = make_int4(0x40000000, 0x40000000, 0x40000000, -1);
-                auto flush = [&]() {
-                    const int padded = (nbuf + SPLIT - 1) / SPLIT * SPLIT;
-                    if (lane < padded - nbuf) sbuf[nbuf + lane] = far;
-                    __syncthreads();
-                    int4 s = sbuf[stream];
-                    // the prefetch past the last source reads a slot below
-                    // padded + SPLIT <= 195 + 4 < SBUF (a flush starts at
-                    // nbuf <= SBUF / 2 + 64), never used
+                unsigned long long executed = 0ull;
+                const int nact = __popcll(amask & ((TS == 64) ? ~0ull : ((1ull << TS) - 1)));
+                // Far dummy of the per-sink lists: offset -2^31 on every
+                // axis, i.e. at least 1/2 - 1/C of the box from every sink of
+                // the tile (a sink lies within one cell of the tile's first
+                // particle), so r > rcut whenever culling is on (C >= 5).
+                const int4 far = make_int4(INT_MIN, INT_MIN, INT_MIN, -1);
+                // Per-sink lists: the tile's sinks as floats in LDS, read by
+                // every lane alike (an inactive sink sits far from every
+                // source: empty list).
+                if constexpr (LISTS) {
+                    if (lane < 16)
+                        sinkf[lane] = make_float4(active ? (float)ox : 3.0e9f, active ? (float)oy : 3.0e9f,
+                                                  active ? (float)oz : 3.0e9f, 0.f);
+                    if (lane == 0) sbuf[LFAR] = far;
+                }
+                // Sources in range of no sink cost nothing but the test: a
+                // source joins sink k's list when its float distance to the
+                // sink is inside the cull distance (float(int32) offsets are
+                // off by <= 8 raw units here, far inside the 512-unit margin),
+                // so no pair with r < rcut is dropped and the pair law's own
+                // cutoff decides as before.
+                auto flush_lists = [&]() {
+                    lds_sync();
+                    int cntv = 0;  // lane k < 16: entries of sink k's list
+                    for (int r0 = 0; r0 < nbuf; r0 += 64) {
+                        const int e = r0 + lane;
+                        // lanes past the staged sources test the far dummy,
+                        // which is in range of no sink
+                        const int4 sv = sbuf[e < nbuf ? e : LFAR];
+                        const float fx = (float)sv.x, fy = (float)sv.y, fz = (float)sv.z;
+                        // sink k's position is read from LDS in its own step
+                        // (48 positions kept live would cost occupancy); its
+                        // count lives in lane k of cntv
+                        auto sink_test = [&](auto kc) {
+                            constexpr int k = decltype(kc)::value;
+                            const float4 sk = sinkf[k];
+                            const float dx = fx - sk.x, dy = fy - sk.y, dz = fz - sk.z;
+                            const bool in = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) < cull2s;
+                            const unsigned long long m = __ballot(in);
+                            const int base = __builtin_amdgcn_readlane(cntv, k);
+                            if (in) lst[k * LCAP + base + (int)count_below(m)] = (uint8_t)e;
+                            const int nb = base + __popcll(m);
+                            asm volatile("v_writelane_b32 %0, %1, %2" : "+v"(cntv) : "s"(nb), "i"(k));
+                        };
+                        static_for<16>(sink_test);
+                    }
+                    lds_sync();
+                    const int mycnt = __shfl(cntv, slot);
+                    const int mx = wave_max(cntv);
+                    int tot = cntv;
+#pragma unroll
+                    for (int o = 1; o < 16; o <<= 1) tot += __shfl_xor(tot, o);
+                    tot = __builtin_amdgcn_readfirstlane(tot);
+                    // stream s of sink k takes entries s, s + 4, ... of list k;
+                    // the index read one step ahead stays inside the lists'
+                    // slack (4 * ceil(LCAP / 4) + 3 < LCAP + 16)
+                    const uint8_t* my = lst + slot * LCAP + stream;
+                    const int steps = (mx + 3) >> 2;
+                    int pidx = stream;
+                    int4 sc = sbuf[pidx < mycnt ? (int)my[0] : LFAR];
 #pragma unroll 2
-                    for (int qq = 0; qq < padded; qq += SPLIT) {
+                    for (int i = 0; i < steps; i++) {
+                        const int pn = pidx + 4;
+                        const int4 sn = sbuf[pn < mycnt ? (int)my[4 * i + 4] : LFAR];
+                        pp_pair<PME>(B, ox, oy, oz, sc, gx, gy, gz, ph);
+                        sc = sn;
+                        pidx = pn;
+                    }
+                    executed += (unsigned long long)tot;
+                    nbuf = 0;
+                    lds_sync();
+                };
+                auto flush = [&]() {
+                    if constexpr (LISTS) {
+                        if (cull) {
+                            flush_lists();
+                            return;
+                        }
+                    }
+                    // whole rounds of SPLIT sources, then the streams that
+                    // still have one (no padding source: with C < 8 a
+                    // stand-in placed anywhere can come within rcut of some
+                    // sink, and kick_pme's law has no cutoff at all)
+                    const int full = nbuf / SPLIT * SPLIT;
+                    lds_sync();
+                    int4 s = sbuf[stream];
+                    // the read one round ahead stays below full + 2 SPLIT <=
+                    // FLUSH_AT + 64 + SPLIT < NSB (and below LFAR with lists);
+                    // what it reads past nbuf is not used
+#pragma unroll 2
+                    for (int qq = 0; qq < full; qq += SPLIT) {
                         const int4 sn = sbuf[qq + SPLIT + stream];
                         pp_pair<PME>(B, ox, oy, oz, s, gx, gy, gz, ph);
                         s = sn;
                     }
-                    staged += (unsigned long long)nbuf;
+                    if (full + stream < nbuf) pp_pair<PME>(B, ox, oy, oz, s, gx, gy, gz, ph);
+                    executed += (unsigned long long)nbuf * (unsigned long long)nact;
                     nbuf = 0;
-                    __syncthreads();
+                    lds_sync();
                 };
                 for (int nb = 0; nb < 27; nb++) {
                     int jx = cx + (nb % 3) - 1, jy = cy + (nb / 3) % 3 - 1, jz = cz + nb / 9 - 1;
@@ -217,7 +352,7 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) 
                         const unsigned long long m = __ballot(keep);
                         if (keep) sbuf[nbuf + (int)count_below(m)] = sv;
                         nbuf += __popcll(m);
-                        if (nbuf > SBUF - 2 * 64) flush();
+                        if (nbuf > FLUSH_AT) flush();
                     }
                 }
                 if (nbuf > 0) flush();
@@ -229,9 +364,7 @@ __global__ __launch_bounds__(64, PERSIST ? 1 : 8) void pp_kick_kernel(PPArgs a) 
                     gz += __shfl_xor(gz, o);
                     ph += __shfl_xor(ph, o);
                 }
-                if (lane == 0 && cnt)
-                    atomicAdd(&cnt[0],
-                              staged * (unsigned long long)__popcll(amask & ((TS == 64) ? ~0ull : ((1ull << TS) - 1))));
+                if (lane == 0 && cnt) atomicAdd(&cnt[0], executed);
                 // back to box units: g in 2^-64, phi in 2^-32
                 gx *= 0x1p64f;
                 gy *= 0x1p64f;
@@ -342,7 +475,19 @@ __global__ void drift_kernel(int64_t n, int nbox, const tpm_box_dev* __restrict_
 }
 
 template <bool BATCH>
-static void pp_launch(int ts, bool persistent, unsigned grid, hipStream_t s, const PPArgs& a, bool pme = false) {
+// grid = workgroups (a multiple of 8); wpg = tiles per workgroup (1 or 4;
+// the persistent and kick_pme launches take one).
+static void pp_launch(int ts, bool persistent, unsigned grid, hipStream_t s, const PPArgs& a, bool pme = false,
+                      int wpg = 1, bool lists = false) {
+    if (lists && ts == 16 && !pme) {
+        if (persistent)
+            pp_kick_kernel<16, BATCH, true, false, 1, true><<<grid, 64, 0, s>>>(a);
+        else if (wpg == 4)
+            pp_kick_kernel<16, BATCH, false, false, 4, true><<<grid, 256, 0, s>>>(a);
+        else
+            pp_kick_kernel<16, BATCH, false, false, 1, true><<<grid, 64, 0, s>>>(a);
+        return;
+    }
     if (pme) {
         switch (ts) {
             case 8: pp_kick_kernel<8, BATCH, false, true><<<grid, 64, 0, s>>>(a); break;
@@ -356,6 +501,13 @@ static void pp_launch(int ts, bool persistent, unsigned grid, hipStream_t s, con
             case 16: pp_kick_kernel<16, BATCH, true><<<grid, 64, 0, s>>>(a); break;
             case 64: pp_kick_kernel<64, BATCH, true><<<grid, 64, 0, s>>>(a); break;
             default: pp_kick_kernel<32, BATCH, true><<<grid, 64, 0, s>>>(a); break;
+        }
+    } else if (wpg == 4) {
+        switch (ts) {
+            case 8: pp_kick_kernel<8, BATCH, false, false, 4><<<grid, 256, 0, s>>>(a); break;
+            case 16: pp_kick_kernel<16, BATCH, false, false, 4><<<grid, 256, 0, s>>>(a); break;
+            case 64: pp_kick_kernel<64, BATCH, false, false, 4><<<grid, 256, 0, s>>>(a); break;
+            default: pp_kick_kernel<32, BATCH, false, false, 4><<<grid, 256, 0, s>>>(a); break;
         }
     } else {
         switch (ts) {
@@ -474,7 +626,10 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     }
     a.flags = flags & (TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK);
 
-    const int64_t blocks = (tiles_max + 7) / 8 * 8;
+    // tiles per workgroup (kick_pme's launch: one)
+    const int wpg = pme ? 1 : ctx->pp_waves;
+    const int64_t per_xcd = (tiles_max + 7) / 8;
+    const int64_t blocks = 8 * ((per_xcd + wpg - 1) / wpg);
     if (blocks > INT_MAX) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: too many tiles");
     if (!ctx->ev[0]) {
         for (int e = 0; e < 8; e++) TPM_HIP(ctx, hipEventCreate(&ctx->ev[e]));
@@ -488,7 +643,7 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     }
     TPM_HIP(ctx, hipEventRecord(e0, s));
     if (ctx->n > 0 && blocks > 0) {
-        pp_launch<false>(ctx->tile, false, (unsigned)blocks, s, a, pme);
+        pp_launch<false>(ctx->tile, false, (unsigned)blocks, s, a, pme, wpg, ctx->pp_lists != 0);
         TPM_HIP(ctx, hipGetLastError());
     }
     TPM_HIP(ctx, hipEventRecord(e1, s));
@@ -555,7 +710,7 @@ tpm_status tpm_batch_pp(tpm_batch* bt, const tpm_batch_layout& L, unsigned flags
     int64_t grid = persistent ? bt->persist_grid : (L.tiles_max + 7) / 8 * 8;
     if (grid < 8) grid = 8;
     if (grid > INT_MAX) return tpm_fail(bt, TPM_ERR_INVALID, "tpm_batch_step: too many tiles");
-    pp_launch<true>(bt->tune.tile_sinks, persistent, (unsigned)grid, bt->stream, a);
+    pp_launch<true>(bt->tune.tile_sinks, persistent, (unsigned)grid, bt->stream, a, false, 1, bt->tune.pp_lists != 0);
     TPM_HIP(bt, hipGetLastError());
     return TPM_OK;
 }
@@ -578,11 +733,17 @@ tpm_status tpm_batch_drift(tpm_batch* bt, const tpm_batch_layout& L) {
 
 // Workgroups of one persistent PP launch: resident workgroups per CU x CUs,
 // a multiple of the 8 XCDs.
-int tpm_pp_persistent_grid(int device, int ts) {
+int tpm_pp_persistent_grid(int device, int ts, bool lists) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
     int per_cu = 0;
     hipError_t e;
+    if (lists && ts == 16) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<16, true, true, false, 1, true>, 64, 0);
+        if (e != hipSuccess || per_cu < 1) per_cu = 8;
+        const int g = per_cu * prop.multiProcessorCount;
+        return (g + 7) / 8 * 8;
+    }
     switch (ts) {
         case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<8, true, true>, 64, 0); break;
         case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pp_kick_kernel<16, true, true>, 64, 0); break;

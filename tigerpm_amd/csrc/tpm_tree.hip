@@ -792,9 +792,8 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
         const int cx = bk.cell % C, cy = (bk.cell / C) % C, cz = bk.cell / (C * C);
         const float rcut = a.rcut, t2 = a.theta2inv;
         const float sr = bk.radius;
-        // Far dummy that pads a compacted list to a multiple of the stream
-        // count: r >= rcut from every sink.
-        const int4 far = make_int4(0x40000000, 0x40000000, 0x40000000, 0);
+        // Placeholder of lanes past the staged sources (never kept).
+        const int4 far = make_int4(INT_MIN, INT_MIN, INT_MIN, 0);
         // the 27 neighbour trees (x outermost, z innermost, kick_treepm.cu:930-940)
         if (lane < 27) {
             int jx = cx + lane / 9 - 1, jy = cy + (lane / 3) % 3 - 1, jz = cz + lane % 3 - 1;
@@ -897,17 +896,21 @@ __device__ __forceinline__ void walk_bucket(const WalkArgs& a, const tpm_tree_bu
                     }
                     const bool hb = HALF && k == K - 1;
                     const int S = hb ? 4 : 2, st = hb ? s4 : stream;  // source streams
-                    const int padded = (m + S - 1) / S * S;
-                    if (lane < padded - m) sh.cbuf[m + lane] = far;
+                    // whole rounds of S sources, then the streams that still
+                    // have one (no padding source: with a small chain mesh
+                    // a stand-in placed anywhere can come within rcut)
+                    const int full = m / S * S;
                     __syncthreads();
                     // one source ahead (the read past the end stays below
-                    // padded + 4 <= WALK_SBUF + 4 and is not used)
+                    // m + 4 <= WALK_SBUF + 4 and is not used)
                     int4 sv = sh.cbuf[st];
-                    for (int j = st; j < padded; j += S) {
+                    for (int j = st; j < full; j += S) {
                         const int4 sn = sh.cbuf[j + S];
                         pp_pair<false>(B, ox[k], oy[k], oz[k], sv, pg[k][0], pg[k][1], pg[k][2], pg[k][3]);
                         sv = sn;
                     }
+                    if (full + st < m)
+                        pp_pair<false>(B, ox[k], oy[k], oz[k], sv, pg[k][0], pg[k][1], pg[k][2], pg[k][3]);
                     __syncthreads();
                 }
                 npairs += (unsigned long long)nbuf * nact;
