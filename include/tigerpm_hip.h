@@ -135,6 +135,11 @@ typedef struct {
                       evaluates only the staged sources within the cull
                       distance of itself, not of the tile's box), 0 = one
                       list per tile, -1 = default */
+    int pp_newton; /* whole-box kicks with every particle active and
+                      chain_dim >= 8: 1 = each pair evaluated once, its
+                      force and potential added to both ends (an extra
+                      240 B per particle of device memory), 0 = from each
+                      end as the reference does, -1 = default */
 } tpm_tuning;
 
 /* Context lifetime.  device = HIP device ordinal. */

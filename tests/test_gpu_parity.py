@@ -460,6 +460,46 @@ def test_sparse_cells_on_a_small_chain_mesh(gpu_ctx, lists):
     assert err.max() < 1e-4
 
 
+@pytest.mark.parametrize("dist,n_side", [("uniform", 48), ("clustered", 48), ("clustered", 64)])
+def test_pairs_once_equal_pairs_from_both_ends(gpu_ctx, dist, n_side):
+    """pp_newton: each pair evaluated once and added to both ends.  Fewer
+    pair evaluations (32-source tiles cull more loosely than 16-sink ones,
+    so not half); forces, potentials, velocities and rungs equal the
+    two-sided kernel's up to FP32 summation order; the whole kick within
+    1e-5 of the oracle; a repeated kick gives the same bits.  The clustered
+    boxes put thousands of particles in one cell, so cells are split into
+    several LDS chunks."""
+    n = n_side ** 3
+    x, y, z = uniform(n, 71) if dist == "uniform" else clustered(n, 72)
+    prev = gpu_ctx.get_tuning()
+    out = []
+    for newton in (0, 1, 1):
+        p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, 3))
+        assert p.chain_dim >= 8
+        gpu_ctx.set_tuning(tile_sinks=16, pp_newton=newton)
+        gpu_ctx.upload(x, y, z)
+        print("pp_newton", newton, dist, n_side, flush=True)
+        out.append((gpu_ctx.kick_treepm(0, 1.0, 1.0, True), gpu_ctx.download()))
+    gpu_ctx.set_tuning(**prev)
+    (s0, d0), (s1, d1), (s2, d2) = out
+    print("pair evaluations: both ends %.4g, once %.4g" % (s0.executed_pairs, s1.executed_pairs))
+    assert s1.candidate_pairs == s0.candidate_pairs
+    assert s1.executed_pairs < 0.75 * s0.executed_pairs
+    for k in ("x", "gx", "gy", "gz", "pot", "vx", "vy", "vz", "rung"):
+        assert np.array_equal(d1[k], d2[k]), k
+    assert np.array_equal(d0["x"], d1["x"])
+    G0 = np.stack([d0["gx"], d0["gy"], d0["gz"]], 1)
+    G1 = np.stack([d1["gx"], d1["gy"], d1["gz"]], 1)
+    assert rel_rms(G1, G0) < 5e-6
+    assert rel_rms(d1["pot"], d0["pot"]) < 5e-6
+    assert (d0["rung"] == d1["rung"]).mean() > 0.999
+    assert s0.max_rung == s1.max_rung
+    zf = np.zeros(n, np.float32)
+    r = O.kick_full(oparams(p), x, y, z, zf, zf, zf, np.zeros(n, np.int8))
+    assert rel_rms(G1, np.stack([r["gx"], r["gy"], r["gz"]], 1)) < 1e-5
+    assert rel_rms(d1["pot"], r["pot"]) < 1e-5
+
+
 def test_pair_rate_probe(gpu_ctx):
     """The pair-body probe runs with the current parameters and reports a
     rate; out-of-range occupancies and missing parameters are errors."""

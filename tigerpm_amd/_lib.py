@@ -67,7 +67,7 @@ class TpmStats(ctypes.Structure):
 
 class TpmTuning(ctypes.Structure):
     _fields_ = [("tile_sinks", ctypes.c_int), ("deposit_block", ctypes.c_int), ("pp_waves", ctypes.c_int),
-                ("pp_lists", ctypes.c_int)]
+                ("pp_lists", ctypes.c_int), ("pp_newton", ctypes.c_int)]
 
 
 TPM_BATCH_DRIFT = 8
@@ -260,15 +260,16 @@ class Context:
         self.params = q
         return q
 
-    def set_tuning(self, tile_sinks=16, deposit_block=0, pp_waves=0, pp_lists=-1):
+    def set_tuning(self, tile_sinks=16, deposit_block=0, pp_waves=0, pp_lists=-1, pp_newton=-1):
         self._chk(self._L.tpm_set_tuning(self._h, ctypes.byref(TpmTuning(int(tile_sinks), int(deposit_block),
-                                                                         int(pp_waves), int(pp_lists)))))
+                                                                         int(pp_waves), int(pp_lists),
+                                                                         int(pp_newton)))))
 
     def get_tuning(self):
         t = TpmTuning()
         self._chk(self._L.tpm_get_tuning(self._h, ctypes.byref(t)))
         return dict(tile_sinks=t.tile_sinks, deposit_block=t.deposit_block, pp_waves=t.pp_waves,
-                    pp_lists=t.pp_lists)
+                    pp_lists=t.pp_lists, pp_newton=t.pp_newton)
 
     def upload(self, x, y, z, vx=None, vy=None, vz=None, rung=None):
         x, y, z = [np.ascontiguousarray(a, np.uint32) for a in (x, y, z)]

@@ -54,7 +54,8 @@ extern "C" tpm_status tpm_ctx_destroy(tpm_ctx* ctx) {
                     ctx->tree_cnt, ctx->node_begin, ctx->bucket_begin, ctx->tree_part, ctx->tree_flags, ctx->nodes,
                     ctx->buckets, ctx->h2d, ctx->h2d2, ctx->ref_perm, ctx->ref_cid, ctx->ref_cid2, ctx->ref_idx2,
                     ctx->ref_posa, ctx->ref_posb, ctx->ref_inv, ctx->ref_flag, ctx->ref_scan, ctx->ref_scan_tmp,
-                    ctx->ref_tab, ctx->moment_buf, ctx->box_tiles, ctx->box_count, ctx->box_buckets, ctx->tree_moved};
+                    ctx->ref_tab, ctx->moment_buf, ctx->box_tiles, ctx->box_count, ctx->box_buckets, ctx->tree_moved,
+                    ctx->gsink, ctx->gsrc, ctx->gmesh};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ctx->ev)
@@ -129,10 +130,12 @@ extern "C" tpm_status tpm_set_tuning(tpm_ctx* ctx, const tpm_tuning* t) {
     if (t->pp_waves != 0 && t->pp_waves != 1 && t->pp_waves != 4)
         return tpm_fail(ctx, TPM_ERR_INVALID, "pp_waves must be 0, 1 or 4");
     if (t->pp_lists < -1 || t->pp_lists > 1) return tpm_fail(ctx, TPM_ERR_INVALID, "pp_lists must be -1, 0 or 1");
+    if (t->pp_newton < -1 || t->pp_newton > 1) return tpm_fail(ctx, TPM_ERR_INVALID, "pp_newton must be -1, 0 or 1");
     ctx->tile = t->tile_sinks;
     ctx->deposit_block = t->deposit_block;
     ctx->pp_waves = t->pp_waves == 0 ? TPM_PP_WAVES_DEFAULT : t->pp_waves;
     ctx->pp_lists = t->pp_lists < 0 ? TPM_PP_LISTS_DEFAULT : t->pp_lists;
+    ctx->pp_newton = t->pp_newton < 0 ? TPM_PP_NEWTON_DEFAULT : t->pp_newton;
     return TPM_OK;
 }
 
@@ -142,6 +145,7 @@ extern "C" tpm_status tpm_get_tuning(const tpm_ctx* ctx, tpm_tuning* t) {
     t->deposit_block = ctx->deposit_block;
     t->pp_waves = ctx->pp_waves;
     t->pp_lists = ctx->pp_lists;
+    t->pp_newton = ctx->pp_newton;
     return TPM_OK;
 }
 

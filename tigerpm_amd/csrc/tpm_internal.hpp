@@ -96,6 +96,9 @@ void tpm_box_fill(tpm_box_dev* b, const tpm_params& p, int min_rung, int first_c
 // Per-sink source lists in the PP kernel (16-sink tiles) unless the tuning
 // says otherwise, for single boxes and batches alike.
 #define TPM_PP_LISTS_DEFAULT 0
+// Whole-box kicks with every particle active evaluate each pair once
+// (tpm_pp_sym.hip) unless the tuning says otherwise.
+#define TPM_PP_NEWTON_DEFAULT 0
 
 struct tpm_ctx {
     int device = 0;
@@ -139,6 +142,13 @@ struct tpm_ctx {
     int deposit_block = 0;  // chain cells per deposit block edge; 0 auto, -1 per-particle atomics
     int pp_waves = TPM_PP_WAVES_DEFAULT;  // PP tiles per workgroup (1 or 4)
     int pp_lists = TPM_PP_LISTS_DEFAULT;  // per-sink source lists (16-sink tiles)
+    int pp_newton = TPM_PP_NEWTON_DEFAULT;  // whole-box kicks: each pair evaluated once
+    // one-sided pairs: sink sums [n], 13 slots of source shares [13 n],
+    // mesh force and potential [n]
+    void* gsink = nullptr;
+    void* gsrc = nullptr;
+    void* gmesh = nullptr;
+    int64_t sym_cap = 0;
     int64_t chain_epoch = 0;  // counts chain-mesh builds (trees remember the one they follow)
 
     // The order the caller sees (tpm_set_host_order).  With
@@ -495,6 +505,8 @@ tpm_status tpm_tree_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, in
 tpm_status tpm_pm_solve(tpm_ctx* ctx);
 // The direct kernel over every tile, or (box != nullptr) over the box_tiles
 // tiles of the box's cells (ntiles_box of them).
+bool tpm_pp_sym_eligible(const tpm_ctx* ctx, int min_rung, unsigned flags, const tpm_cell_box* box);
+tpm_status tpm_pp_kick_sym(tpm_ctx* ctx, double scale, double t0, int first_call, unsigned flags, tpm_stats* stats);
 tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                        tpm_stats* stats, const tpm_cell_box* box = nullptr, int64_t ntiles_box = 0);
 // Kick of the sinks in box (nullptr: all) on the direct or tree path,

@@ -567,6 +567,7 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     if (use_pm && (!ctx->pm_valid || ctx->green_M != ctx->M))
         return tpm_fail(ctx, TPM_ERR_STATE, "tpm_kick: PM mesh not computed (call tpm_gravity_long_compute)");
     if (t0 <= 0.0) return tpm_fail(ctx, TPM_ERR_INVALID, "tpm_kick: t0 must be positive");
+    if (tpm_pp_sym_eligible(ctx, min_rung, flags, box)) return tpm_pp_kick_sym(ctx, scale, t0, first_call, flags, stats);
     hipStream_t s = ctx->stream;
     if (!ctx->counters) TPM_HIP(ctx, hipMalloc(&ctx->counters, sizeof(unsigned long long) * TPM_COUNTER_WORDS));
     TPM_TRY(tpm_fill_u64(ctx, ctx->counters, TPM_COUNTER_WORDS));

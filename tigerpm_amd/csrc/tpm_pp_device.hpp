@@ -37,12 +37,14 @@ __device__ __forceinline__ int wave_max(int v) {
 // softening branch gives no force and exactly -35/16 h^-1 to phi.
 // PME = true: kick_pme_kernel's law instead (kick_pme.cu:280-310): no cutoff,
 // and inside h the plain softened polynomials without the (erfc - 1) terms.
+// pp_law gives the pair's float offset (sink - source) and its force and
+// potential factors; pp_pair adds them to the sink's sums.
 template <bool PME>
-__device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, int oz, int4 s, float& gx, float& gy,
-                                        float& gz, float& ph) {
-    const float dx = (float)(int)((unsigned)ox - (unsigned)s.x);
-    const float dy = (float)(int)((unsigned)oy - (unsigned)s.y);
-    const float dz = (float)(int)((unsigned)oz - (unsigned)s.z);
+__device__ __forceinline__ void pp_law(const tpm_box_dev& a, int ox, int oy, int oz, int4 s, float& dx, float& dy,
+                                       float& dz, float& rinv3, float& rinv1) {
+    dx = (float)(int)((unsigned)ox - (unsigned)s.x);
+    dy = (float)(int)((unsigned)oy - (unsigned)s.y);
+    dz = (float)(int)((unsigned)oz - (unsigned)s.z);
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     // Distinct particles differ by >= 1 raw unit, so r2 = 0 only for the
     // sink itself: there rinv = inf and every value below that depends on it
@@ -60,8 +62,8 @@ __device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, in
     const float er = e * rinv;
     // erfc(x) / r and (erfc(x) + 2 x / sqrt(pi) exp(-x^2)) / r^3, the latter
     // as (erfc(x) / r + (2 / sqrt(pi)) / (2 rs) exp(-x^2)) / r^2
-    float rinv1 = poly * er;
-    float rinv3 = fmaf(a.tsp_inv2rs_s, e, rinv1) * (rinv * rinv);
+    rinv1 = poly * er;
+    rinv3 = fmaf(a.tsp_inv2rs_s, e, rinv1) * (rinv * rinv);
     if (r2 <= a.h2s) {
         // Closer than h: Plummer polynomial plus the (erfc - 1) correction
         // (kick_treepm.cu:263-276).  At r = 0 (the sink itself) only d1
@@ -88,6 +90,13 @@ __device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, in
             rinv1 = fmaf(em1, ri, d0);
         }
     }
+}
+
+template <bool PME>
+__device__ __forceinline__ void pp_pair(const tpm_box_dev& a, int ox, int oy, int oz, int4 s, float& gx, float& gy,
+                                        float& gz, float& ph) {
+    float dx, dy, dz, rinv3, rinv1;
+    pp_law<PME>(a, ox, oy, oz, s, dx, dy, dz, rinv3, rinv1);
     gx = fmaf(-dx, rinv3, gx);
     gy = fmaf(-dy, rinv3, gy);
     gz = fmaf(-dz, rinv3, gz);
