@@ -500,6 +500,90 @@ def test_pairs_once_equal_pairs_from_both_ends(gpu_ctx, dist, n_side):
     assert rel_rms(d1["pot"], r["pot"]) < 1e-5
 
 
+def cells_with(counts, C, seed, background=0):
+    """Fixed-point positions with counts[k] particles inside chain cell
+    cells k = (x, y, z) (uniform in the cell), plus a uniform background."""
+    rng = np.random.default_rng(seed)
+    pos = [[], [], []]
+    for (cx, cy, cz), m in counts.items():
+        for d, c0 in enumerate((cx, cy, cz)):
+            pos[d].append((c0 + rng.random(m)) / C)
+    for d in range(3):
+        pos[d].append(rng.random(background))
+    return [np.minimum(np.floor(np.concatenate(p) * 2.0 ** 32), 2.0 ** 32 - 1).astype(np.uint32) for p in pos]
+
+
+@pytest.mark.parametrize("case", ["sparse", "chunks"])
+def test_pairs_once_edge_cases(gpu_ctx, case):
+    """pp_newton on sparse boxes (most cells empty or nearly so) and on
+    cells of exactly 255, 256, 257, 512 and 513 particles next to each
+    other (LDS chunks of 256 sinks); with the mesh off, with the kick off
+    and with both on, the pair-once kick equals the two-sided one to FP32
+    summation order.  A kick with min_rung > 0 is not eligible and runs the
+    two-sided kernel: the same bits."""
+    n_side = 48
+    C = T.baseline_params(n_side, n_side, 3).chain_dim
+    assert C == 8
+    if case == "sparse":
+        x, y, z = uniform(3000, 77)
+    else:
+        counts = {(1, 1, 1): 255, (2, 1, 1): 256, (2, 2, 1): 257, (1, 2, 2): 512, (2, 2, 2): 513, (5, 5, 5): 1}
+        x, y, z = cells_with(counts, C, 78, background=20000)
+    n = len(x)
+    prev = gpu_ctx.get_tuning()
+    p = gpu_ctx.set_params(T.baseline_params(n_side, n_side, 3, n_total=n))
+    rng = np.random.default_rng(79)
+    v0 = [rng.standard_normal(n).astype(np.float32) * 1e-3 for _ in range(3)]
+    r0 = rng.integers(0, 3, n).astype(np.int8)
+    operm, _ = O.chainmesh_sort(x, y, z, C)
+    out = {}
+    try:
+        for flags in (T.TPM_SKIP_PM, T.TPM_SKIP_KICK, 0):
+            for newton in (0, 1):
+                gpu_ctx.set_tuning(tile_sinks=16, pp_newton=newton)
+                gpu_ctx.upload(x, y, z, *v0, r0)
+                gpu_ctx.chainmesh_create()
+                gpu_ctx.gravity_long_compute()
+                print("pp_newton", newton, case, "flags", flags, flush=True)
+                st = gpu_ctx.kick(0, 1.0, 1.0, True, flags)
+                out[flags, newton] = (st, gpu_ctx.download())
+        for newton in (0, 1):
+            gpu_ctx.set_tuning(tile_sinks=16, pp_newton=newton)
+            gpu_ctx.upload(x, y, z, *v0, r0)
+            gpu_ctx.chainmesh_create()
+            gpu_ctx.gravity_long_compute()
+            print("pp_newton", newton, case, "min_rung 1", flush=True)
+            out["rung1", newton] = (gpu_ctx.kick(1, 1.0, 1.0, True, 0), gpu_ctx.download())
+    finally:
+        gpu_ctx.set_tuning(**prev)
+    for flags in (T.TPM_SKIP_PM, T.TPM_SKIP_KICK, 0):
+        (s0, d0), (s1, d1) = out[flags, 0], out[flags, 1]
+        assert s1.candidate_pairs == s0.candidate_pairs
+        assert s1.executed_pairs < s0.executed_pairs
+        assert np.array_equal(d0["x"], d1["x"])
+        G0 = np.stack([d0["gx"], d0["gy"], d0["gz"]], 1)
+        G1 = np.stack([d1["gx"], d1["gy"], d1["gz"]], 1)
+        assert rel_rms(G1, G0) < 5e-6, flags
+        assert rel_rms(d1["pot"], d0["pot"]) < 5e-6, flags
+        V0 = np.stack([d0["vx"], d0["vy"], d0["vz"]], 1)
+        V1 = np.stack([d1["vx"], d1["vy"], d1["vz"]], 1)
+        assert rel_rms(V1, V0) < 5e-6, flags
+        if flags & T.TPM_SKIP_KICK:
+            assert np.array_equal(d1["vx"], v0[0][operm]) and np.array_equal(d1["rung"], r0[operm])
+        assert (d0["rung"] == d1["rung"]).mean() > 0.999
+    (s0, d0), (s1, d1) = out["rung1", 0], out["rung1", 1]
+    for k in ("gx", "gy", "gz", "pot", "vx", "vy", "vz", "rung"):
+        assert np.array_equal(d0[k], d1[k]), k
+    assert s0.executed_pairs == s1.executed_pairs
+    # the whole kick against the oracle (positions in chain-mesh order)
+    d1 = out[0, 1][1]
+    r = O.kick_full(oparams(p), x, y, z, v0[0], v0[1], v0[2], r0)
+    G1 = np.stack([d1["gx"], d1["gy"], d1["gz"]], 1)
+    assert np.array_equal(d1["x"], r["x"])
+    assert rel_rms(G1, np.stack([r["gx"], r["gy"], r["gz"]], 1)) < 1e-5
+    assert rel_rms(d1["pot"], r["pot"]) < 1e-5
+
+
 def test_pair_rate_probe(gpu_ctx):
     """The pair-body probe runs with the current parameters and reports a
     rate; out-of-range occupancies and missing parameters are errors."""

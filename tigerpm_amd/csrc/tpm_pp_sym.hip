@@ -175,6 +175,16 @@ __global__ __launch_bounds__(64, WAVES) void pp_sym_kernel(SymArgs a) {
         return ((sym_wrap(cz + dz, C) * C) + sym_wrap(cy + dy, C)) * C + sym_wrap(cx + dx, C);
     };
     const int cb = a.cell_begin[c], ce = a.cell_begin[c + 1];
+    if (cb == ce) {
+        // No sinks, so no source tiles either: the slots this cell owns in
+        // its 13 forward neighbours still get their (zero) shares.
+        for (int k = 0; k < SYM_SELF; k++) {
+            const int ck = slot_cell(k);
+            const int b2 = a.cell_begin[ck], e2 = a.cell_begin[ck + 1];
+            for (int j = b2 + lane; j < e2; j += 64) a.gsrc[k * n + j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        return;
+    }
     const bool chunked = ce - cb > SMAX;
     if (chunked) {
         for (int j = cb + lane; j < ce; j += 64) a.gsink[j] = make_float4(0.f, 0.f, 0.f, 0.f);
