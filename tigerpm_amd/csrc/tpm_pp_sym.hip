@@ -19,8 +19,9 @@
 //    registers over the sinks, then over the groups, and stored once into
 //    slot k(d) of the source (for a source in c itself: into its LDS sum).
 //    Slot k(d) of a particle of cell c' is written by the wave of cell
-//    c' - d alone, so no two waves touch the same word, no sum is formed by
-//    atomics and nothing is read back;
+//    c' - d alone (with zeros when that cell is empty), so no two waves
+//    touch the same word, no sum is formed by atomics and nothing is read
+//    back;
 //  * the same wave interpolates the mesh force of its particles (the 6^3
 //    stencil split over 4 lanes per particle), so that its memory latency
 //    hides behind the pair work of other waves;
@@ -32,7 +33,7 @@
 // summation order.  Every particle must be active (min_rung 0): a pair is met
 // at one end only, so an inactive end could not be skipped.  Used for whole-
 // box kicks with chain_dim >= 8 (see the far sources below).
-#include <type_traits>
+#include <vector>
 
 #include "tpm_pp_device.hpp"
 
