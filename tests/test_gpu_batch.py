@@ -55,6 +55,36 @@ def test_batch_equals_plain_kick(lists):
             c.close()
 
 
+def test_sparse_box_in_a_batch():
+    """A box whose chain mesh is mostly empty (2000 particles, chain_dim 8)
+    beside a dense one, with the persistent and the plain schedule: each
+    equals its plain kick bit for bit over two steps."""
+    rng = np.random.default_rng(91)
+    n = 2000
+    x, y, z = [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    vx, vy, vz = [rng.standard_normal(n).astype(np.float32) * 1e-3 for _ in range(3)]
+    rung = rng.integers(0, 2, n).astype(np.int8)
+    sparse = T.box_spec(T.baseline_params(48, 48, 3, n_total=n, interlace=1), t0=T0)
+    for sched in (0, 1):
+        with new_batch() as bt:
+            bt.set_tuning(tile_sinks=16, sched=sched)
+            ids = [bt.add(sparse, x, y, z, vx, vy, vz, rung), bt.add_zeldovich(spec(24), 24, 5, 0.8)]
+            ctxs = []
+            for i in ids:
+                c = T.Context(0)
+                bt.to_ctx(i, c)
+                c.set_tuning(tile_sinks=16)
+                ctxs.append(c)
+            assert ctxs[0].params.chain_dim == 8
+            for k in range(2):
+                bt.step(0)
+                for i, c in zip(ids, ctxs):
+                    c.kick_treepm(min_rung=0, scale=1.0, t0=T0, first_call=(k == 0))
+                    assert_same(bt.download(i), c.download(), "sched %d box %d step %d" % (sched, i, k))
+            for c in ctxs:
+                c.close()
+
+
 def test_batch_drift_matches_oracle():
     """Kick skipped, drift on: positions move by round(v t0 2^32) exactly, in
     the chain-mesh order of the positions before the drift."""
