@@ -584,6 +584,56 @@ def test_pairs_once_edge_cases(gpu_ctx, case):
     assert rel_rms(d1["pot"], r["pot"]) < 1e-5
 
 
+@pytest.mark.parametrize("n", [1, 2, 17])
+def test_tiny_boxes_both_kernels(gpu_ctx, n):
+    """One, two and seventeen particles in a box with chain_dim 8 (all but
+    a few cells empty; n = 2 puts both in one cell, 0.013 h apart): the
+    short-range forces of the two-sided and the pair-once kernel equal the
+    FP64 pair law's (a single particle feels none, only its own potential
+    term).  Deep inside the softening length the law as written loses its
+    digits to cancellation in FP32 (the FP32 oracle is 0.7 % off there); the
+    kernels evaluate those terms by their series.  The mesh force is left
+    out: for a handful of particles it is mostly the assignment's
+    self-force, a residual of large cancelling terms that FP32 and FP64
+    FFTs resolve differently."""
+    rng = np.random.default_rng(100 + n)
+    if n == 2:
+        x = np.array([0x40000000, 0x40010000], np.uint32)
+        y = np.array([0x40000000, 0x40002000], np.uint32)
+        z = np.array([0x40000000, 0x3fff0000], np.uint32)
+    else:
+        x, y, z = [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) for _ in range(3)]
+    zf = np.zeros(n, np.float32)
+    prev = gpu_ctx.get_tuning()
+    try:
+        for newton in (0, 1):
+            p = gpu_ctx.set_params(T.baseline_params(48, 48, 3, n_total=n))
+            gpu_ctx.set_tuning(tile_sinks=16, pp_newton=newton)
+            gpu_ctx.upload(x, y, z, zf, zf, zf, np.zeros(n, np.int8))
+            gpu_ctx.chainmesh_create()
+            print("tiny box", n, "pp_newton", newton, flush=True)
+            gpu_ctx.kick(flags=T.TPM_SKIP_PM | T.TPM_SKIP_KICK)
+            d = gpu_ctx.download()
+            operm, ocb = O.chainmesh_sort(x, y, z, p.chain_dim)
+            assert np.array_equal(d["x"], x[operm])
+            xs, ys, zs = x[operm], y[operm], z[operm]
+            (gx, gy, gz, _), _ = O.pp_sum(oparams(p), xs, ys, zs, ocb, np.arange(n), fp64=True)
+            # the potential against the FP32 law, which shares the reference's
+            # own-term convention (at r = 0 no erfc correction; the FP64 law
+            # takes the r -> 0 limit there)
+            (_, _, _, pot), _ = O.pp_sum(oparams(p), xs, ys, zs, ocb, np.arange(n), fp64=False)
+            G = np.stack([d["gx"], d["gy"], d["gz"]], 1)
+            OG = np.stack([gx, gy, gz], 1) * p.GM
+            phi0 = p.n_total * 4 * np.pi * p.rs ** 2 + (35.0 / 16.0) / p.hsoft
+            if n == 1:
+                assert np.all(G == 0.0)
+            else:
+                assert rel_rms(G, OG) < 1e-5, newton
+            assert rel_rms(d["pot"], (pot + phi0) * p.GM) < 1e-5, newton
+    finally:
+        gpu_ctx.set_tuning(**prev)
+
+
 def test_pair_rate_probe(gpu_ctx):
     """The pair-body probe runs with the current parameters and reports a
     rate; out-of-range occupancies and missing parameters are errors."""

@@ -82,6 +82,9 @@ struct tpm_box_dev {
     float rcut2s, cull2s, inv2rs_s, tsp, h2s, h2inv_s, hinv_s, h3inv_s;
     float e2s;  // -log2(e) / (4 rs^2) in 2^-64 units: exp(-x^2) = exp2(r2 * e2s)
     float p_inv2rs_s, tsp_inv2rs_s;  // 0.3275911 / (2 rs) and 2 / sqrt(pi) / (2 rs), 2^-32 units
+    // inside the softening length: 1 / (2 rs)^2 in 2^-64 units (x^2 = r2 *
+    // inv4rs2_s) and 4 / (3 sqrt(pi)) / (2 rs)^3 in 2^-96 units
+    float inv4rs2_s, k3_s;
     // kick epilogue
     float GM, phi0, eta, t0, scale, hsoft;
     double drift;  // t0 of the drift step (batched path)

@@ -545,6 +545,8 @@ void tpm_box_fill(tpm_box_dev* b, const tpm_params& p, int min_rung, int first_c
     // A&S 7.1.26 p and 2/sqrt(pi), each times 1 / (2 rs) in 2^-32 units
     b->p_inv2rs_s = (float)(0.3275911 / (2.0 * p.rs) / 4294967296.0);
     b->tsp_inv2rs_s = (float)(2.0 / std::sqrt(M_PI) / (2.0 * p.rs) / 4294967296.0);
+    b->inv4rs2_s = (float)(1.0 / (4.0 * p.rs * p.rs) / (4294967296.0 * 4294967296.0));
+    b->k3_s = (float)(4.0 / (3.0 * std::sqrt(M_PI)) / std::pow(2.0 * p.rs * 4294967296.0, 3));
     b->h2s = (h * h) * (S * S);
     const float hinv = 1.f / h;
     b->hinv_s = hinv / S;

@@ -86,8 +86,19 @@ __device__ __forceinline__ void pp_law(const tpm_box_dev& a, int ox, int oy, int
             const float ri = pos ? rinv : 0.f;
             const float em1 = pos ? fmaf(poly, e, -1.0f) : 0.f;
             const float tre = pos ? a.tsp_inv2rs_s * r * e : 0.f;
-            rinv3 = fmaf((em1 + tre) * ri * ri, ri, d1);
-            rinv1 = fmaf(em1, ri, d0);
+            // (erfc(x) - 1) / r and (erfc(x) - 1 + 2 x e^{-x^2} / sqrt(pi)) / r^3
+            // are differences of O(1) terms that cancel to O(x) and O(x^3):
+            // at r << h the formula as written keeps no digits in FP32 (1 %
+            // off at r = 0.013 h).  For x^2 = u < 0.04 their Taylor series are
+            // exact to 3e-8:  -(2 / sqrt(pi)) / (2 rs) (1 - u/3 + u^2/10 - u^3/42)
+            // and -(4 / (3 sqrt(pi))) / (2 rs)^3 (1 - 3u/5 + 3u^2/14 - u^3/18).
+            // At r = 0 (the sink itself) both terms stay zero, as written.
+            const float u = r2 * a.inv4rs2_s;
+            const float s1 = fmaf(fmaf(fmaf(-1.0f / 42.0f, u, 0.1f), u, -1.0f / 3.0f), u, 1.0f);
+            const float s3 = fmaf(fmaf(fmaf(-1.0f / 18.0f, u, 3.0f / 14.0f), u, -0.6f), u, 1.0f);
+            const bool ser = pos && u < 0.04f;
+            rinv3 = ser ? fmaf(-a.k3_s, s3, d1) : fmaf((em1 + tre) * ri * ri, ri, d1);
+            rinv1 = ser ? fmaf(-a.tsp_inv2rs_s, s1, d0) : fmaf(em1, ri, d0);
         }
     }
 }
