@@ -19,7 +19,10 @@
 // Pair distances are int32(sink - source) converted to float, which is the
 // reference's distance() (gravity_short.hpp:20-22) in units of 2^-32; every
 // constant is pre-scaled by the matching power of two so the FP32 arithmetic
-// is the reference's up to exact power-of-two factors.  The self pair is
+// is the reference's up to exact power-of-two factors, except that inside
+// the softening length the two (erfc - 1) corrections are evaluated by their
+// series in x^2 (pp_law), where the formula as written cancels to nothing in
+// FP32.  The self pair is
 // evaluated like any other source, as in the reference: inside the
 // softening branch it contributes -35/16 h^-1 to phi and nothing to g.
 #include <type_traits>
