@@ -400,6 +400,8 @@ def main():
     ap.add_argument("--pp-lists", type=int, default=-1, help="per-sink source lists (0, 1; -1: default)")
     ap.add_argument("--pp-newton", type=int, default=-1,
                     help="each pair evaluated once, both ends (0, 1; -1: default)")
+    ap.add_argument("--fft", type=int, default=0,
+                    help="mesh FFTs: 0 = LDS-tiled transform for power-of-two meshes, 1 = rocFFT")
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run one plain 1 GiB device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -420,7 +422,7 @@ def main():
     ctx = T.Context(0)
     p = ctx.set_params(T.baseline_params(w["n_side"], w["mesh"], w["order"], interlace=w["interlace"]))
     ctx.set_tuning(tile_sinks=args.tile_sinks, deposit_block=args.deposit_block, pp_waves=args.pp_waves,
-                   pp_lists=args.pp_lists, pp_newton=args.pp_newton)
+                   pp_lists=args.pp_lists, pp_newton=args.pp_newton, fft=args.fft)
     if args.only == "tree":
         if w["ic"] == "zeldovich":
             ctx.ic_zeldovich(w["n_side"], w["seed"], w["rms"], -2.0)

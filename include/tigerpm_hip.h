@@ -140,6 +140,10 @@ typedef struct {
                       force and potential added to both ends (an extra
                       240 B per particle of device memory), 0 = from each
                       end as the reference does, -1 = default */
+    int fft;       /* mesh FFTs: 0 = this library's LDS-tiled transform for
+                      power-of-two meshes 16..1024 and rocFFT otherwise,
+                      1 = rocFFT for every size.  Both agree to FP32
+                      rounding; a batch uses the default. */
 } tpm_tuning;
 
 /* Context lifetime.  device = HIP device ordinal. */
@@ -211,6 +215,14 @@ TPM_API tpm_status tpm_gravity_long_get_phi(tpm_ctx* ctx, float* phi);
 TPM_API tpm_status tpm_gravity_long_get_density(tpm_ctx* ctx, uint64_t* grid);
 /* The FP32 Green's table in use, M*M*(M/2+1) floats. */
 TPM_API tpm_status tpm_gravity_long_get_green(tpm_ctx* ctx, float* table);
+/* fft3d_execute / fft3d_inv_execute (fft.cpp:51-84) on host arrays, with
+ * the PM's transforms (the tuning's fft choice): r2c takes M^3 floats
+ * [x][y][z] z fastest and gives M*M*(M/2+1) complex values (interleaved
+ * float pairs), unnormalised; c2r the reverse, unnormalised (M^3 times the
+ * inverse).  4 <= M <= 2048.  Uses the context's mesh buffers: the PM mesh
+ * must be computed again afterwards. */
+TPM_API tpm_status tpm_fft3d_r2c(tpm_ctx* ctx, int M, const float* in, float* out);
+TPM_API tpm_status tpm_fft3d_c2r(tpm_ctx* ctx, int M, const float* in, float* out);
 
 /* kick_treepm / kick_pme kernel step (kick_treepm.cu:460-759,
  * kick_pme.cu:128-357): per active sink (rung >= min_rung) the 6-point mesh

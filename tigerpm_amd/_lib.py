@@ -67,7 +67,7 @@ class TpmStats(ctypes.Structure):
 
 class TpmTuning(ctypes.Structure):
     _fields_ = [("tile_sinks", ctypes.c_int), ("deposit_block", ctypes.c_int), ("pp_waves", ctypes.c_int),
-                ("pp_lists", ctypes.c_int), ("pp_newton", ctypes.c_int)]
+                ("pp_lists", ctypes.c_int), ("pp_newton", ctypes.c_int), ("fft", ctypes.c_int)]
 
 
 TPM_BATCH_DRIFT = 8
@@ -165,6 +165,8 @@ def lib():
         "tpm_gravity_long_get_phi": (_int, [vp, _f32p]),
         "tpm_gravity_long_get_density": (_int, [vp, _u64p]),
         "tpm_gravity_long_get_green": (_int, [vp, _f32p]),
+        "tpm_fft3d_r2c": (_int, [vp, _int, _f32p, _f32p]),
+        "tpm_fft3d_c2r": (_int, [vp, _int, _f32p, _f32p]),
         "tpm_kick": (_int, [vp, _int, _dbl, _dbl, _int, ctypes.c_uint, ctypes.POINTER(TpmStats)]),
         "tpm_kick_treepm": (_int, [vp, _int, _dbl, _dbl, _int, ctypes.POINTER(TpmStats)]),
         "tpm_tree_build": (_int, [vp]),
@@ -260,16 +262,16 @@ class Context:
         self.params = q
         return q
 
-    def set_tuning(self, tile_sinks=16, deposit_block=0, pp_waves=0, pp_lists=-1, pp_newton=-1):
+    def set_tuning(self, tile_sinks=16, deposit_block=0, pp_waves=0, pp_lists=-1, pp_newton=-1, fft=0):
         self._chk(self._L.tpm_set_tuning(self._h, ctypes.byref(TpmTuning(int(tile_sinks), int(deposit_block),
                                                                          int(pp_waves), int(pp_lists),
-                                                                         int(pp_newton)))))
+                                                                         int(pp_newton), int(fft)))))
 
     def get_tuning(self):
         t = TpmTuning()
         self._chk(self._L.tpm_get_tuning(self._h, ctypes.byref(t)))
         return dict(tile_sinks=t.tile_sinks, deposit_block=t.deposit_block, pp_waves=t.pp_waves,
-                    pp_lists=t.pp_lists, pp_newton=t.pp_newton)
+                    pp_lists=t.pp_lists, pp_newton=t.pp_newton, fft=t.fft)
 
     def upload(self, x, y, z, vx=None, vy=None, vz=None, rung=None):
         x, y, z = [np.ascontiguousarray(a, np.uint32) for a in (x, y, z)]
@@ -331,6 +333,22 @@ class Context:
         a = np.empty(M * M * (M // 2 + 1), np.float32)
         self._chk(self._L.tpm_gravity_long_get_green(self._h, a))
         return a
+
+    def fft3d_r2c(self, a):
+        """R2C of an (M, M, M) float32 mesh: (M, M, M//2+1) complex64, unnormalised."""
+        M = a.shape[0]
+        a = np.ascontiguousarray(a, np.float32)
+        out = np.empty(M * M * (M // 2 + 1) * 2, np.float32)
+        self._chk(self._L.tpm_fft3d_r2c(self._h, M, a.ravel(), out))
+        return out.view(np.complex64).reshape(M, M, M // 2 + 1)
+
+    def fft3d_c2r(self, c):
+        """C2R of an (M, M, M//2+1) spectrum: (M, M, M) float32, unnormalised."""
+        M = c.shape[0]
+        c = np.ascontiguousarray(c, np.complex64)
+        out = np.empty(M ** 3, np.float32)
+        self._chk(self._L.tpm_fft3d_c2r(self._h, M, c.view(np.float32).ravel(), out))
+        return out.reshape(M, M, M)
 
     def kick(self, min_rung=0, scale=1.0, t0=1.0, first_call=True, flags=0):
         st = TpmStats()

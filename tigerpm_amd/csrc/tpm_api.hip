@@ -131,11 +131,16 @@ extern "C" tpm_status tpm_set_tuning(tpm_ctx* ctx, const tpm_tuning* t) {
         return tpm_fail(ctx, TPM_ERR_INVALID, "pp_waves must be 0, 1 or 4");
     if (t->pp_lists < -1 || t->pp_lists > 1) return tpm_fail(ctx, TPM_ERR_INVALID, "pp_lists must be -1, 0 or 1");
     if (t->pp_newton < -1 || t->pp_newton > 1) return tpm_fail(ctx, TPM_ERR_INVALID, "pp_newton must be -1, 0 or 1");
+    if (t->fft < 0 || t->fft > 1) return tpm_fail(ctx, TPM_ERR_INVALID, "fft must be 0 or 1");
     ctx->tile = t->tile_sinks;
     ctx->deposit_block = t->deposit_block;
     ctx->pp_waves = t->pp_waves == 0 ? TPM_PP_WAVES_DEFAULT : t->pp_waves;
     ctx->pp_lists = t->pp_lists < 0 ? TPM_PP_LISTS_DEFAULT : t->pp_lists;
     ctx->pp_newton = t->pp_newton < 0 ? TPM_PP_NEWTON_DEFAULT : t->pp_newton;
+    if (ctx->fft_rocfft != t->fft) {
+        ctx->fft_rocfft = t->fft;
+        ctx->pm_valid = false;
+    }
     return TPM_OK;
 }
 
@@ -146,6 +151,7 @@ extern "C" tpm_status tpm_get_tuning(const tpm_ctx* ctx, tpm_tuning* t) {
     t->pp_waves = ctx->pp_waves;
     t->pp_lists = ctx->pp_lists;
     t->pp_newton = ctx->pp_newton;
+    t->fft = ctx->fft_rocfft;
     return TPM_OK;
 }
 
@@ -351,6 +357,36 @@ extern "C" tpm_status tpm_gravity_long_get_green(tpm_ctx* ctx, float* table) {
     (void)hipSetDevice(ctx->device);
     const int64_t M = ctx->M;
     return copy_mesh(ctx, table, ctx->green, sizeof(float) * M * M * (M / 2 + 1));
+}
+
+// One 3-D transform through the context's mesh buffers: real data in rho,
+// spectra in rho_hat (fft.cpp:51-84).
+static tpm_status fft3d_host(tpm_ctx* ctx, int M, bool inverse, const float* in, float* out) {
+    if (!ctx || !in || !out || M < 4 || M > 2048) return TPM_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    ctx->pm_valid = false;
+    TPM_TRY(tpm_ensure_mesh(ctx, M));
+    const int64_t M3 = (int64_t)M * M * M, MC = (int64_t)M * M * (M / 2 + 1);
+    hipStream_t s = ctx->stream;
+    if (!inverse) {
+        TPM_HIP(ctx, hipMemcpyAsync(ctx->rho, in, sizeof(float) * M3, hipMemcpyHostToDevice, s));
+        TPM_TRY(tpm_fft_forward(ctx, ctx->rho, ctx->rho_hat));
+        TPM_HIP(ctx, hipMemcpyAsync(out, ctx->rho_hat, sizeof(float2) * MC, hipMemcpyDeviceToHost, s));
+    } else {
+        TPM_HIP(ctx, hipMemcpyAsync(ctx->rho_hat, in, sizeof(float2) * MC, hipMemcpyHostToDevice, s));
+        TPM_TRY(tpm_fft_inverse(ctx, ctx->rho_hat, ctx->rho));
+        TPM_HIP(ctx, hipMemcpyAsync(out, ctx->rho, sizeof(float) * M3, hipMemcpyDeviceToHost, s));
+    }
+    TPM_HIP(ctx, hipStreamSynchronize(s));
+    return TPM_OK;
+}
+
+extern "C" tpm_status tpm_fft3d_r2c(tpm_ctx* ctx, int M, const float* in, float* out) {
+    return fft3d_host(ctx, M, false, in, out);
+}
+
+extern "C" tpm_status tpm_fft3d_c2r(tpm_ctx* ctx, int M, const float* in, float* out) {
+    return fft3d_host(ctx, M, true, in, out);
 }
 
 // ---------------------------------------------------------------- box kicks

@@ -39,27 +39,40 @@ inline void tpm_reduce_counters(const unsigned long long* h, unsigned long long 
 
 #define TPM_MAX_CHAIN 406    // C^3 * 64 must fit the 32-bit sort key
 
-// rocFFT plans of one cubic size (single precision, out of place), shared by
-// every context and batch on the device: plans are made once per (device, M)
-// and kept until the process ends.
+// Plans of one cubic size (single precision, out of place), shared by every
+// context and batch on the device: made once per (device, M, backend) and
+// kept until the process ends.  Backend: rocFFT, or this library's LDS-tiled
+// transform (tpm_fft.hip) for power-of-two M in [16, 1024], whose plan is its
+// twiddle table.
 struct tpm_fft_plans {
     rocfft_plan fwd = nullptr;  // R2C
     rocfft_plan inv = nullptr;  // C2R
+    float2* tw = nullptr;       // LDS-tiled: e^{-2 pi i q / M}, q < M
     size_t work_bytes = 0;
 };
-// Returns nullptr (and sets *err) when rocFFT cannot plan the size.
-const tpm_fft_plans* tpm_fft_plans_get(int device, int M, std::string* err);
+// Returns nullptr (and sets *err) when the size cannot be planned.
+const tpm_fft_plans* tpm_fft_plans_get(int device, int M, bool lds, std::string* err);
 
 // What one caller needs to execute cached plans on its stream: execution
 // infos bound to the stream and a work buffer large enough for every plan it
-// has used.
+// has used.  The LDS-tiled transform's inverse overwrites its input, as
+// rocFFT's real inverse may.
 struct tpm_fft {
     int M = 0;
     const tpm_fft_plans* plans = nullptr;
     rocfft_execution_info info = nullptr;
+    hipStream_t stream = nullptr;
     void* work = nullptr;
     size_t work_bytes = 0;
 };
+bool tpm_lds_fft_size(int M);
+tpm_status tpm_lds_fft_twiddles(std::string& err, int M, float2** tf);
+// in: M^3 floats, or with q32 the int64 Q32 density grid; green: the PM's
+// Green's table applied to the spectrum as it is stored (nullptr: none).
+tpm_status tpm_lds_fft_r2c(std::string& err, hipStream_t st, int M, const void* in, bool q32, float2* out,
+                           float2* work, const float2* tf, const float* green);
+tpm_status tpm_lds_fft_c2r(std::string& err, hipStream_t st, int M, float2* in, float* out, float2* work,
+                           const float2* tf);
 
 // Per-box constants of the fused PP + interpolation + kick kernel, and the
 // box's place in the batched arenas.  One entry per box: the single-box path
@@ -146,6 +159,7 @@ struct tpm_ctx {
     int pp_waves = TPM_PP_WAVES_DEFAULT;  // PP tiles per workgroup (1 or 4)
     int pp_lists = TPM_PP_LISTS_DEFAULT;  // per-sink source lists (16-sink tiles)
     int pp_newton = TPM_PP_NEWTON_DEFAULT;  // whole-box kicks: each pair evaluated once
+    int fft_rocfft = 0;                     // 1: rocFFT for every mesh size
     // one-sided pairs: sink sums [n], 13 slots of source shares [13 n],
     // mesh force and potential [n]
     void* gsink = nullptr;
@@ -517,7 +531,7 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
 tpm_status tpm_kick_any(tpm_ctx* ctx, const tpm_cell_box* box, int min_rung, double scale, double t0,
                         int first_call, unsigned flags, tpm_stats* stats);
 tpm_status tpm_fft_plan(tpm_ctx* ctx, int M);
-tpm_status tpm_fft_use(std::string& err, tpm_fft& f, int device, hipStream_t s, int M);
+tpm_status tpm_fft_use(std::string& err, tpm_fft& f, int device, hipStream_t s, int M, bool lds = true);
 tpm_status tpm_fft_r2c(std::string& err, tpm_fft& f, float* in, float2* out);
 tpm_status tpm_fft_c2r(std::string& err, tpm_fft& f, float2* in, float* out);
 void tpm_fft_free(tpm_fft& f);

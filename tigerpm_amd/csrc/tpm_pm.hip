@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <tuple>
 
 #include "tpm_internal.hpp"
 
@@ -425,14 +426,26 @@ tpm_status tpm_fill_u64(tpm_ctx* ctx, unsigned long long* p, int64_t n) {
 // ---------------------------------------------------------------------------
 namespace {
 std::mutex g_plan_mu;
-std::map<std::pair<int, int>, tpm_fft_plans*> g_plans;
+std::map<std::tuple<int, int, bool>, tpm_fft_plans*> g_plans;
 }  // namespace
 
-const tpm_fft_plans* tpm_fft_plans_get(int device, int M, std::string* err) {
+const tpm_fft_plans* tpm_fft_plans_get(int device, int M, bool lds, std::string* err) {
+    lds = lds && tpm_lds_fft_size(M);
     std::lock_guard<std::mutex> lk(g_plan_mu);
-    auto it = g_plans.find({device, M});
+    auto it = g_plans.find({device, M, lds});
     if (it != g_plans.end()) return it->second;
     auto* pl = new tpm_fft_plans();
+    if (lds) {
+        std::string e;
+        if (tpm_lds_fft_twiddles(e, M, &pl->tw) != TPM_OK) {
+            delete pl;
+            if (err) *err = e;
+            return nullptr;
+        }
+        pl->work_bytes = sizeof(float2) * (size_t)M * M * (M / 2 + 1);
+        g_plans[{device, M, lds}] = pl;
+        return pl;
+    }
     const size_t len[3] = {(size_t)M, (size_t)M, (size_t)M};  // fastest first; cubic
     if (rocfft_plan_create(&pl->fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
                            rocfft_precision_single, 3, len, 1, nullptr) != rocfft_status_success ||
@@ -447,13 +460,14 @@ const tpm_fft_plans* tpm_fft_plans_get(int device, int M, std::string* err) {
     rocfft_plan_get_work_buffer_size(pl->fwd, &w1);
     rocfft_plan_get_work_buffer_size(pl->inv, &w2);
     pl->work_bytes = std::max(w1, w2);
-    g_plans[{device, M}] = pl;
+    g_plans[{device, M, lds}] = pl;
     return pl;
 }
 
-tpm_status tpm_fft_use(std::string& err, tpm_fft& f, int device, hipStream_t s, int M) {
-    if (f.M == M && f.plans) return TPM_OK;
-    const tpm_fft_plans* pl = tpm_fft_plans_get(device, M, &err);
+tpm_status tpm_fft_use(std::string& err, tpm_fft& f, int device, hipStream_t s, int M, bool lds) {
+    lds = lds && tpm_lds_fft_size(M);
+    if (f.M == M && f.plans && (f.plans->tw != nullptr) == lds) return TPM_OK;
+    const tpm_fft_plans* pl = tpm_fft_plans_get(device, M, lds, &err);
     if (!pl) return TPM_ERR_FFT;
     if (!f.info) {
         if (rocfft_execution_info_create(&f.info) != rocfft_status_success ||
@@ -462,6 +476,7 @@ tpm_status tpm_fft_use(std::string& err, tpm_fft& f, int device, hipStream_t s, 
             return TPM_ERR_FFT;
         }
     }
+    f.stream = s;
     if (pl->work_bytes > f.work_bytes) {
         if (f.work) TPM_HIPE(err, hipFree(f.work));
         f.work = nullptr;
@@ -478,6 +493,7 @@ tpm_status tpm_fft_use(std::string& err, tpm_fft& f, int device, hipStream_t s, 
 }
 
 tpm_status tpm_fft_r2c(std::string& err, tpm_fft& f, float* in, float2* out) {
+    if (f.plans->tw) return tpm_lds_fft_r2c(err, f.stream, f.M, in, false, out, (float2*)f.work, f.plans->tw, nullptr);
     void* ip[1] = {in};
     void* op[1] = {out};
     if (rocfft_execute(f.plans->fwd, ip, op, f.info) != rocfft_status_success) {
@@ -488,6 +504,7 @@ tpm_status tpm_fft_r2c(std::string& err, tpm_fft& f, float* in, float2* out) {
 }
 
 tpm_status tpm_fft_c2r(std::string& err, tpm_fft& f, float2* in, float* out) {
+    if (f.plans->tw) return tpm_lds_fft_c2r(err, f.stream, f.M, in, out, (float2*)f.work, f.plans->tw);
     void* ip[1] = {in};
     void* op[1] = {out};
     if (rocfft_execute(f.plans->inv, ip, op, f.info) != rocfft_status_success) {
@@ -503,7 +520,9 @@ void tpm_fft_free(tpm_fft& f) {
     f = tpm_fft{};
 }
 
-tpm_status tpm_fft_plan(tpm_ctx* ctx, int M) { return tpm_fft_use(ctx->err, ctx->fft, ctx->device, ctx->stream, M); }
+tpm_status tpm_fft_plan(tpm_ctx* ctx, int M) {
+    return tpm_fft_use(ctx->err, ctx->fft, ctx->device, ctx->stream, M, !ctx->fft_rocfft);
+}
 
 tpm_status tpm_fft_forward(tpm_ctx* ctx, float* in, float2* out) { return tpm_fft_r2c(ctx->err, ctx->fft, in, out); }
 
@@ -559,9 +578,9 @@ void tpm_deposit_blocking(int C, int M, int interlace, int want, int* Bout, int*
     }
 }
 
-// Integer deposit (grid displaced by shift / 2^32 cells) into rho_q and its
-// float image into rho.
-static tpm_status deposit_density(tpm_ctx* ctx, uint64_t shift) {
+// Integer deposit (grid displaced by shift / 2^32 cells) into rho_q and,
+// with to_float, its float image into rho.
+static tpm_status deposit_density(tpm_ctx* ctx, uint64_t shift, bool to_float = true) {
     const int M = ctx->M;
     const int64_t M3 = (int64_t)M * M * M;
     hipStream_t s = ctx->stream;
@@ -601,8 +620,10 @@ static tpm_status deposit_density(tpm_ctx* ctx, uint64_t shift) {
         }
         TPM_HIP(ctx, hipGetLastError());
     }
-    q32_to_float_kernel<<<grid_for(M3, 256), 256, 0, s>>>(M3, ctx->rho_q, ctx->rho);
-    TPM_HIP(ctx, hipGetLastError());
+    if (to_float) {
+        q32_to_float_kernel<<<grid_for(M3, 256), 256, 0, s>>>(M3, ctx->rho_q, ctx->rho);
+        TPM_HIP(ctx, hipGetLastError());
+    }
     return TPM_OK;
 }
 
@@ -620,6 +641,16 @@ tpm_status tpm_pm_solve(tpm_ctx* ctx) {
         ctx->green_M = M;
         ctx->green_rs = rs_eff;
         ctx->green_order = ctx->p.order;
+    }
+    if (ctx->fft.plans->tw && !ctx->p.interlace) {
+        // LDS-tiled transform: the Q32 conversion and the Green's multiply
+        // happen inside its first and last passes (same arithmetic)
+        TPM_TRY(deposit_density(ctx, 0, false));
+        TPM_TRY(tpm_lds_fft_r2c(ctx->err, s, M, ctx->rho_q, true, ctx->rho_hat, (float2*)ctx->fft.work,
+                                ctx->fft.plans->tw, ctx->green));
+        TPM_TRY(tpm_fft_inverse(ctx, ctx->rho_hat, ctx->phi));
+        ctx->pm_valid = true;
+        return TPM_OK;
     }
     TPM_TRY(deposit_density(ctx, 0));
     TPM_TRY(tpm_fft_forward(ctx, ctx->rho, ctx->rho_hat));
