@@ -122,7 +122,7 @@ typedef struct {
  *                  64/tile_sinks source streams of one culled source list);
  *   deposit_block: chain cells per edge of an LDS-privatised deposit block,
  *                  0 = the largest edge <= 3 whose int64 footprint fits
- *                  64 KiB, k > 0 = the largest edge <= k that fits a
+ *                  80 KiB, k > 0 = the largest edge <= k that fits a
  *                  workgroup's 160 KiB, -1 = per-particle global atomics.
  *                  The integer grid is the same bits for every choice. */
 typedef struct {

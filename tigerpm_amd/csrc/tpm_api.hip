@@ -30,6 +30,8 @@ extern "C" tpm_status tpm_ctx_create(int device, tpm_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return TPM_ERR_HIP;
     std::call_once(g_fft_once, []() { g_fft_ok = rocfft_setup() == rocfft_status_success; });
     if (!g_fft_ok) return TPM_ERR_FFT;
+    std::string err;
+    if (tpm_deposit_lds_setup(err) != TPM_OK) return TPM_ERR_HIP;
     tpm_ctx* ctx = new tpm_ctx();
     ctx->device = device;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {

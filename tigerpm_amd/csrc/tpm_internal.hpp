@@ -572,4 +572,11 @@ tpm_status tpm_green_table(std::string& err, hipStream_t s, int M, double rs, in
 // LDS of one workgroup on gfx950 (160 KiB), the cap of a requested
 // deposit block's footprint.
 #define TPM_DEPOSIT_LDS_MAX (160 * 1024)
+#define TPM_MAX_DEPOSIT_BLOCK 8       // chain cells per deposit block edge
+#define TPM_DEPOSIT_STATIC_LDS 1024   // the block's row table, beside its mesh nodes
+#ifndef TPM_DEPOSIT_THREADS
+#define TPM_DEPOSIT_THREADS 512       // per deposit block (workgroup)
+#endif
+#define TPM_DEPOSIT_LDS_AUTO (80 * 1024)  // automatic block edge: 2 workgroups per CU
+tpm_status tpm_deposit_lds_setup(std::string& err);  // allow the deposit kernels their LDS
 void tpm_deposit_blocking(int C, int M, int interlace, int want, int* B, int* wmax);

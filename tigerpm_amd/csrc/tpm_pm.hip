@@ -132,28 +132,43 @@ __device__ __forceinline__ void deposit_block(const uint32_t* __restrict__ x, co
     const int c1x = min(c0[0] + B, C) - 1, c1y = min(c0[1] + B, C) - 1, c1z = min(c0[2] + B, C) - 1;
     const int wy = w[1], wz = w[2];
     const int wtot = w[0] * wy * wz;
+    // the block's particles: the cells c0x..c1x of each (cy, cz) row are
+    // contiguous in memory; the rows are numbered into one index space so
+    // that every thread has work until the block's last particle
+    __shared__ int row_first[TPM_MAX_DEPOSIT_BLOCK * TPM_MAX_DEPOSIT_BLOCK];
+    __shared__ int row_pre[TPM_MAX_DEPOSIT_BLOCK * TPM_MAX_DEPOSIT_BLOCK + 1];
+    const int ny = c1y - c0[1] + 1, nrows = ny * (c1z - c0[2] + 1);
     for (int l = threadIdx.x; l < wtot; l += blockDim.x) sg[l] = 0ull;
-    __syncthreads();
-    for (int cz = c0[2]; cz <= c1z; cz++) {
-        for (int cy = c0[1]; cy <= c1y; cy++) {
-            // cells c0x..c1x of one (cy, cz) row are contiguous in memory
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int r = 0; r < nrows; r++) {
+            const int cz = c0[2] + r / ny, cy = c0[1] + r % ny;
             const int rb = cell_begin[cell0 + (cz * C + cy) * C + c0[0]];
             const int re = cell_begin[cell0 + (cz * C + cy) * C + c1x + 1];
-            for (int i = rb + threadIdx.x; i < re; i += blockDim.x) {
-                int bxn, byn, bzn;
-                uint64_t w3[NP * NP * NP];
-                weights_3d<ORDER>(x[i], y[i], z[i], M, shift, &bxn, &byn, &bzn, w3);
-                const int ox = bxn - lo[0], oy = byn - lo[1], oz = bzn - lo[2];
-#pragma unroll
-                for (int p = 0; p < NP; p++)
-#pragma unroll
-                    for (int r = 0; r < NP; r++)
-#pragma unroll
-                        for (int q = 0; q < NP; q++)
-                            atomicAdd(&sg[((ox + p) * wy + (oy + r)) * wz + (oz + q)],
-                                      (unsigned long long)w3[(p * NP + r) * NP + q]);
-            }
+            row_first[r] = rb;
+            row_pre[r] = acc;
+            acc += re - rb;
         }
+        row_pre[nrows] = acc;
+    }
+    __syncthreads();
+    const int total = row_pre[nrows];
+    int row = 0;
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        while (k >= row_pre[row + 1]) row++;
+        const int i = row_first[row] + (k - row_pre[row]);
+        int bxn, byn, bzn;
+        uint64_t w3[NP * NP * NP];
+        weights_3d<ORDER>(x[i], y[i], z[i], M, shift, &bxn, &byn, &bzn, w3);
+        const int ox = bxn - lo[0], oy = byn - lo[1], oz = bzn - lo[2];
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+#pragma unroll
+            for (int r = 0; r < NP; r++)
+#pragma unroll
+                for (int q = 0; q < NP; q++)
+                    atomicAdd(&sg[((ox + p) * wy + (oy + r)) * wz + (oz + q)],
+                              (unsigned long long)w3[(p * NP + r) * NP + q]);
     }
     __syncthreads();
     const int64_t M2 = (int64_t)M * M;
@@ -168,7 +183,7 @@ __device__ __forceinline__ void deposit_block(const uint32_t* __restrict__ x, co
 }
 
 template <int ORDER>
-__global__ __launch_bounds__(256) void deposit_blocked_kernel(BlockedDepositArgs a) {
+__global__ __launch_bounds__(TPM_DEPOSIT_THREADS) void deposit_blocked_kernel(BlockedDepositArgs a) {
     extern __shared__ unsigned long long sg[];
     const uint32_t* bounds = a.bounds;
     deposit_block<ORDER>(a.x, a.y, a.z, a.cell_begin, 0, a.grid, a.shift, a.C, a.M, a.B, a.nb, (int)blockIdx.x,
@@ -177,7 +192,7 @@ __global__ __launch_bounds__(256) void deposit_blocked_kernel(BlockedDepositArgs
 
 // Batched: work item -> (box, block) through the boxes' first items dep0.
 template <int ORDER>
-__global__ __launch_bounds__(256) void bdeposit_kernel(const uint32_t* __restrict__ x, const uint32_t* __restrict__ y,
+__global__ __launch_bounds__(TPM_DEPOSIT_THREADS) void bdeposit_kernel(const uint32_t* __restrict__ x, const uint32_t* __restrict__ y,
                                                        const uint32_t* __restrict__ z,
                                                        const int32_t* __restrict__ cell_begin, int nbox,
                                                        const tpm_box_dev* __restrict__ boxes,
@@ -305,8 +320,9 @@ tpm_status tpm_batch_fill_u64(tpm_batch* bt, unsigned long long* p, int64_t n) {
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel: the deposit
-// kernels may take up to TPM_DEPOSIT_LDS_MAX (once per process and device).
-static tpm_status deposit_lds_limit(std::string& err) {
+// kernels may take up to TPM_DEPOSIT_LDS_MAX (once per process and device;
+// every context does it when it is made, before any graph capture).
+tpm_status tpm_deposit_lds_setup(std::string& err) {
     static std::mutex mu;
     static bool done[64] = {};
     int dev = 0;
@@ -317,7 +333,8 @@ static tpm_status deposit_lds_limit(std::string& err) {
                         (const void*)deposit_blocked_kernel<4>, (const void*)bdeposit_kernel<2>,
                         (const void*)bdeposit_kernel<3>,        (const void*)bdeposit_kernel<4>};
     for (const void* f : k)
-        TPM_HIPE(err, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, TPM_DEPOSIT_LDS_MAX));
+        TPM_HIPE(err, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          TPM_DEPOSIT_LDS_MAX - TPM_DEPOSIT_STATIC_LDS));
     if (dev >= 0 && dev < 64) done[dev] = true;
     return TPM_OK;
 }
@@ -329,18 +346,17 @@ static tpm_status batch_deposit(tpm_batch* bt, const tpm_batch_layout& L, uint64
     TPM_TRY(tpm_batch_fill_u64(bt, bt->rho_q, L.mesh_hw));
     if (L.ndep > 0) {
         const size_t lds = (size_t)L.lds;
-        if (lds > 65536) TPM_TRY(deposit_lds_limit(bt->err));
         switch (L.order) {
             case 2:
-                bdeposit_kernel<2><<<(unsigned)L.ndep, 256, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
+                bdeposit_kernel<2><<<(unsigned)L.ndep, TPM_DEPOSIT_THREADS, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
                                                                        bt->d_boxes, bt->rho_q, shift);
                 break;
             case 3:
-                bdeposit_kernel<3><<<(unsigned)L.ndep, 256, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
+                bdeposit_kernel<3><<<(unsigned)L.ndep, TPM_DEPOSIT_THREADS, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
                                                                        bt->d_boxes, bt->rho_q, shift);
                 break;
             default:
-                bdeposit_kernel<4><<<(unsigned)L.ndep, 256, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
+                bdeposit_kernel<4><<<(unsigned)L.ndep, TPM_DEPOSIT_THREADS, lds, s>>>(bt->x, bt->y, bt->z, bt->cell_begin, L.nbox,
                                                                        bt->d_boxes, bt->rho_q, shift);
                 break;
         }
@@ -551,10 +567,12 @@ tpm_status tpm_ensure_mesh(tpm_ctx* ctx, int M) {
 
 void tpm_deposit_blocking(int C, int M, int interlace, int want, int* Bout, int* wout) {
     // the largest block edge (cells, <= want or 3) whose footprint of int64
-    // nodes fits the LDS of one workgroup: 64 KiB when chosen automatically
-    // (want = 0: several workgroups per CU), up to gfx950's 160 KiB when the
-    // caller asks for a block edge (fewer, larger blocks: less halo traffic)
-    const int64_t lds_cap = want > 0 ? TPM_DEPOSIT_LDS_MAX : 65536;
+    // nodes fits the LDS of one workgroup: 80 KiB when chosen automatically
+    // (want = 0: two workgroups per CU), up to gfx950's 160 KiB when the
+    // caller asks for a block edge (fewer, larger blocks: less halo traffic).
+    // At 512^3 (5.6 mesh cells per chain cell) the automatic edge is 3
+    // cells: 2.5 ms against 3.0 ms for 2-cell blocks and 2.9-3.2 ms for 4.
+    const int64_t lds_cap = (want > 0 ? TPM_DEPOSIT_LDS_MAX : TPM_DEPOSIT_LDS_AUTO) - TPM_DEPOSIT_STATIC_LDS;
     const double ninv = 1.0 / C;
     std::vector<uint64_t> b(C + 1);
     for (int k = 0; k < C; k++) b[k] = tpm_split_plane(k, ninv);
@@ -604,11 +622,10 @@ static tpm_status deposit_density(tpm_ctx* ctx, uint64_t shift, bool to_float = 
         da.nb = (ctx->C + B - 1) / B;
         const unsigned nblk = (unsigned)((int64_t)da.nb * da.nb * da.nb);
         const size_t lds = sizeof(unsigned long long) * (size_t)wmax * wmax * wmax;
-        if (lds > 65536) TPM_TRY(deposit_lds_limit(ctx->err));
         switch (ctx->p.order) {
-            case 2: deposit_blocked_kernel<2><<<nblk, 256, lds, s>>>(da); break;
-            case 3: deposit_blocked_kernel<3><<<nblk, 256, lds, s>>>(da); break;
-            default: deposit_blocked_kernel<4><<<nblk, 256, lds, s>>>(da); break;
+            case 2: deposit_blocked_kernel<2><<<nblk, TPM_DEPOSIT_THREADS, lds, s>>>(da); break;
+            case 3: deposit_blocked_kernel<3><<<nblk, TPM_DEPOSIT_THREADS, lds, s>>>(da); break;
+            default: deposit_blocked_kernel<4><<<nblk, TPM_DEPOSIT_THREADS, lds, s>>>(da); break;
         }
         TPM_HIP(ctx, hipGetLastError());
     } else if (ctx->n > 0) {
