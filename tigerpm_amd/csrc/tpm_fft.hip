@@ -359,6 +359,49 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_x_kernel(const float2* __rest
         });
 }
 
+// X, Green's multiply and X' in one pass, in place on B (the PM's Poisson
+// solve): the rows B[kz0 + t][ky][*] go forward along x, each mode (kx, ky,
+// kz0 + t) is multiplied by g (layout of S), and the rows go back.  The same
+// float operations as X, a Green's pass and X', without storing S.
+// Grid M * ceil(H / T).
+template <int M, int T>
+__global__ __launch_bounds__(FFT_THREADS) void fft_xgx_kernel(float2* __restrict__ b, const float2* __restrict__ tf,
+                                                             const float* __restrict__ g) {
+    constexpr int H = M / 2 + 1;
+    constexpr int NKT = (H + T - 1) / T;
+    constexpr int NU = T * M / FFT_THREADS;
+    static_assert(T * M % FFT_THREADS == 0, "whole tiles per thread");
+    __shared__ float2 s[T * M];
+    const int kz0 = (int)(blockIdx.x % NKT) * T;
+    const int64_t ky = blockIdx.x / NKT;
+    const int tn = min(T, H - kz0);
+    float gv[NU];
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+        const int e = (int)threadIdx.x + FFT_THREADS * u;
+        gv[u] = e % T < tn ? g[((int64_t)(e / T) * M + ky) * H + kz0 + e % T] : 0.f;
+    }
+    move_elems<T * M>(
+        [&](int e) {
+            return e / M < tn ? b[((int64_t)(kz0 + e / M) * M + ky) * M + e % M] : make_float2(0.f, 0.f);
+        },
+        [&](int e, float2 v) { s[lds_at<T>(e / M, e % M)] = v; });
+    __syncthreads();
+    fft_rows<M, T, -1, 1>(s, tf);
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+        const int e = (int)threadIdx.x + FFT_THREADS * u;
+        float2& v = s[lds_at<T>(e % T, e / T)];
+        v.x *= gv[u];
+        v.y *= gv[u];
+    }
+    __syncthreads();
+    fft_rows<M, T, 1, 1>(s, tf);
+    for_each_elem<T * M>([&](int e) {
+        if (e / M < tn) b[((int64_t)(kz0 + e / M) * M + ky) * M + e % M] = s[lds_at<T>(e / M, e % M)];
+    });
+}
+
 #ifndef TPM_FFT_TILE
 #define TPM_FFT_TILE 16
 #endif
@@ -385,6 +428,19 @@ void launch_c2r(hipStream_t st, float2* in, float* out, float2* work, const floa
     fft_x_kernel<M, T, false><<<M * ((H + T - 1) / T), FFT_THREADS, 0, st>>>(in, work, tf);
     fft_y_kernel<M, T, false><<<H * (M / T), FFT_THREADS, 0, st>>>(work, in, tf);
     fft_z_c2r_kernel<M, T><<<M * (M / T), FFT_THREADS, 0, st>>>(in, out, tf);
+}
+
+// The PM's Poisson solve: Q32 density -> phi through spec (A layout) and
+// work (B layout), both M * M * (M/2 + 1) complex.
+template <int M>
+void launch_pm(hipStream_t st, const unsigned long long* q, float* phi, float2* spec, float2* work,
+               const float2* tf, const float* green) {
+    constexpr int T = tile_of<M>(), H = M / 2 + 1;
+    fft_z_r2c_kernel<M, T, true><<<M * (M / T), FFT_THREADS, 0, st>>>(q, spec, tf);
+    fft_y_kernel<M, T, true><<<H * (M / T), FFT_THREADS, 0, st>>>(spec, work, tf);
+    fft_xgx_kernel<M, T><<<M * ((H + T - 1) / T), FFT_THREADS, 0, st>>>(work, tf, green);
+    fft_y_kernel<M, T, false><<<H * (M / T), FFT_THREADS, 0, st>>>(work, spec, tf);
+    fft_z_c2r_kernel<M, T><<<M * (M / T), FFT_THREADS, 0, st>>>(spec, phi, tf);
 }
 
 }  // namespace
@@ -428,6 +484,22 @@ tpm_status tpm_lds_fft_c2r(std::string& err, hipStream_t st, int M, float2* in, 
         case 256: launch_c2r<256>(st, in, out, work, tf); break;
         case 512: launch_c2r<512>(st, in, out, work, tf); break;
         case 1024: launch_c2r<1024>(st, in, out, work, tf); break;
+        default: err = "LDS FFT: unsupported size " + std::to_string(M); return TPM_ERR_FFT;
+    }
+    TPM_HIPE(err, hipGetLastError());
+    return TPM_OK;
+}
+
+tpm_status tpm_lds_pm_solve(std::string& err, hipStream_t st, int M, const unsigned long long* q, float* phi,
+                            float2* spec, float2* work, const float2* tf, const float* green) {
+    switch (M) {
+        case 16: launch_pm<16>(st, q, phi, spec, work, tf, green); break;
+        case 32: launch_pm<32>(st, q, phi, spec, work, tf, green); break;
+        case 64: launch_pm<64>(st, q, phi, spec, work, tf, green); break;
+        case 128: launch_pm<128>(st, q, phi, spec, work, tf, green); break;
+        case 256: launch_pm<256>(st, q, phi, spec, work, tf, green); break;
+        case 512: launch_pm<512>(st, q, phi, spec, work, tf, green); break;
+        case 1024: launch_pm<1024>(st, q, phi, spec, work, tf, green); break;
         default: err = "LDS FFT: unsupported size " + std::to_string(M); return TPM_ERR_FFT;
     }
     TPM_HIPE(err, hipGetLastError());

@@ -73,6 +73,11 @@ tpm_status tpm_lds_fft_r2c(std::string& err, hipStream_t st, int M, const void* 
                            float2* work, const float2* tf, const float* green);
 tpm_status tpm_lds_fft_c2r(std::string& err, hipStream_t st, int M, float2* in, float* out, float2* work,
                            const float2* tf);
+// The PM's Poisson solve on the LDS-tiled transform: Q32 density q ->
+// potential phi (Green's table green applied between the transforms); spec
+// and work are M * M * (M/2 + 1) complex scratch.
+tpm_status tpm_lds_pm_solve(std::string& err, hipStream_t st, int M, const unsigned long long* q, float* phi,
+                            float2* spec, float2* work, const float2* tf, const float* green);
 
 // Per-box constants of the fused PP + interpolation + kick kernel, and the
 // box's place in the batched arenas.  One entry per box: the single-box path

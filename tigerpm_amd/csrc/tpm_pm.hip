@@ -660,12 +660,12 @@ tpm_status tpm_pm_solve(tpm_ctx* ctx) {
         ctx->green_order = ctx->p.order;
     }
     if (ctx->fft.plans->tw && !ctx->p.interlace) {
-        // LDS-tiled transform: the Q32 conversion and the Green's multiply
-        // happen inside its first and last passes (same arithmetic)
+        // LDS-tiled transform: the Q32 conversion inside its first pass, the
+        // Green's multiply between the forward and inverse x transforms of
+        // one pass (same arithmetic as the separate kernels)
         TPM_TRY(deposit_density(ctx, 0, false));
-        TPM_TRY(tpm_lds_fft_r2c(ctx->err, s, M, ctx->rho_q, true, ctx->rho_hat, (float2*)ctx->fft.work,
-                                ctx->fft.plans->tw, ctx->green));
-        TPM_TRY(tpm_fft_inverse(ctx, ctx->rho_hat, ctx->phi));
+        TPM_TRY(tpm_lds_pm_solve(ctx->err, s, M, ctx->rho_q, ctx->phi, ctx->rho_hat, (float2*)ctx->fft.work,
+                                 ctx->fft.plans->tw, ctx->green));
         ctx->pm_valid = true;
         return TPM_OK;
     }
