@@ -330,6 +330,17 @@ __device__ __forceinline__ int tpm_chain_coord(uint32_t raw, int C, const uint32
 // One-axis integer mass-assignment weights (see the CPU restatement for the
 // derivation): CIC/TSC in Q21, cloud4 in Q16.  Returns the stencil width.
 // shift = 2^31 deposits onto the grid displaced by half a cell (interlacing).
+// Exact products on the 24-bit multipliers (v_mul_u32_u24 and
+// v_mul_hi_u32_u24 issue at full rate; 32-bit and 64-bit integer multiplies
+// at a quarter of it).  a, b < 2^24.
+__device__ __forceinline__ uint64_t tpm_mul24(uint32_t a, uint32_t b) {
+    return (uint64_t)(a & 0xFFFFFFu) * (uint64_t)(b & 0xFFFFFFu);
+}
+// x < 2^48, c < 2^24 and x c < 2^64: x c exactly.
+__device__ __forceinline__ uint64_t tpm_mul48x24(uint64_t x, uint32_t c) {
+    return tpm_mul24((uint32_t)x, c) + (tpm_mul24((uint32_t)(x >> 24), c) << 24);
+}
+
 __device__ __forceinline__ int tpm_weights_1d(int order, uint32_t raw, int M, uint64_t shift, int* base,
                                               uint64_t w[4], int* qbits) {
     const uint64_t P = (uint64_t)raw * (uint64_t)M + shift;
@@ -347,8 +358,8 @@ __device__ __forceinline__ int tpm_weights_1d(int order, uint32_t raw, int M, ui
         const int64_t hp = 1ll << 20;
         const uint64_t a = (uint64_t)(hp - ds);
         const uint64_t c = (uint64_t)(hp + ds);
-        const uint64_t wm = (a * a) >> 22;
-        const uint64_t wp = (c * c) >> 22;
+        const uint64_t wm = tpm_mul24((uint32_t)a, (uint32_t)a) >> 22;  // a, c <= 2^21
+        const uint64_t wp = tpm_mul24((uint32_t)c, (uint32_t)c) >> 22;
         *base = (int)j0 - 1;
         w[0] = wm;
         w[1] = (1ull << 21) - wm - wp;

@@ -44,16 +44,20 @@ __device__ __forceinline__ void weights_3d(uint32_t x, uint32_t y, uint32_t z, i
     tpm_weights_1d(ORDER, z, M, shift, bz, wz, &q);
     const int s = 3 * q - 32;
     uint64_t sum = 0;
+    // 1-D weights are at most 2^q <= 2^21, so the products are exact on the
+    // 24-bit multipliers
 #pragma unroll
     for (int a = 0; a < NP; a++)
 #pragma unroll
-        for (int b = 0; b < NP; b++)
+        for (int b = 0; b < NP; b++) {
+            const uint64_t xy = tpm_mul24((uint32_t)wx[a], (uint32_t)wy[b]);
 #pragma unroll
             for (int c = 0; c < NP; c++) {
-                const uint64_t v = (wx[a] * wy[b] * wz[c]) >> s;
+                const uint64_t v = tpm_mul48x24(xy, (uint32_t)wz[c]) >> s;
                 w3[(a * NP + b) * NP + c] = v;
                 sum += v;
             }
+        }
     constexpr int centre = (ORDER == 2) ? 0 : ((1 * NP + 1) * NP + 1);
     w3[centre] += (1ull << 32) - sum;
 }
