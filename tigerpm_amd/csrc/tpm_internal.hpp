@@ -367,14 +367,19 @@ __device__ __forceinline__ int tpm_weights_1d(int order, uint32_t raw, int M, ui
         *qbits = 21;
         return 3;
     } else {
-        const int64_t t = (int64_t)((uint32_t)P >> 16);
-        const int64_t u = 65536 - t;
-        const int64_t one = 1ll << 16;
-        w[0] = (uint64_t)((u * u * u / 6) >> 32);
-        w[3] = (uint64_t)((t * t * t / 6) >> 32);
-        const int64_t num = 4 * one * one * one - 6 * one * t * t + 3 * t * t * t;
-        w[1] = (uint64_t)((num / 6) >> 32);
-        w[2] = (uint64_t)one - w[0] - w[1] - w[3];
+        // floor(floor(a / 6) / 2^32) = floor(floor(a / 2^32) / 6) for a >= 0,
+        // so each cubic is shifted first and divided as a 32-bit integer
+        const uint32_t t = (uint32_t)P >> 16;
+        const uint32_t u = 65536u - t;
+        const uint64_t one3 = 1ull << 48;
+        const uint64_t t2 = tpm_mul24(t, t), t3 = tpm_mul48x24(t2, t);
+        const uint64_t u3 = tpm_mul48x24(tpm_mul24(u, u), u);
+        w[0] = (uint32_t)(u3 >> 32) / 6u;
+        w[3] = (uint32_t)(t3 >> 32) / 6u;
+        // 4 - 6 t^2 + 3 t^3 (Q48) > 0 for t in [0, 1]
+        const uint64_t num = 4 * one3 - 6 * (t2 << 16) + 3 * t3;
+        w[1] = (uint32_t)(num >> 32) / 6u;
+        w[2] = (1ull << 16) - w[0] - w[1] - w[3];
         *base = (int)(P >> 32) - 1;
         *qbits = 16;
         return 4;
