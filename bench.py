@@ -484,7 +484,7 @@ def main():
         # unit_kind says what actually bounds it: FP32 on the vector ALUs.
         "roofline": {"bound": "mfma", "unit_kind": "valu_fp32",
                      "bound_note": "FP32 vector ALU peak; the kernel issues no MFMA instruction",
-                     "kernel": "pp_kick_kernel",
+                     "kernel": "pp_kick_kernel + mesh_force_kernel (the PP stage: pairs, mesh interpolation, kick)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "flops_per_launch": FLOPS_PER_PAIR * W, "kernel_ms": pp_ms,
@@ -524,7 +524,7 @@ def main():
     out["roofline"]["pair_body_rate"] = {
         "pairs_per_s": body, "kernel_executed_pairs_per_s": kernel_rate, "ratio": kernel_rate / body,
         "waves_per_simd": 8,
-        "note": "pp_kick_kernel's executed pairs/s over the pair body's rate alone (a reference, not a ceiling)"}
+        "note": "the PP stage's executed pairs/s over the pair body's rate alone (a reference, not a ceiling)"}
     log("pair body alone %.4g pairs/s; kernel %.4g executed pairs/s (%.3f)" % (body, kernel_rate, kernel_rate / body))
     up = useful_pairs(ctx, p, 2048, w["seed"])
     up["candidate_per_particle"] = W / n

@@ -171,6 +171,9 @@ struct tpm_ctx {
     void* gsrc = nullptr;
     void* gmesh = nullptr;
     int64_t sym_cap = 0;
+    void* mesh_force = nullptr;  // float4 per particle: the mesh term of whole-box kicks
+    int64_t mesh_force_cap = 0;
+    int mesh_force_w = 0;        // nodes per axis around one chain cell
     int64_t chain_epoch = 0;  // counts chain-mesh builds (trees remember the one they follow)
 
     // The order the caller sees (tpm_set_host_order).  With

@@ -25,6 +25,7 @@
 // FP32.  The self pair is
 // evaluated like any other source, as in the reference: inside the
 // softening branch it contributes -35/16 h^-1 to phi and nothing to g.
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -63,6 +64,8 @@ struct PPArgs {
     const int4* __restrict__ tiles;
     const int32_t* __restrict__ ntiles;
     const float* __restrict__ phi;  // mesh arena (box offset mesh0), nullptr: no mesh term
+    const float4* __restrict__ gmesh;  // mesh force and potential per sink, made before the
+                                       // launch (nullptr: interpolated in the epilogue)
     unsigned long long* __restrict__ counters;
     int* __restrict__ queue;        // 8 per-XCD tile queues (persistent launch)
     const tpm_box_dev* __restrict__ boxes;  // batched: per-box table, tile.w = box
@@ -375,10 +378,11 @@ __global__ __launch_bounds__(64 * WPG, PERSIST ? 1 : (LISTS ? 7 : 8)) void pp_ki
                 ph *= 0x1p32f;
             }
             // Mesh force and potential: the source streams split the six x
-            // planes of the 6^3 stencil, then sum.
+            // planes of the 6^3 stencil, then sum (or mesh_force_kernel
+            // did the same before the launch).
             float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
             const bool use_mesh = a.phi && !(a.flags & TPM_SKIP_PM);
-            if (use_mesh) {
+            if (use_mesh && !a.gmesh) {
                 mesh_interp6_part(a.phi + B.mesh0, B.M, px, py, pz, stream, SPLIT, sx, sy, sz, sp);
 #pragma unroll
                 for (int o = TS; o < 64; o <<= 1) {
@@ -394,14 +398,89 @@ __global__ __launch_bounds__(64 * WPG, PERSIST ? 1 : (LISTS ? 7 : 8)) void pp_ki
                 int nr = -1;
                 if (active && stream == 0) {
                     const float Mf = (float)B.M;
-                    const float pgx = use_mesh ? -sx * Mf : 0.f, pgy = use_mesh ? -sy * Mf : 0.f;
-                    const float pgz = use_mesh ? -sz * Mf : 0.f, pph = use_mesh ? sp : 0.f;
+                    float pgx = use_mesh ? -sx * Mf : 0.f, pgy = use_mesh ? -sy * Mf : 0.f;
+                    float pgz = use_mesh ? -sz * Mf : 0.f, pph = use_mesh ? sp : 0.f;
+                    if (use_mesh && a.gmesh) {
+                        const float4 m = a.gmesh[i];
+                        pgx = m.x;
+                        pgy = m.y;
+                        pgz = m.z;
+                        pph = m.w;
+                    }
                     nr = kick_epilogue(B, a.flags, o, i, r, pgx, pgy, pgz, pph, gx, gy, gz, ph);
                 }
                 max_rung_update(o, nr);
             }
         }
         if constexpr (!PERSIST) return;
+    }
+}
+
+// The mesh force and potential of every active sink of one box, as the PP
+// kernel's epilogue makes them, one workgroup per chain cell: the potential
+// around the cell (W^3 nodes, periodic) is read once into LDS, then each
+// sink's 6 x planes go in the epilogue's SPLIT = 64 / TS parts with the
+// arithmetic of mesh_interp6_part on the LDS copy, the parts summed in the
+// order of the epilogue's xor shuffles -- the same bits as the epilogue.  Inside
+// the PP kernel the stencil's 216 scattered loads per sink took 11 ms of a
+// 204 ms launch at 512^3; here each potential value is fetched once per
+// neighbouring cell.  Cell k spans raw [bounds[k], bounds[k + 1]).
+template <int TS>
+__global__ __launch_bounds__(256) void mesh_force_kernel(const uint32_t* __restrict__ x,
+                                                        const uint32_t* __restrict__ y,
+                                                        const uint32_t* __restrict__ z,
+                                                        const int8_t* __restrict__ rung, int min_rung,
+                                                        const int32_t* __restrict__ cell_begin,
+                                                        const uint32_t* __restrict__ bounds, int C,
+                                                        const float* __restrict__ phi, int M, int W,
+                                                        float4* __restrict__ out) {
+    extern __shared__ float sphi[];  // W^3, z fastest
+    constexpr int SPLIT = 64 / TS;
+    const int c = (int)blockIdx.x;
+    const int cb = cell_begin[c], ce = cell_begin[c + 1];
+    if (cb == ce) return;
+    const int cc3[3] = {c % C, (c / C) % C, c / (C * C)};
+    int r0[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) r0[d] = (int)(((uint64_t)bounds[cc3[d]] * (uint64_t)M) >> 32) - 2;
+    const int W2 = W * W;
+    for (int l = (int)threadIdx.x; l < W2 * W; l += (int)blockDim.x) {
+        // periodic; r0 >= -2, and W may exceed M on tiny meshes
+        const int gx = (r0[0] + l / W2 + M) % M, gy = (r0[1] + (l / W) % W + M) % M, gz = (r0[2] + l % W + M) % M;
+        sphi[l] = phi[((int64_t)gx * M + gy) * M + gz];
+    }
+    __syncthreads();
+    const float Mf = (float)M;
+    // one lane per sink: its SPLIT parts in turn, summed as the epilogue's
+    // xor shuffles sum them (pairs of neighbouring parts, then pairs of
+    // pairs, ...)
+    for (int i = cb + (int)threadIdx.x; i < ce; i += (int)blockDim.x) {
+        if (rung[i] < min_rung) continue;
+        int I0[3];
+        float X[3];
+        interp6_origin(x[i], y[i], z[i], M, I0, X);
+        const float* base = sphi + ((I0[0] - r0[0]) * W + (I0[1] - r0[1])) * W + (I0[2] - r0[2]);
+        const auto load = [&](int aa, int bb, int cc) { return base[(aa * W + bb) * W + cc]; };
+        float wy[6], dwy[6], wz[6], dwz[6];
+        interp6_weights(X[1], wy, dwy);
+        interp6_weights(X[2], wz, dwz);
+        float4 part[SPLIT];
+#pragma unroll
+        for (int k = 0; k < SPLIT; k++) {
+            float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
+            interp6_planes<(6 + SPLIT - 1) / SPLIT>(X, wy, dwy, wz, dwz, k, SPLIT, load, sx, sy, sz, sp);
+            part[k] = make_float4(sx, sy, sz, sp);
+        }
+#pragma unroll
+        for (int h = 1; h < SPLIT; h <<= 1)
+#pragma unroll
+            for (int k = 0; k < SPLIT; k += 2 * h) {
+                part[k].x += part[k + h].x;
+                part[k].y += part[k + h].y;
+                part[k].z += part[k + h].z;
+                part[k].w += part[k + h].w;
+            }
+        out[i] = make_float4(-part[0].x * Mf, -part[0].y * Mf, -part[0].z * Mf, part[0].w);
     }
 }
 
@@ -564,6 +643,21 @@ void tpm_box_fill(tpm_box_dev* b, const tpm_params& p, int min_rung, int first_c
     b->drift = t0;
 }
 
+// Nodes per axis of the potential around one chain cell: the sinks of cell k
+// (raw [b_k, b_k+1)) take stencil nodes from mesh index (b_k M >> 32) - 2 to
+// ((b_k+1 - 1) M >> 32) + 3.  0 when the largest cube exceeds 32 KiB.
+static int mesh_force_width(int C, int M) {
+    const double ninv = 1.0 / C;
+    int w = 0;
+    for (int k = 0; k < C; k++) {
+        const uint64_t lo = tpm_split_plane(k, ninv);
+        const uint64_t hi = (k + 1 < C ? tpm_split_plane(k + 1, ninv) : (1ull << 32)) - 1;
+        const int a = (int)((lo * (uint64_t)M) >> 32) - 2, b = (int)((hi * (uint64_t)M) >> 32) + 3;
+        w = std::max(w, b - a + 1);
+    }
+    return (int64_t)w * w * w * 4 <= 32768 ? w : 0;
+}
+
 tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int first_call, unsigned flags,
                        tpm_stats* stats, const tpm_cell_box* box, int64_t ntiles_box) {
     if (!ctx->chain_valid || ctx->chain_C != ctx->C || ctx->chain_tile != ctx->tile)
@@ -593,12 +687,28 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     a.tiles = ctx->tiles;
     a.ntiles = ctx->ntiles;
     a.phi = use_pm ? ctx->phi : nullptr;
+    a.gmesh = nullptr;
     a.counters = ctx->counters;
     a.queue = nullptr;
     a.boxes = nullptr;
     a.flags = flags;
     const int64_t ncell = (int64_t)ctx->C * ctx->C * ctx->C;
     int64_t tiles_max = ctx->n / ctx->tile + ncell + 1;
+    // whole-box kicks take the mesh force from its own pass (same bits) when
+    // a chain cell's potential neighbourhood fits 32 KiB of LDS
+    const int mfw = mesh_force_width(ctx->C, ctx->M);
+    ctx->mesh_force_w = mfw;
+    if (use_pm && !box && ctx->n > 0 && mfw > 0 && ctx->bounds) {
+        const int64_t n = ctx->n;
+        if (n > ctx->mesh_force_cap) {
+            if (ctx->mesh_force) TPM_HIP(ctx, hipFree(ctx->mesh_force));
+            ctx->mesh_force = nullptr;
+            ctx->mesh_force_cap = 0;
+            TPM_HIP(ctx, hipMalloc(&ctx->mesh_force, sizeof(float4) * n));
+            ctx->mesh_force_cap = n;
+        }
+        a.gmesh = static_cast<float4*>(ctx->mesh_force);
+    }
     int3 blo = make_int3(0, 0, 0), bhi = make_int3(INT_MAX, INT_MAX, INT_MAX);
     if (box) {
         // the box's tiles into box_tiles (ntiles_box of them, counted by the caller)
@@ -648,6 +758,26 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
         ctx->prof_pp.emplace_back(e0, e1);
     }
     TPM_HIP(ctx, hipEventRecord(e0, s));
+    if (a.gmesh) {
+        float4* mf = const_cast<float4*>(a.gmesh);
+        const size_t lds = sizeof(float) * (size_t)ctx->mesh_force_w * ctx->mesh_force_w * ctx->mesh_force_w;
+        const unsigned nc = (unsigned)ncell;
+        switch (ctx->tile) {
+            case 8: mesh_force_kernel<8><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,
+                                                              ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi,
+                                                              ctx->M, ctx->mesh_force_w, mf); break;
+            case 16: mesh_force_kernel<16><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,
+                                                                ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi,
+                                                                ctx->M, ctx->mesh_force_w, mf); break;
+            case 32: mesh_force_kernel<32><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,
+                                                                ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi,
+                                                                ctx->M, ctx->mesh_force_w, mf); break;
+            default: mesh_force_kernel<64><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,
+                                                                ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi,
+                                                                ctx->M, ctx->mesh_force_w, mf); break;
+        }
+        TPM_HIP(ctx, hipGetLastError());
+    }
     if (ctx->n > 0 && blocks > 0) {
         pp_launch<false>(ctx->tile, false, (unsigned)blocks, s, a, pme, wpg, ctx->pp_lists != 0);
         TPM_HIP(ctx, hipGetLastError());

@@ -226,6 +226,63 @@ __device__ __forceinline__ void mesh_interp6_part(const float* __restrict__ phi,
     }
 }
 
+// The arithmetic of mesh_interp6_part (the same products and sums in the
+// same order, so the same bits) with every stencil value of a lane's planes
+// fetched before the first product: load(aa, bb, cc) gives phi at x plane aa,
+// y row bb, z column cc of the sink's stencil.  NPL = planes per lane, at
+// least ceil((6 - aa0) / astep).  X: the sink's in-cell offsets (as made by
+// mesh_interp6_part).
+// wy, dwy, wz, dwz: interp6_weights of X[1] and X[2].
+template <int NPL, class Load>
+__device__ __forceinline__ void interp6_planes(const float X[3], const float wy[6], const float dwy[6],
+                                               const float wz[6], const float dwz[6], int aa0, int astep, Load load,
+                                               float& sx, float& sy, float& sz, float& sp) {
+    float p[NPL][6][6];
+#pragma unroll
+    for (int l = 0; l < NPL; l++) {
+        const int aa = min(aa0 + l * astep, 5);
+#pragma unroll
+        for (int bb = 0; bb < 6; bb++)
+#pragma unroll
+            for (int cc = 0; cc < 6; cc++) p[l][bb][cc] = load(aa, bb, cc);
+    }
+#pragma unroll
+    for (int l = 0; l < NPL; l++) {
+        const int aa = aa0 + l * astep;
+        if (aa >= 6) break;
+        float u = 0.f, v = 0.f, t = 0.f;
+#pragma unroll
+        for (int bb = 0; bb < 6; bb++) {
+            float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+            for (int cc = 0; cc < 6; cc++) {
+                s0 = fmaf(wz[cc], p[l][bb][cc], s0);
+                s1 = fmaf(dwz[cc], p[l][bb][cc], s1);
+            }
+            u = fmaf(wy[bb], s0, u);
+            v = fmaf(dwy[bb], s0, v);
+            t = fmaf(wy[bb], s1, t);
+        }
+        float wx, dwx;
+        interp6_weight(X[0], aa, &wx, &dwx);
+        sx = fmaf(dwx, u, sx);
+        sy = fmaf(wx, v, sy);
+        sz = fmaf(wx, t, sz);
+        sp = fmaf(wx, u, sp);
+    }
+}
+
+// The stencil's first node (mesh index - 2) and in-cell offset per axis.
+__device__ __forceinline__ void interp6_origin(uint32_t px, uint32_t py, uint32_t pz, int M, int I0[3], float X[3]) {
+    const uint32_t pr[3] = {px, py, pz};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const uint64_t P = (uint64_t)pr[d] * (uint64_t)M;
+        I0[d] = (int)(P >> 32) - 2;
+        X[d] = (float)((uint32_t)P >> 8) * 0x1p-24f;
+    }
+}
+
 // The whole stencil in one lane.
 __device__ __forceinline__ void mesh_interp6(const float* __restrict__ phi, int M, uint32_t px, uint32_t py,
                                              uint32_t pz, float& pgx, float& pgy, float& pgz, float& pph) {
