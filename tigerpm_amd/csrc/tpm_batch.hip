@@ -54,7 +54,8 @@ Caps caps_of(const tpm_batch_limits& l) {
 
 int64_t arena_bytes(const tpm_batch_limits& l) {
     const Caps c = caps_of(l);
-    const int64_t per_p = 12 * 2 + 12 * 2 + 2 + 16 + 16;  // positions, velocities (x2), rungs, g/phi, sort
+    // positions, velocities (x2), rungs, g/phi, sort, mesh force
+    const int64_t per_p = 12 * 2 + 12 * 2 + 2 + 16 + 16 + 16;
     const int64_t per_m = 8 + 4 + 4 + (l.interlace ? 4 : 0);  // Q32 density, float densities, phi
     const int64_t per_k = 8 + 4 + (l.interlace ? 8 : 0);  // spectrum, Green's, shifted spectrum
     const int64_t per_c = 4 * 4;                          // counts, cell_begin, tiles per cell, tile offsets
@@ -93,7 +94,7 @@ tpm_status build_layout(tpm_batch* bt, tpm_batch_layout* L) {
     TPM_HIP(bt, hipEventSynchronize(bt->table_ev));  // previous upload done with the staging copy
     const int ts = bt->tune.tile_sinks;
     int64_t cell = 0, dep = 0, q = 0, tiles = 0;
-    int lds = 0, nk = 0;
+    int lds = 0, nk = 0, mfw = 0;
     const int nb = (int)bt->boxes.size();
     for (int i = 0; i < nb; i++) {
         const tpm_batch_box& b = bt->boxes[i];
@@ -117,6 +118,8 @@ tpm_status build_layout(tpm_batch* bt, tpm_batch_layout* L) {
             q += b.n;
             tiles += b.n / ts + cells_of(b.C) + 1;
             lds = std::max(lds, b.wmax * b.wmax * b.wmax * 8);
+            const int w = tpm_mesh_force_width(b.C, b.M);
+            mfw = (w == 0 || mfw < 0) ? -1 : std::max(mfw, w);
             bt->h_kreg[nk++] = tpm_kregion{b.k0, kmodes(b.M), b.M, 0};
         }
     }
@@ -134,6 +137,7 @@ tpm_status build_layout(tpm_batch* bt, tpm_batch_layout* L) {
     L->order = bt->lim.order;
     L->interlace = bt->lim.interlace;
     L->lds = lds;
+    L->mfw = mfw > 0 ? mfw : 0;
     if (nb > 0)
         TPM_HIP(bt, hipMemcpyAsync(bt->d_boxes, bt->h_boxes, sizeof(tpm_box_dev) * nb, hipMemcpyHostToDevice,
                                    bt->stream));
@@ -435,7 +439,7 @@ extern "C" tpm_status tpm_batch_create(int device, const tpm_batch_limits* l, tp
         alloc(bt, &bt->phi, c.m) || alloc(bt, &bt->rho_hat, c.k) || alloc(bt, &bt->green, c.k) ||
         (l->interlace && alloc(bt, &bt->rho_hat1, c.k)) || alloc(bt, &bt->d_boxes, l->max_boxes) ||
         alloc(bt, &bt->d_kreg, l->max_boxes) || alloc(bt, &bt->counters, TPM_COUNTER_WORDS) ||
-        alloc(bt, &bt->moment_buf, 8 * TPM_MOMENT_BLOCKS + 8))
+        alloc(bt, &bt->moment_buf, 8 * TPM_MOMENT_BLOCKS + 8) || alloc(bt, &bt->mesh_force, P))
         return fail(TPM_ERR_BUDGET);
     if (l->interlace && alloc(bt, &bt->rho1, c.m)) return fail(TPM_ERR_BUDGET);
     for (int k = 0; k < tpm_batch::NSIDE; k++)
@@ -481,7 +485,7 @@ extern "C" tpm_status tpm_batch_destroy(tpm_batch* bt) {
                     bt->gx,    bt->gy,       bt->gz,       bt->pot,      bt->key,    bt->slot,    bt->tmp,
                     bt->perm,  bt->cnt,      bt->cell_begin, bt->tile_cnt, bt->tile_off, bt->scan_part, bt->tiles,
                     bt->rho_q, bt->rho,      bt->phi,      bt->rho_hat,  bt->rho_hat1, bt->green,  bt->d_boxes,
-                    bt->d_kreg, bt->counters, bt->moment_buf};
+                    bt->d_kreg, bt->counters, bt->moment_buf, bt->mesh_force};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (bt->h_boxes) (void)hipHostFree(bt->h_boxes);

@@ -461,6 +461,7 @@ struct tpm_batch {
     float *vx = nullptr, *vy = nullptr, *vz = nullptr, *vx2 = nullptr, *vy2 = nullptr, *vz2 = nullptr;
     int8_t *rung = nullptr, *rung2 = nullptr;
     float *gx = nullptr, *gy = nullptr, *gz = nullptr, *pot = nullptr;
+    float4* mesh_force = nullptr;  // mesh force and potential per particle (separate pass)
     uint32_t *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr;
     // chain cells and tiles of all live boxes
     int32_t *cnt = nullptr, *cell_begin = nullptr, *tile_cnt = nullptr, *tile_off = nullptr, *scan_part = nullptr;
@@ -521,7 +522,11 @@ struct tpm_batch_layout {
     int64_t mesh_hw = 0, k_hw = 0;
     int nkreg = 0;
     int order = 3, interlace = 0, lds = 0;
+    int mfw = 0;           // potential nodes per axis around a chain cell (0: no separate mesh-force pass)
 };
+// Potential nodes per axis around one chain cell of C (the separate
+// mesh-force pass's LDS cube); 0 when the cube exceeds 32 KiB.
+int tpm_mesh_force_width(int C, int M);
 
 // Internal entry points implemented in the .hip files.
 tpm_status tpm_check_params(std::string& err, tpm_params* p);

@@ -425,24 +425,15 @@ __global__ __launch_bounds__(64 * WPG, PERSIST ? 1 : (LISTS ? 7 : 8)) void pp_ki
 // the PP kernel the stencil's 216 scattered loads per sink took 11 ms of a
 // 204 ms launch at 512^3; here each potential value is fetched once per
 // neighbouring cell.  Cell k spans raw [bounds[k], bounds[k + 1]).
+// One chain cell's sinks [cb, ce); r0: the cell's first stencil node per
+// axis (mesh index of its lower split plane - 2), W nodes per axis in LDS.
 template <int TS>
-__global__ __launch_bounds__(256) void mesh_force_kernel(const uint32_t* __restrict__ x,
-                                                        const uint32_t* __restrict__ y,
-                                                        const uint32_t* __restrict__ z,
-                                                        const int8_t* __restrict__ rung, int min_rung,
-                                                        const int32_t* __restrict__ cell_begin,
-                                                        const uint32_t* __restrict__ bounds, int C,
-                                                        const float* __restrict__ phi, int M, int W,
-                                                        float4* __restrict__ out) {
-    extern __shared__ float sphi[];  // W^3, z fastest
+__device__ __forceinline__ void mesh_force_cell(float* sphi, const uint32_t* __restrict__ x,
+                                                const uint32_t* __restrict__ y, const uint32_t* __restrict__ z,
+                                                const int8_t* __restrict__ rung, int min_rung, int cb, int ce,
+                                                const int r0[3], const float* __restrict__ phi, int M, int W,
+                                                float4* __restrict__ out) {
     constexpr int SPLIT = 64 / TS;
-    const int c = (int)blockIdx.x;
-    const int cb = cell_begin[c], ce = cell_begin[c + 1];
-    if (cb == ce) return;
-    const int cc3[3] = {c % C, (c / C) % C, c / (C * C)};
-    int r0[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) r0[d] = (int)(((uint64_t)bounds[cc3[d]] * (uint64_t)M) >> 32) - 2;
     const int W2 = W * W;
     for (int l = (int)threadIdx.x; l < W2 * W; l += (int)blockDim.x) {
         // periodic; r0 >= -2, and W may exceed M on tiny meshes
@@ -482,6 +473,53 @@ __global__ __launch_bounds__(256) void mesh_force_kernel(const uint32_t* __restr
             }
         out[i] = make_float4(-part[0].x * Mf, -part[0].y * Mf, -part[0].z * Mf, part[0].w);
     }
+}
+
+// One box: cell c = (cz C + cy) C + cx spans raw [bounds[k], bounds[k + 1]).
+template <int TS>
+__global__ __launch_bounds__(256) void mesh_force_kernel(const uint32_t* __restrict__ x,
+                                                        const uint32_t* __restrict__ y,
+                                                        const uint32_t* __restrict__ z,
+                                                        const int8_t* __restrict__ rung, int min_rung,
+                                                        const int32_t* __restrict__ cell_begin,
+                                                        const uint32_t* __restrict__ bounds, int C,
+                                                        const float* __restrict__ phi, int M, int W,
+                                                        float4* __restrict__ out) {
+    extern __shared__ float sphi[];  // W^3, z fastest
+    const int c = (int)blockIdx.x;
+    const int cb = cell_begin[c], ce = cell_begin[c + 1];
+    if (cb == ce) return;
+    const int k3[3] = {c % C, (c / C) % C, c / (C * C)};
+    int r0[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) r0[d] = (int)(((uint64_t)bounds[k3[d]] * (uint64_t)M) >> 32) - 2;
+    mesh_force_cell<TS>(sphi, x, y, z, rung, min_rung, cb, ce, r0, phi, M, W, out);
+}
+
+// Batched: global cell c belongs to the last box whose cell0 <= c; its split
+// planes are fixed32(k / C) and its mesh starts at mesh0.
+template <int TS>
+__global__ __launch_bounds__(256) void bmesh_force_kernel(const uint32_t* __restrict__ x,
+                                                         const uint32_t* __restrict__ y,
+                                                         const uint32_t* __restrict__ z,
+                                                         const int8_t* __restrict__ rung,
+                                                         const int32_t* __restrict__ cell_begin, int nbox,
+                                                         const tpm_box_dev* __restrict__ boxes,
+                                                         const float* __restrict__ phi, int W,
+                                                         float4* __restrict__ out) {
+    extern __shared__ float sphi[];
+    const int c = (int)blockIdx.x;
+    const int cb = cell_begin[c], ce = cell_begin[c + 1];
+    if (cb == ce) return;
+    const int b = tpm_upper_index(nbox, c, [&](int k) { return (int64_t)boxes[k].cell0; });
+    const tpm_box_dev& B = boxes[b];
+    const int C = B.C, M = B.M, l = c - B.cell0;
+    const int k3[3] = {l % C, (l / C) % C, l / (C * C)};
+    int r0[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        r0[d] = (int)(((uint64_t)tpm_split_plane(k3[d], B.ninv) * (uint64_t)M) >> 32) - 2;
+    mesh_force_cell<TS>(sphi, x, y, z, rung, B.min_rung, cb, ce, r0, phi + B.mesh0, M, W, out);
 }
 
 // W_sr = sum_c n_c * sum_{27 neighbours} n_c' (exact, 64-bit integer) over
@@ -646,7 +684,7 @@ void tpm_box_fill(tpm_box_dev* b, const tpm_params& p, int min_rung, int first_c
 // Nodes per axis of the potential around one chain cell: the sinks of cell k
 // (raw [b_k, b_k+1)) take stencil nodes from mesh index (b_k M >> 32) - 2 to
 // ((b_k+1 - 1) M >> 32) + 3.  0 when the largest cube exceeds 32 KiB.
-static int mesh_force_width(int C, int M) {
+int tpm_mesh_force_width(int C, int M) {
     const double ninv = 1.0 / C;
     int w = 0;
     for (int k = 0; k < C; k++) {
@@ -696,7 +734,7 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     int64_t tiles_max = ctx->n / ctx->tile + ncell + 1;
     // whole-box kicks take the mesh force from its own pass (same bits) when
     // a chain cell's potential neighbourhood fits 32 KiB of LDS
-    const int mfw = mesh_force_width(ctx->C, ctx->M);
+    const int mfw = tpm_mesh_force_width(ctx->C, ctx->M);
     ctx->mesh_force_w = mfw;
     if (use_pm && !box && ctx->n > 0 && mfw > 0 && ctx->bounds) {
         const int64_t n = ctx->n;
@@ -842,6 +880,28 @@ tpm_status tpm_batch_pp(tpm_batch* bt, const tpm_batch_layout& L, unsigned flags
     a.queue = reinterpret_cast<int*>(bt->counters + 8);
     a.boxes = bt->d_boxes;
     a.flags = flags & (TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK);
+    a.gmesh = nullptr;
+    if (a.phi && L.mfw > 0) {
+        // the mesh force of every box's active sinks, before the pair kernel
+        const size_t lds = sizeof(float) * (size_t)L.mfw * L.mfw * L.mfw;
+        const unsigned nc = (unsigned)L.ncell;
+        switch (bt->tune.tile_sinks) {
+            case 8: bmesh_force_kernel<8><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,
+                                                                        bt->cell_begin, L.nbox, bt->d_boxes,
+                                                                        bt->phi, L.mfw, bt->mesh_force); break;
+            case 16: bmesh_force_kernel<16><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,
+                                                                          bt->cell_begin, L.nbox, bt->d_boxes,
+                                                                          bt->phi, L.mfw, bt->mesh_force); break;
+            case 32: bmesh_force_kernel<32><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,
+                                                                          bt->cell_begin, L.nbox, bt->d_boxes,
+                                                                          bt->phi, L.mfw, bt->mesh_force); break;
+            default: bmesh_force_kernel<64><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,
+                                                                          bt->cell_begin, L.nbox, bt->d_boxes,
+                                                                          bt->phi, L.mfw, bt->mesh_force); break;
+        }
+        TPM_HIP(bt, hipGetLastError());
+        a.gmesh = bt->mesh_force;
+    }
     const bool persistent = bt->tune.sched == 1;
     int64_t grid = persistent ? bt->persist_grid : (L.tiles_max + 7) / 8 * 8;
     if (grid < 8) grid = 8;
