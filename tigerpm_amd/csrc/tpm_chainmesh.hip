@@ -34,16 +34,32 @@ __global__ void key_kernel(int64_t n, const uint32_t* __restrict__ x, const uint
     __shared__ uint32_t b[TPM_MAX_CHAIN];
     for (int k = threadIdx.x; k < C; k += blockDim.x) b[k] = bnd[k];
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        int sx, sy, sz;
-        const int cx = tpm_chain_coord(x[i], C, b, &sx);
-        const int cy = tpm_chain_coord(y[i], C, b, &sy);
-        const int cz = tpm_chain_coord(z[i], C, b, &sz);
-        const uint32_t cid = ((uint32_t)cz * C + cy) * C + cx;
-        const uint32_t m = ((sz >> 1) << 5) | ((sy >> 1) << 4) | ((sx >> 1) << 3) | ((sz & 1) << 2) |
-                           ((sy & 1) << 1) | (sx & 1);
-        key[i] = cid * 64u + m;
-        idx[i] = (uint32_t)i;
+    // 4 particles per thread and pass, their loads issued together (a
+    // capped grid: the split-plane table is loaded once per workgroup)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+        uint32_t px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t i = min(i0 + u * stride, n - 1);
+            px[u] = x[i];
+            py[u] = y[i];
+            pz[u] = z[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t i = i0 + u * stride;
+            if (i >= n) break;
+            int sx, sy, sz;
+            const int cx = tpm_chain_coord(px[u], C, b, &sx);
+            const int cy = tpm_chain_coord(py[u], C, b, &sy);
+            const int cz = tpm_chain_coord(pz[u], C, b, &sz);
+            const uint32_t cid = ((uint32_t)cz * C + cy) * C + cx;
+            const uint32_t m = ((sz >> 1) << 5) | ((sy >> 1) << 4) | ((sx >> 1) << 3) | ((sz & 1) << 2) |
+                               ((sy & 1) << 1) | (sx & 1);
+            key[i] = cid * 64u + m;
+            idx[i] = (uint32_t)i;
+        }
     }
 }
 
@@ -416,7 +432,8 @@ tpm_status tpm_chainmesh_build(tpm_ctx* ctx) {
         return TPM_OK;
     }
 
-    key_kernel<<<grid_for(n, 256), 256, 0, s>>>(n, ctx->x, ctx->y, ctx->z, C, ctx->bounds, ctx->keys, ctx->idx);
+    key_kernel<<<std::min(grid_for(n, 256 * 4), 8192u), 256, 0, s>>>(n, ctx->x, ctx->y, ctx->z, C, ctx->bounds,
+                                                                        ctx->keys, ctx->idx);
     TPM_HIP(ctx, hipGetLastError());
 
     unsigned end_bit = 1;
