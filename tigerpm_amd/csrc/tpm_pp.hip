@@ -63,9 +63,8 @@ struct PPArgs {
     const int32_t* __restrict__ cell_begin;
     const int4* __restrict__ tiles;
     const int32_t* __restrict__ ntiles;
-    const float* __restrict__ phi;  // mesh arena (box offset mesh0), nullptr: no mesh term
-    const float4* __restrict__ gmesh;  // mesh force and potential per sink, made before the
-                                       // launch (nullptr: interpolated in the epilogue)
+    const float4* __restrict__ mesh;  // mesh force and potential per sink, made before the launch
+                                      // by mesh_force_kernel (nullptr: no mesh term)
     unsigned long long* __restrict__ counters;
     int* __restrict__ queue;        // 8 per-XCD tile queues (persistent launch)
     const tpm_box_dev* __restrict__ boxes;  // batched: per-box table, tile.w = box
@@ -377,31 +376,15 @@ __global__ __launch_bounds__(64 * WPG, PERSIST ? 1 : (LISTS ? 7 : 8)) void pp_ki
                 gz *= 0x1p64f;
                 ph *= 0x1p32f;
             }
-            // Mesh force and potential: the source streams split the six x
-            // planes of the 6^3 stencil, then sum (or mesh_force_kernel
-            // did the same before the launch).
-            float sx = 0.f, sy = 0.f, sz = 0.f, sp = 0.f;
-            const bool use_mesh = a.phi && !(a.flags & TPM_SKIP_PM);
-            if (use_mesh && !a.gmesh) {
-                mesh_interp6_part(a.phi + B.mesh0, B.M, px, py, pz, stream, SPLIT, sx, sy, sz, sp);
-#pragma unroll
-                for (int o = TS; o < 64; o <<= 1) {
-                    sx += __shfl_xor(sx, o);
-                    sy += __shfl_xor(sy, o);
-                    sz += __shfl_xor(sz, o);
-                    sp += __shfl_xor(sp, o);
-                }
-            }
             {
-                // Then the kick, by each active sink's stream-0 lane.
+                // Then the kick, by each active sink's stream-0 lane, with the
+                // mesh force and potential made before the launch.
                 const tpm_kick_out o{a.gx, a.gy, a.gz, a.pot, a.vx, a.vy, a.vz, a.rung, cnt};
                 int nr = -1;
                 if (active && stream == 0) {
-                    const float Mf = (float)B.M;
-                    float pgx = use_mesh ? -sx * Mf : 0.f, pgy = use_mesh ? -sy * Mf : 0.f;
-                    float pgz = use_mesh ? -sz * Mf : 0.f, pph = use_mesh ? sp : 0.f;
-                    if (use_mesh && a.gmesh) {
-                        const float4 m = a.gmesh[i];
+                    float pgx = 0.f, pgy = 0.f, pgz = 0.f, pph = 0.f;
+                    if (a.mesh) {
+                        const float4 m = a.mesh[i];
                         pgx = m.x;
                         pgy = m.y;
                         pgz = m.z;
@@ -426,8 +409,10 @@ __global__ __launch_bounds__(64 * WPG, PERSIST ? 1 : (LISTS ? 7 : 8)) void pp_ki
 // 204 ms launch at 512^3; here each potential value is fetched once per
 // neighbouring cell.  Cell k spans raw [bounds[k], bounds[k + 1]).
 // One chain cell's sinks [cb, ce); r0: the cell's first stencil node per
-// axis (mesh index of its lower split plane - 2), W nodes per axis in LDS.
-template <int TS>
+// axis (mesh index of its lower split plane - 2).  The W^3 nodes from r0 on
+// are read into LDS first; GLOBAL: the stencil is read from phi directly (a
+// cell's neighbourhood larger than the LDS cube).
+template <int TS, bool GLOBAL>
 __device__ __forceinline__ void mesh_force_cell(float* sphi, const uint32_t* __restrict__ x,
                                                 const uint32_t* __restrict__ y, const uint32_t* __restrict__ z,
                                                 const int8_t* __restrict__ rung, int min_rung, int cb, int ce,
@@ -435,23 +420,37 @@ __device__ __forceinline__ void mesh_force_cell(float* sphi, const uint32_t* __r
                                                 float4* __restrict__ out) {
     constexpr int SPLIT = 64 / TS;
     const int W2 = W * W;
-    for (int l = (int)threadIdx.x; l < W2 * W; l += (int)blockDim.x) {
+    for (int l = (int)threadIdx.x; l < (GLOBAL ? 0 : W2 * W); l += (int)blockDim.x) {
         // periodic; r0 >= -2, and W may exceed M on tiny meshes
         const int gx = (r0[0] + l / W2 + M) % M, gy = (r0[1] + (l / W) % W + M) % M, gz = (r0[2] + l % W + M) % M;
         sphi[l] = phi[((int64_t)gx * M + gy) * M + gz];
     }
     __syncthreads();
     const float Mf = (float)M;
-    // one lane per sink: its SPLIT parts in turn, summed as the epilogue's
-    // xor shuffles sum them (pairs of neighbouring parts, then pairs of
-    // pairs, ...)
+    // one lane per sink: its SPLIT parts in turn, summed as the xor shuffles
+    // of a wave that splits each sink over 64 / TS lanes would sum them
+    // (pairs of neighbouring parts, then pairs of pairs, ...)
     for (int i = cb + (int)threadIdx.x; i < ce; i += (int)blockDim.x) {
         if (rung[i] < min_rung) continue;
         int I0[3];
         float X[3];
         interp6_origin(x[i], y[i], z[i], M, I0, X);
         const float* base = sphi + ((I0[0] - r0[0]) * W + (I0[1] - r0[1])) * W + (I0[2] - r0[2]);
-        const auto load = [&](int aa, int bb, int cc) { return base[(aa * W + bb) * W + cc]; };
+        int jx[6], jy[6], jz[6];
+        if (GLOBAL) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                jx[k] = (I0[0] + k + M) % M;
+                jy[k] = (I0[1] + k + M) % M;
+                jz[k] = (I0[2] + k + M) % M;
+            }
+        }
+        const auto load = [&](int aa, int bb, int cc) {
+            if constexpr (GLOBAL)
+                return phi[((int64_t)jx[aa] * M + jy[bb]) * M + jz[cc]];
+            else
+                return base[(aa * W + bb) * W + cc];
+        };
         float wy[6], dwy[6], wz[6], dwz[6];
         interp6_weights(X[1], wy, dwy);
         interp6_weights(X[2], wz, dwz);
@@ -475,8 +474,9 @@ __device__ __forceinline__ void mesh_force_cell(float* sphi, const uint32_t* __r
     }
 }
 
-// One box: cell c = (cz C + cy) C + cx spans raw [bounds[k], bounds[k + 1]).
-template <int TS>
+// One box: cell c = (cz C + cy) C + cx spans raw [bounds[k], bounds[k + 1]);
+// only the cells in [lo, hi) per axis.
+template <int TS, bool GLOBAL>
 __global__ __launch_bounds__(256) void mesh_force_kernel(const uint32_t* __restrict__ x,
                                                         const uint32_t* __restrict__ y,
                                                         const uint32_t* __restrict__ z,
@@ -484,21 +484,22 @@ __global__ __launch_bounds__(256) void mesh_force_kernel(const uint32_t* __restr
                                                         const int32_t* __restrict__ cell_begin,
                                                         const uint32_t* __restrict__ bounds, int C,
                                                         const float* __restrict__ phi, int M, int W,
-                                                        float4* __restrict__ out) {
+                                                        int3 lo, int3 hi, float4* __restrict__ out) {
     extern __shared__ float sphi[];  // W^3, z fastest
     const int c = (int)blockIdx.x;
+    const int k3[3] = {c % C, (c / C) % C, c / (C * C)};
+    if (k3[0] < lo.x || k3[0] >= hi.x || k3[1] < lo.y || k3[1] >= hi.y || k3[2] < lo.z || k3[2] >= hi.z) return;
     const int cb = cell_begin[c], ce = cell_begin[c + 1];
     if (cb == ce) return;
-    const int k3[3] = {c % C, (c / C) % C, c / (C * C)};
     int r0[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) r0[d] = (int)(((uint64_t)bounds[k3[d]] * (uint64_t)M) >> 32) - 2;
-    mesh_force_cell<TS>(sphi, x, y, z, rung, min_rung, cb, ce, r0, phi, M, W, out);
+    mesh_force_cell<TS, GLOBAL>(sphi, x, y, z, rung, min_rung, cb, ce, r0, phi, M, W, out);
 }
 
 // Batched: global cell c belongs to the last box whose cell0 <= c; its split
 // planes are fixed32(k / C) and its mesh starts at mesh0.
-template <int TS>
+template <int TS, bool GLOBAL>
 __global__ __launch_bounds__(256) void bmesh_force_kernel(const uint32_t* __restrict__ x,
                                                          const uint32_t* __restrict__ y,
                                                          const uint32_t* __restrict__ z,
@@ -519,7 +520,7 @@ __global__ __launch_bounds__(256) void bmesh_force_kernel(const uint32_t* __rest
 #pragma unroll
     for (int d = 0; d < 3; d++)
         r0[d] = (int)(((uint64_t)tpm_split_plane(k3[d], B.ninv) * (uint64_t)M) >> 32) - 2;
-    mesh_force_cell<TS>(sphi, x, y, z, rung, B.min_rung, cb, ce, r0, phi + B.mesh0, M, W, out);
+    mesh_force_cell<TS, GLOBAL>(sphi, x, y, z, rung, B.min_rung, cb, ce, r0, phi + B.mesh0, M, W, out);
 }
 
 // W_sr = sum_c n_c * sum_{27 neighbours} n_c' (exact, 64-bit integer) over
@@ -724,19 +725,17 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     a.cell_begin = ctx->cell_begin;
     a.tiles = ctx->tiles;
     a.ntiles = ctx->ntiles;
-    a.phi = use_pm ? ctx->phi : nullptr;
-    a.gmesh = nullptr;
+    float4* gmesh = nullptr;
     a.counters = ctx->counters;
     a.queue = nullptr;
     a.boxes = nullptr;
     a.flags = flags;
     const int64_t ncell = (int64_t)ctx->C * ctx->C * ctx->C;
     int64_t tiles_max = ctx->n / ctx->tile + ncell + 1;
-    // whole-box kicks take the mesh force from its own pass (same bits) when
-    // a chain cell's potential neighbourhood fits 32 KiB of LDS
-    const int mfw = tpm_mesh_force_width(ctx->C, ctx->M);
-    ctx->mesh_force_w = mfw;
-    if (use_pm && !box && ctx->n > 0 && mfw > 0 && ctx->bounds) {
+    // the mesh force and potential of the active sinks, made before the
+    // pair kernel (LDS cube per chain cell when it fits 32 KiB)
+    ctx->mesh_force_w = tpm_mesh_force_width(ctx->C, ctx->M);
+    if (use_pm && ctx->n > 0) {
         const int64_t n = ctx->n;
         if (n > ctx->mesh_force_cap) {
             if (ctx->mesh_force) TPM_HIP(ctx, hipFree(ctx->mesh_force));
@@ -745,7 +744,7 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
             TPM_HIP(ctx, hipMalloc(&ctx->mesh_force, sizeof(float4) * n));
             ctx->mesh_force_cap = n;
         }
-        a.gmesh = static_cast<float4*>(ctx->mesh_force);
+        gmesh = static_cast<float4*>(ctx->mesh_force);
     }
     int3 blo = make_int3(0, 0, 0), bhi = make_int3(INT_MAX, INT_MAX, INT_MAX);
     if (box) {
@@ -779,6 +778,7 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
         a.box0.phi0 = 0.f;
     }
     a.flags = flags & (TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK);
+    a.mesh = gmesh;
 
     // tiles per workgroup (kick_pme's launch: one)
     const int wpg = pme ? 1 : ctx->pp_waves;
@@ -796,24 +796,26 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
         ctx->prof_pp.emplace_back(e0, e1);
     }
     TPM_HIP(ctx, hipEventRecord(e0, s));
-    if (a.gmesh) {
-        float4* mf = const_cast<float4*>(a.gmesh);
-        const size_t lds = sizeof(float) * (size_t)ctx->mesh_force_w * ctx->mesh_force_w * ctx->mesh_force_w;
+    if (gmesh) {
+        const int W = ctx->mesh_force_w;
+        const size_t lds = sizeof(float) * (size_t)W * W * W;
         const unsigned nc = (unsigned)ncell;
+#define TPM_MF(TSV)                                                                                               \
+    if (W > 0)                                                                                                  \
+        mesh_force_kernel<TSV, false><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,         \
+                                                           ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi,      \
+                                                           ctx->M, W, blo, bhi, gmesh);                         \
+    else                                                                                                        \
+        mesh_force_kernel<TSV, true><<<nc, 256, 0, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,            \
+                                                        ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi, ctx->M, \
+                                                        0, blo, bhi, gmesh)
         switch (ctx->tile) {
-            case 8: mesh_force_kernel<8><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,
-                                                              ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi,
-                                                              ctx->M, ctx->mesh_force_w, mf); break;
-            case 16: mesh_force_kernel<16><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,
-                                                                ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi,
-                                                                ctx->M, ctx->mesh_force_w, mf); break;
-            case 32: mesh_force_kernel<32><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,
-                                                                ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi,
-                                                                ctx->M, ctx->mesh_force_w, mf); break;
-            default: mesh_force_kernel<64><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,
-                                                                ctx->cell_begin, ctx->bounds, ctx->C, ctx->phi,
-                                                                ctx->M, ctx->mesh_force_w, mf); break;
+            case 8: TPM_MF(8); break;
+            case 16: TPM_MF(16); break;
+            case 32: TPM_MF(32); break;
+            default: TPM_MF(64); break;
         }
+#undef TPM_MF
         TPM_HIP(ctx, hipGetLastError());
     }
     if (ctx->n > 0 && blocks > 0) {
@@ -875,32 +877,33 @@ tpm_status tpm_batch_pp(tpm_batch* bt, const tpm_batch_layout& L, unsigned flags
     a.cell_begin = bt->cell_begin;
     a.tiles = bt->tiles;
     a.ntiles = bt->tile_off + L.ncell;
-    a.phi = (flags & TPM_SKIP_PM) ? nullptr : bt->phi;
+    a.mesh = nullptr;
     a.counters = bt->counters;
     a.queue = reinterpret_cast<int*>(bt->counters + 8);
     a.boxes = bt->d_boxes;
     a.flags = flags & (TPM_SKIP_PP | TPM_SKIP_PM | TPM_SKIP_KICK);
-    a.gmesh = nullptr;
-    if (a.phi && L.mfw > 0) {
+    if (!(flags & TPM_SKIP_PM)) {
         // the mesh force of every box's active sinks, before the pair kernel
         const size_t lds = sizeof(float) * (size_t)L.mfw * L.mfw * L.mfw;
         const unsigned nc = (unsigned)L.ncell;
+#define TPM_BMF(TSV)                                                                                              \
+    if (L.mfw > 0)                                                                                              \
+        bmesh_force_kernel<TSV, false><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,             \
+                                                                     bt->cell_begin, L.nbox, bt->d_boxes,       \
+                                                                     bt->phi, L.mfw, bt->mesh_force);           \
+    else                                                                                                        \
+        bmesh_force_kernel<TSV, true><<<nc, 256, 0, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,                \
+                                                                  bt->cell_begin, L.nbox, bt->d_boxes, bt->phi, \
+                                                                  0, bt->mesh_force)
         switch (bt->tune.tile_sinks) {
-            case 8: bmesh_force_kernel<8><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,
-                                                                        bt->cell_begin, L.nbox, bt->d_boxes,
-                                                                        bt->phi, L.mfw, bt->mesh_force); break;
-            case 16: bmesh_force_kernel<16><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,
-                                                                          bt->cell_begin, L.nbox, bt->d_boxes,
-                                                                          bt->phi, L.mfw, bt->mesh_force); break;
-            case 32: bmesh_force_kernel<32><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,
-                                                                          bt->cell_begin, L.nbox, bt->d_boxes,
-                                                                          bt->phi, L.mfw, bt->mesh_force); break;
-            default: bmesh_force_kernel<64><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,
-                                                                          bt->cell_begin, L.nbox, bt->d_boxes,
-                                                                          bt->phi, L.mfw, bt->mesh_force); break;
+            case 8: TPM_BMF(8); break;
+            case 16: TPM_BMF(16); break;
+            case 32: TPM_BMF(32); break;
+            default: TPM_BMF(64); break;
         }
+#undef TPM_BMF
         TPM_HIP(bt, hipGetLastError());
-        a.gmesh = bt->mesh_force;
+        a.mesh = bt->mesh_force;
     }
     const bool persistent = bt->tune.sched == 1;
     int64_t grid = persistent ? bt->persist_grid : (L.tiles_max + 7) / 8 * 8;
