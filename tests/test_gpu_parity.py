@@ -203,6 +203,38 @@ def test_pp_on_a_larger_chain_mesh(gpu_ctx):
     assert rel_rms(V, np.stack(ov, 1)[same]) < 1e-5
 
 
+def test_mesh_term_with_and_without_the_lds_cube(gpu_ctx):
+    """The mesh force pass reads a chain cell's potential neighbourhood into
+    LDS when it fits 32 KiB (chain_dim 11 at a 64^3 mesh) and the potential
+    directly otherwise (chain_dim 3): the same arithmetic, so the mesh term of
+    every particle is the same bits either way, and matches the oracle's
+    6-point interpolation."""
+    n_side, M = 24, 64
+    n = n_side ** 3
+    x, y, z = clustered(n, 61)
+    out = {}
+    for C in (11, 3):
+        p = T.baseline_params(n_side, M, 3)
+        p.chain_dim = C
+        p.rcut = min(p.rcut, 0.99 / C)
+        p = gpu_ctx.set_params(p)
+        gpu_ctx.upload(x, y, z)
+        gpu_ctx.chainmesh_create()
+        gpu_ctx.gravity_long_compute()
+        gpu_ctx.kick(flags=T.TPM_SKIP_PP | T.TPM_SKIP_KICK)
+        d = gpu_ctx.download()
+        perm = gpu_ctx.chainmesh_perm()  # perm[new] = old
+        out[C] = {k: np.empty_like(d[k]) for k in ("gx", "gy", "gz", "pot")}
+        for k in out[C]:
+            out[C][k][perm] = d[k]
+        phi = gpu_ctx.get_phi().reshape(M, M, M)
+    for k in ("gx", "gy", "gz", "pot"):
+        assert np.array_equal(out[11][k], out[3][k]), k
+    ogx, ogy, ogz, _ = O.interp6(M, phi.ravel(), x, y, z)
+    G = np.stack([out[3]["gx"], out[3]["gy"], out[3]["gz"]], 1)
+    assert rel_rms(G, np.stack([ogx, ogy, ogz], 1) * p.GM) < 1e-5
+
+
 @pytest.mark.parametrize("order", [2, 3])
 def test_full_kick_vs_oracle(gpu_ctx, order):
     n_side = 32
