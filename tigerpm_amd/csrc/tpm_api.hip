@@ -57,7 +57,7 @@ extern "C" tpm_status tpm_ctx_destroy(tpm_ctx* ctx) {
                     ctx->buckets, ctx->h2d, ctx->h2d2, ctx->ref_perm, ctx->ref_cid, ctx->ref_cid2, ctx->ref_idx2,
                     ctx->ref_posa, ctx->ref_posb, ctx->ref_inv, ctx->ref_flag, ctx->ref_scan, ctx->ref_scan_tmp,
                     ctx->ref_tab, ctx->moment_buf, ctx->box_tiles, ctx->box_count, ctx->box_buckets, ctx->tree_moved,
-                    ctx->gsink, ctx->gsrc, ctx->gmesh, ctx->mesh_force};
+                    ctx->gsink, ctx->gsrc, ctx->gmesh, ctx->mesh_force, ctx->green_t};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : ctx->ev)

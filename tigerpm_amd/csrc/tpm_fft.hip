@@ -25,6 +25,7 @@
 // holds T transforms with element (t, idx) at idx*T + (t ^ (idx % T)): rows
 // of T consecutive t are whole bank rows, and the xor spreads a single
 // transform's consecutive elements over the banks when rows are loaded.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -361,12 +362,14 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_x_kernel(const float2* __rest
 
 // X, Green's multiply and X' in one pass, in place on B (the PM's Poisson
 // solve): the rows B[kz0 + t][ky][*] go forward along x, each mode (kx, ky,
-// kz0 + t) is multiplied by g (layout of S), and the rows go back.  The same
-// float operations as X, a Green's pass and X', without storing S.
+// kz0 + t) is multiplied by its Green's factor, and the rows go back.  The
+// same float operations as X, a Green's pass and X', without storing S.
+// gt is the table regrouped by green_tiles_kernel: workgroup w reads the
+// T * M factors gt[w * T * M ...] in one contiguous run.
 // Grid M * ceil(H / T).
 template <int M, int T>
 __global__ __launch_bounds__(FFT_THREADS) void fft_xgx_kernel(float2* __restrict__ b, const float2* __restrict__ tf,
-                                                             const float* __restrict__ g) {
+                                                             const float* __restrict__ gt) {
     constexpr int H = M / 2 + 1;
     constexpr int NKT = (H + T - 1) / T;
     constexpr int NU = T * M / FFT_THREADS;
@@ -377,10 +380,7 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_xgx_kernel(float2* __restrict
     const int tn = min(T, H - kz0);
     float gv[NU];
 #pragma unroll
-    for (int u = 0; u < NU; u++) {
-        const int e = (int)threadIdx.x + FFT_THREADS * u;
-        gv[u] = e % T < tn ? g[((int64_t)(e / T) * M + ky) * H + kz0 + e % T] : 0.f;
-    }
+    for (int u = 0; u < NU; u++) gv[u] = gt[(int64_t)blockIdx.x * (T * M) + threadIdx.x + FFT_THREADS * u];
     move_elems<T * M>(
         [&](int e) {
             return e / M < tn ? b[((int64_t)(kz0 + e / M) * M + ky) * M + e % M] : make_float2(0.f, 0.f);
@@ -408,6 +408,45 @@ __global__ __launch_bounds__(FFT_THREADS) void fft_xgx_kernel(float2* __restrict
 template <int M>
 constexpr int tile_of() {
     return M * TPM_FFT_TILE <= 8192 ? TPM_FFT_TILE : 8192 / M;  // T * M complex <= 64 KiB of LDS
+}
+
+// Rows per workgroup of the fused x pass.  Its loads and stores are whole
+// rows of B whatever T is, so T only sets the LDS per workgroup: with 4 rows
+// (16 KiB at M = 512) eight workgroups share a CU, against two at 16 rows
+// (64 KiB), and the pass takes 0.29 instead of 0.69 ms at M = 512 (8 rows:
+// 0.30 ms).
+#ifndef TPM_FFT_XGX_TILE
+#define TPM_FFT_XGX_TILE 4
+#endif
+template <int M>
+constexpr int xgx_tile_of() {
+    constexpr int t = M * TPM_FFT_XGX_TILE <= 8192 ? TPM_FFT_XGX_TILE : 8192 / M;
+    return t * M < FFT_THREADS ? FFT_THREADS / M : t;
+}
+
+// The Green's table (layout of S: [kx][ky][kz], kz < H) regrouped for
+// fft_xgx_kernel: the factors of its workgroup w = ky * NKT + kz tile, in the
+// order e = kx * T + t its threads use, zeros past kz = M/2.  The values are
+// copied, not recomputed.
+template <int M>
+__global__ void green_tiles_kernel(const float* __restrict__ g, float* __restrict__ gt) {
+    constexpr int T = xgx_tile_of<M>(), H = M / 2 + 1, NKT = (H + T - 1) / T;
+    const int64_t total = (int64_t)M * NKT * T * M;
+    for (int64_t l = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; l < total; l += (int64_t)gridDim.x * blockDim.x) {
+        const int e = (int)(l % (T * M));
+        const int64_t w = l / (T * M);
+        const int kz = (int)(w % NKT) * T + e % T;
+        const int64_t ky = w / NKT;
+        gt[l] = kz < H ? g[((int64_t)(e / T) * M + ky) * H + kz] : 0.f;
+    }
+}
+
+template <int M>
+int64_t green_tiles(hipStream_t st, const float* g, float* gt) {
+    constexpr int T = xgx_tile_of<M>(), H = M / 2 + 1;
+    const int64_t n = (int64_t)M * ((H + T - 1) / T) * T * M;
+    if (gt) green_tiles_kernel<M><<<(unsigned)std::min<int64_t>((n + 255) / 256, 65536), 256, 0, st>>>(g, gt);
+    return n;
 }
 
 template <int M>
@@ -438,7 +477,8 @@ void launch_pm(hipStream_t st, const unsigned long long* q, float* phi, float2* 
     constexpr int T = tile_of<M>(), H = M / 2 + 1;
     fft_z_r2c_kernel<M, T, true><<<M * (M / T), FFT_THREADS, 0, st>>>(q, spec, tf);
     fft_y_kernel<M, T, true><<<H * (M / T), FFT_THREADS, 0, st>>>(spec, work, tf);
-    fft_xgx_kernel<M, T><<<M * ((H + T - 1) / T), FFT_THREADS, 0, st>>>(work, tf, green);
+    constexpr int TX = xgx_tile_of<M>();
+    fft_xgx_kernel<M, TX><<<M * ((H + TX - 1) / TX), FFT_THREADS, 0, st>>>(work, tf, green);
     fft_y_kernel<M, T, false><<<H * (M / T), FFT_THREADS, 0, st>>>(work, spec, tf);
     fft_z_c2r_kernel<M, T><<<M * (M / T), FFT_THREADS, 0, st>>>(spec, phi, tf);
 }
@@ -486,6 +526,24 @@ tpm_status tpm_lds_fft_c2r(std::string& err, hipStream_t st, int M, float2* in, 
         case 1024: launch_c2r<1024>(st, in, out, work, tf); break;
         default: err = "LDS FFT: unsupported size " + std::to_string(M); return TPM_ERR_FFT;
     }
+    TPM_HIPE(err, hipGetLastError());
+    return TPM_OK;
+}
+
+tpm_status tpm_lds_green_tiles(std::string& err, hipStream_t st, int M, const float* green, float* gt,
+                               int64_t* count) {
+    int64_t n = 0;
+    switch (M) {
+        case 16: n = green_tiles<16>(st, green, gt); break;
+        case 32: n = green_tiles<32>(st, green, gt); break;
+        case 64: n = green_tiles<64>(st, green, gt); break;
+        case 128: n = green_tiles<128>(st, green, gt); break;
+        case 256: n = green_tiles<256>(st, green, gt); break;
+        case 512: n = green_tiles<512>(st, green, gt); break;
+        case 1024: n = green_tiles<1024>(st, green, gt); break;
+        default: err = "LDS FFT: unsupported size " + std::to_string(M); return TPM_ERR_FFT;
+    }
+    if (count) *count = n;
     TPM_HIPE(err, hipGetLastError());
     return TPM_OK;
 }

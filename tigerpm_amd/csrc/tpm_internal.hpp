@@ -73,11 +73,16 @@ tpm_status tpm_lds_fft_r2c(std::string& err, hipStream_t st, int M, const void* 
                            float2* work, const float2* tf, const float* green);
 tpm_status tpm_lds_fft_c2r(std::string& err, hipStream_t st, int M, float2* in, float* out, float2* work,
                            const float2* tf);
+// The Green's table regrouped into the fused x pass's workgroup tiles: with
+// gt null only *count (floats) is set, else gt[0 .. *count) is written.
+tpm_status tpm_lds_green_tiles(std::string& err, hipStream_t st, int M, const float* green, float* gt,
+                               int64_t* count);
 // The PM's Poisson solve on the LDS-tiled transform: Q32 density q ->
-// potential phi (Green's table green applied between the transforms); spec
-// and work are M * M * (M/2 + 1) complex scratch.
+// potential phi (Green's factors green_tiles, from tpm_lds_green_tiles,
+// applied between the transforms); spec and work are M * M * (M/2 + 1)
+// complex scratch.
 tpm_status tpm_lds_pm_solve(std::string& err, hipStream_t st, int M, const unsigned long long* q, float* phi,
-                            float2* spec, float2* work, const float2* tf, const float* green);
+                            float2* spec, float2* work, const float2* tf, const float* green_tiles);
 
 // Per-box constants of the fused PP + interpolation + kick kernel, and the
 // box's place in the batched arenas.  One entry per box: the single-box path
@@ -224,6 +229,9 @@ struct tpm_ctx {
     float* green = nullptr;
     int green_M = 0, green_order = 0;
     double green_rs = 0.0;
+    float* green_t = nullptr;             // green regrouped for the fused x pass
+    int64_t green_t_cap = 0;
+    bool green_t_valid = false;
     tpm_fft fft;
     bool pm_valid = false;
 

@@ -565,6 +565,7 @@ tpm_status tpm_ensure_mesh(tpm_ctx* ctx, int M) {
         TPM_HIP(ctx, hipMalloc(&ctx->green, sizeof(float) * MC));
         ctx->mesh_cap = M3;
         ctx->green_M = 0;
+        ctx->green_t_valid = false;
     }
     return tpm_fft_plan(ctx, M);
 }
@@ -662,14 +663,29 @@ tpm_status tpm_pm_solve(tpm_ctx* ctx) {
         ctx->green_M = M;
         ctx->green_rs = rs_eff;
         ctx->green_order = ctx->p.order;
+        ctx->green_t_valid = false;
     }
     if (ctx->fft.plans->tw && !ctx->p.interlace) {
         // LDS-tiled transform: the Q32 conversion inside its first pass, the
         // Green's multiply between the forward and inverse x transforms of
-        // one pass (same arithmetic as the separate kernels)
+        // one pass (same arithmetic as the separate kernels), its factors
+        // read from a copy of the table grouped by that pass's workgroups
+        if (!ctx->green_t_valid) {
+            int64_t n = 0;
+            TPM_TRY(tpm_lds_green_tiles(ctx->err, s, M, nullptr, nullptr, &n));
+            if (n > ctx->green_t_cap) {
+                if (ctx->green_t) TPM_HIP(ctx, hipFree(ctx->green_t));
+                ctx->green_t = nullptr;
+                ctx->green_t_cap = 0;
+                TPM_HIP(ctx, hipMalloc(&ctx->green_t, sizeof(float) * n));
+                ctx->green_t_cap = n;
+            }
+            TPM_TRY(tpm_lds_green_tiles(ctx->err, s, M, ctx->green, ctx->green_t, &n));
+            ctx->green_t_valid = true;
+        }
         TPM_TRY(deposit_density(ctx, 0, false));
         TPM_TRY(tpm_lds_pm_solve(ctx->err, s, M, ctx->rho_q, ctx->phi, ctx->rho_hat, (float2*)ctx->fft.work,
-                                 ctx->fft.plans->tw, ctx->green));
+                                 ctx->fft.plans->tw, ctx->green_t));
         ctx->pm_valid = true;
         return TPM_OK;
     }
