@@ -420,10 +420,38 @@ __device__ __forceinline__ void mesh_force_cell(float* sphi, const uint32_t* __r
                                                 float4* __restrict__ out) {
     constexpr int SPLIT = 64 / TS;
     const int W2 = W * W;
-    for (int l = (int)threadIdx.x; l < (GLOBAL ? 0 : W2 * W); l += (int)blockDim.x) {
-        // periodic; r0 >= -2, and W may exceed M on tiny meshes
-        const int gx = (r0[0] + l / W2 + M) % M, gy = (r0[1] + (l / W) % W + M) % M, gz = (r0[2] + l % W + M) % M;
-        sphi[l] = phi[((int64_t)gx * M + gy) * M + gz];
+    if (!GLOBAL && W <= M) {
+        // node l = (a W + b) W + c is r0 + (a, b, c), periodic (r0 >= -2, so
+        // every coordinate lies in [-2, 2M)); a thread's next node is
+        // blockDim.x further on, its (a, b, c) advanced by carries instead of
+        // divisions
+        const int d = (int)blockDim.x, dc = d % W, db = (d / W) % W, da = d / W2;
+        int l = (int)threadIdx.x, a = l / W2, b = (l / W) % W, c = l % W;
+        for (; l < W2 * W; l += d) {
+            int gx = r0[0] + a, gy = r0[1] + b, gz = r0[2] + c;
+            gx = gx < 0 ? gx + M : (gx >= M ? gx - M : gx);
+            gy = gy < 0 ? gy + M : (gy >= M ? gy - M : gy);
+            gz = gz < 0 ? gz + M : (gz >= M ? gz - M : gz);
+            sphi[l] = phi[((int64_t)gx * M + gy) * M + gz];
+            c += dc;
+            if (c >= W) {
+                c -= W;
+                b++;
+            }
+            b += db;
+            if (b >= W) {
+                b -= W;
+                a++;
+            }
+            a += da;
+        }
+    } else {
+        // W may exceed M on tiny meshes
+        for (int l = (int)threadIdx.x; l < (GLOBAL ? 0 : W2 * W); l += (int)blockDim.x) {
+            const int gx = (r0[0] + l / W2 + M) % M, gy = (r0[1] + (l / W) % W + M) % M,
+                      gz = (r0[2] + l % W + M) % M;
+            sphi[l] = phi[((int64_t)gx * M + gy) * M + gz];
+        }
     }
     __syncthreads();
     const float Mf = (float)M;
