@@ -639,11 +639,27 @@ __global__ void bscatter_kernel(int64_t n, const uint32_t* __restrict__ key, con
 // One wave per cell (grid-stride over the cells): the rank of each particle
 // of the cell by (Morton code, previous position); every (code, position)
 // pair is distinct, so the ranks are a permutation of the cell's slots.
+// Cells of up to BCO_CAP particles: the rank is the number of particles with
+// a smaller code (a histogram of the 64 codes and its scan) plus those of the
+// same code with a smaller position (each code's particles are gathered in
+// LDS and counted, a handful per code).  Larger cells (clustered boxes):
+// every particle is compared with every other.
+constexpr int BCO_CAP = 1024;
 __global__ __launch_bounds__(256) void bcell_order_kernel(int32_t ncell, const int32_t* __restrict__ cell_begin,
                                                           const uint32_t* __restrict__ tmp,
                                                           const uint32_t* __restrict__ key,
                                                           uint32_t* __restrict__ perm) {
+    __shared__ uint32_t s_fill[4][64], s_start[4][64], s_item[4][BCO_CAP];
     const int lane = threadIdx.x & 63;
+    const int wl = (int)(threadIdx.x >> 6);
+    uint32_t* const fill = s_fill[wl];
+    uint32_t* const first = s_start[wl];
+    uint32_t* const item = s_item[wl];
+    // the LDS words of one wave are written and read by its own lanes
+    const auto wsync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    };
     const int wave = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
     const int nwave = (int)(((int64_t)gridDim.x * blockDim.x) >> 6);
     for (int c = wave; c < ncell; c += nwave) {
@@ -651,6 +667,41 @@ __global__ __launch_bounds__(256) void bcell_order_kernel(int32_t ncell, const i
         const int n = cell_begin[c + 1] - b;
         if (n == 1 && lane == 0) perm[b] = tmp[b];
         if (n <= 1) continue;
+        if (n <= BCO_CAP) {
+            fill[lane] = 0u;
+            wsync();
+            for (int g = lane; g < n; g += 64) atomicAdd(&fill[key[tmp[b + g]] & 63u], 1u);
+            wsync();
+            // exclusive scan of the code counts, lane l holding code l
+            const uint32_t h = fill[lane];
+            uint32_t x = h;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o);
+                if (lane >= o) x += y;
+            }
+            wsync();
+            first[lane] = x - h;
+            fill[lane] = x - h;
+            wsync();
+            // each code's particles into its range, in any order
+            for (int g = lane; g < n; g += 64) {
+                const uint32_t me = tmp[b + g];
+                item[atomicAdd(&fill[key[me] & 63u], 1u)] = me;
+            }
+            wsync();
+            // fill[code] is now the end of the code's range
+            for (int g = lane; g < n; g += 64) {
+                const uint32_t me = tmp[b + g];
+                const uint32_t code = key[me] & 63u;
+                const int s0 = (int)first[code], s1 = (int)fill[code];
+                int rank = s0;
+                for (int k = s0; k < s1; k++) rank += item[k] < me ? 1 : 0;
+                perm[b + rank] = me;
+            }
+            wsync();
+            continue;
+        }
         for (int g = 0; g < n; g += 64) {
             const bool valid = g + lane < n;
             const uint32_t me = valid ? tmp[b + g + lane] : 0u;
