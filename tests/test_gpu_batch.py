@@ -127,6 +127,44 @@ def test_sparse_box_in_a_batch():
                 c.close()
 
 
+def test_crowded_cells_in_a_batch():
+    """A box with one chain cell of 2500 particles and one of 700 (chain_dim
+    8, 800 more spread over the box) beside a Zel'dovich box: the batch's
+    in-cell order counts Morton codes for cells up to 1024 particles and
+    compares every pair above that; both give the single-box sort's order,
+    so each box equals its plain kick bit for bit over two steps."""
+    rng = np.random.default_rng(17)
+
+    def cube(n, lo, hi):
+        return [(rng.uniform(lo, hi, n) * 2.0 ** 32).astype(np.uint32) for _ in range(3)]
+
+    parts = [cube(2500, 0.30, 0.36), cube(700, 0.55, 0.60), cube(800, 0.0, 1.0)]
+    x, y, z = [np.concatenate([p[d] for p in parts]) for d in range(3)]
+    n = x.size
+    vx, vy, vz = [rng.standard_normal(n).astype(np.float32) * 1e-3 for _ in range(3)]
+    rung = rng.integers(0, 2, n).astype(np.int8)
+    crowded = T.box_spec(T.baseline_params(48, 48, 3, n_total=n, interlace=1), t0=T0)
+    with new_batch() as bt:
+        bt.set_tuning(tile_sinks=16)
+        ids = [bt.add(crowded, x, y, z, vx, vy, vz, rung), bt.add_zeldovich(spec(24), 24, 6, 1.2)]
+        ctxs = []
+        for i in ids:
+            c = T.Context(0)
+            bt.to_ctx(i, c)
+            c.set_tuning(tile_sinks=16)
+            ctxs.append(c)
+        assert ctxs[0].params.chain_dim == 8
+        for k in range(2):
+            bt.step(0)
+            for i, c in zip(ids, ctxs):
+                c.kick_treepm(min_rung=0, scale=1.0, t0=T0, first_call=(k == 0))
+                assert_same(bt.download(i), c.download(), "box %d step %d" % (i, k))
+        cells = np.diff(ctxs[0].chainmesh_cells())
+        assert cells.max() > 1024 and ((cells > 64) & (cells <= 1024)).any()
+        for c in ctxs:
+            c.close()
+
+
 def test_batch_drift_matches_oracle():
     """Kick skipped, drift on: positions move by round(v t0 2^32) exactly, in
     the chain-mesh order of the positions before the drift."""
