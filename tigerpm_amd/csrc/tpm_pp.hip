@@ -514,8 +514,17 @@ __global__ __launch_bounds__(256) void mesh_force_kernel(const uint32_t* __restr
                                                         const float* __restrict__ phi, int M, int W,
                                                         int3 lo, int3 hi, float4* __restrict__ out) {
     extern __shared__ float sphi[];  // W^3, z fastest
-    const int c = (int)blockIdx.x;
-    const int k3[3] = {c % C, (c / C) % C, c / (C * C)};
+    // Each of the 8 XCDs (workgroup index mod 8) takes one contiguous run of
+    // cells in an order with z fastest, the potential's fastest axis, so
+    // that the workgroups running together on one XCD read neighbouring
+    // cubes through its L2 (in cell-id order, x fastest, the cubes of
+    // consecutive workgroups lie a megabyte apart at 512^3 and every cube
+    // came from HBM: 20.7 GB per launch).
+    const int nc = C * C * C, per = (nc + 7) >> 3;
+    const int rk = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    if (rk >= nc) return;
+    const int k3[3] = {rk / (C * C), (rk / C) % C, rk % C};
+    const int c = (k3[2] * C + k3[1]) * C + k3[0];
     if (k3[0] < lo.x || k3[0] >= hi.x || k3[1] < lo.y || k3[1] >= hi.y || k3[2] < lo.z || k3[2] >= hi.z) return;
     const int cb = cell_begin[c], ce = cell_begin[c + 1];
     if (cb == ce) return;
@@ -827,7 +836,7 @@ tpm_status tpm_pp_kick(tpm_ctx* ctx, int min_rung, double scale, double t0, int 
     if (gmesh) {
         const int W = ctx->mesh_force_w;
         const size_t lds = sizeof(float) * (size_t)W * W * W;
-        const unsigned nc = (unsigned)ncell;
+        const unsigned nc = (unsigned)(8 * ((ncell + 7) / 8));  // 8 XCD runs
 #define TPM_MF(TSV)                                                                                               \
     if (W > 0)                                                                                                  \
         mesh_force_kernel<TSV, false><<<nc, 256, lds, s>>>(ctx->x, ctx->y, ctx->z, ctx->rung, min_rung,         \
