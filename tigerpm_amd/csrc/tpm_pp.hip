@@ -543,16 +543,30 @@ __global__ __launch_bounds__(256) void bmesh_force_kernel(const uint32_t* __rest
                                                          const int8_t* __restrict__ rung,
                                                          const int32_t* __restrict__ cell_begin, int nbox,
                                                          const tpm_box_dev* __restrict__ boxes,
-                                                         const float* __restrict__ phi, int W,
+                                                         const float* __restrict__ phi, int W, int ncell,
                                                          float4* __restrict__ out) {
     extern __shared__ float sphi[];
-    const int c = (int)blockIdx.x;
+    // per-XCD runs of the global cell space, each box's cells with z
+    // fastest, as in mesh_force_kernel
+    const int per = (ncell + 7) >> 3;
+    const int rk = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    if (rk >= ncell) return;
+    const int b = tpm_upper_index(nbox, rk, [&](int k) { return (int64_t)boxes[k].cell0; });
+    const tpm_box_dev& B = boxes[b];
+    const int C = B.C, M = B.M, rl = rk - B.cell0;
+    int k3[3];
+    if (rl < C * C * C) {
+        k3[0] = rl / (C * C);
+        k3[1] = (rl / C) % C;
+        k3[2] = rl % C;
+    } else {
+        k3[0] = rl % C;
+        k3[1] = (rl / C) % C;
+        k3[2] = rl / (C * C);
+    }
+    const int c = B.cell0 + (k3[2] * C + k3[1]) * C + k3[0];
     const int cb = cell_begin[c], ce = cell_begin[c + 1];
     if (cb == ce) return;
-    const int b = tpm_upper_index(nbox, c, [&](int k) { return (int64_t)boxes[k].cell0; });
-    const tpm_box_dev& B = boxes[b];
-    const int C = B.C, M = B.M, l = c - B.cell0;
-    const int k3[3] = {l % C, (l / C) % C, l / (C * C)};
     int r0[3];
 #pragma unroll
     for (int d = 0; d < 3; d++)
@@ -922,16 +936,17 @@ tpm_status tpm_batch_pp(tpm_batch* bt, const tpm_batch_layout& L, unsigned flags
     if (!(flags & TPM_SKIP_PM)) {
         // the mesh force of every box's active sinks, before the pair kernel
         const size_t lds = sizeof(float) * (size_t)L.mfw * L.mfw * L.mfw;
-        const unsigned nc = (unsigned)L.ncell;
+        const unsigned nc = (unsigned)(8 * ((L.ncell + 7) / 8));  // 8 XCD runs
 #define TPM_BMF(TSV)                                                                                              \
     if (L.mfw > 0)                                                                                              \
         bmesh_force_kernel<TSV, false><<<nc, 256, lds, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,             \
                                                                      bt->cell_begin, L.nbox, bt->d_boxes,       \
-                                                                     bt->phi, L.mfw, bt->mesh_force);           \
+                                                                     bt->phi, L.mfw, (int)L.ncell,              \
+                                                                     bt->mesh_force);                           \
     else                                                                                                        \
         bmesh_force_kernel<TSV, true><<<nc, 256, 0, bt->stream>>>(bt->x, bt->y, bt->z, bt->rung,                \
                                                                   bt->cell_begin, L.nbox, bt->d_boxes, bt->phi, \
-                                                                  0, bt->mesh_force)
+                                                                  0, (int)L.ncell, bt->mesh_force)
         switch (bt->tune.tile_sinks) {
             case 8: TPM_BMF(8); break;
             case 16: TPM_BMF(16); break;
